@@ -1,0 +1,184 @@
+#!/usr/bin/env python3
+"""Benchmark: receivers + lakeflow + accumulation on the 4096^2 depressions
+workload (periodic east-west, 2000 Gaussian pits, dx=10 m), one MI355X.
+
+One step = compute_receivers -> reroute_flow(carve=True) ->
+accumulate_constant_Q(1e-5, area=True) over the whole grid, with the DEM
+already resident in HBM.  Prints one JSON line (see the keys below).
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 4096] [--no-cpu]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = "Mcells/s route+accumulate at 4096² DEM; graphflood s-to-steady warm vs cold"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--size", type=int, default=4096)
+    p.add_argument("--pits", type=int, default=2000)
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and full-size parity check")
+    return p.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the routing path runs on a single MI355X here "
+              "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import pyfastflow_amd as pf
+    from pyfastflow_amd import _lib, synth
+
+    n = args.size
+    pits = args.pits if n == 4096 else int(round(args.pits * n * n / 4096.0 ** 2))
+    log(f"[bench] generating {n}x{n} depressions DEM ({pits} pits)")
+    c = synth.depressions(n=n, n_pits=pits)
+    nx, ny, dx = c["nx"], c["ny"], c["dx"]
+    N = nx * ny
+    router = pf.flow.FlowRouter(nx, ny, dx, boundary_mode=c["boundary_mode"])
+    router.set_z(c["z"])
+    torch.cuda.synchronize()
+
+    def step():
+        router.compute_receivers()
+        router.reroute_flow(carve=True)
+        router.accumulate_constant_Q(c["precipitation"], area=True)
+
+    log(f"[bench] warmup {args.warmup}")
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    # ---- timed region ---------------------------------------------------------
+    s = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(s)
+    for _ in range(args.steps):
+        step()
+    e1.record(s)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = e0.elapsed_time(e1) / args.steps
+    log(f"[bench] {args.steps} steps: {ms_step:.3f} ms/step (event), {1e3 * wall / args.steps:.3f} ms/step (wall)")
+    st = router.last_lake_status
+    lake = dict(ndep0=int(st.ndep0), trips=int(st.n_iter), ndep=[int(st.ndep[i]) for i in range(min(st.n_loop, 64))])
+    log(f"[bench] lakeflow: {lake}")
+
+    # ---- per-stage times ------------------------------------------------------
+    def time_stage(fn, reps=5):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        torch.cuda.synchronize()
+        ev[0].record(s)
+        for _ in range(reps):
+            fn()
+        ev[1].record(s)
+        torch.cuda.synchronize()
+        return ev[0].elapsed_time(ev[1]) / reps
+
+    z_copy = router.z.data.clone()
+    t_rcv = time_stage(router.compute_receivers, 20)
+    rec_after = router.receivers.data.clone()
+
+    def lake_only():
+        router.compute_receivers()
+        router.reroute_flow(carve=True)
+
+    t_lake = time_stage(lake_only, 3) - t_rcv
+    t_acc = time_stage(lambda: router.accumulate_constant_Q(c["precipitation"], area=True), 5)
+    stages = {"receivers_ms": round(t_rcv, 4), "lakeflow_ms": round(t_lake, 4), "accumulate_ms": round(t_acc, 4)}
+    log(f"[bench] stages: {stages}")
+
+    # ---- roofline of the receivers kernel (12 B/cell: z in, rcv + gradient out)
+    bytes_rcv = 12.0 * N
+    achieved = bytes_rcv / (t_rcv * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_receivers",
+                "bytes_per_launch": bytes_rcv}
+
+    # ---- CPU baseline (oracle, the reference algorithm restated in C) and
+    #      full-size parity of the GPU result against it ----------------------
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        sys.path.insert(0, HERE)
+        from oracle import pff_oracle as O
+
+        g = O.Grid(nx, ny, dx, c["boundary_mode"])
+        log(f"[bench] CPU baseline: oracle with {O.num_threads()} threads on the full {n}x{n} workload")
+        t0 = time.perf_counter()
+        rcv0, _ = O.compute_receivers(g, c["z"])
+        lk = O.reroute_flow(g, c["z"], rcv0, carve=True)
+        Qc, done = O.accumulate_constant_Q(g, lk["rec"], c["precipitation"], True)
+        tc = time.perf_counter() - t0
+        cpu = {"value": round(N / tc / 1e6, 3), "unit": "Mcells/s", "cores": O.num_threads(), "kind": "port",
+               "sample": f"one full {n}x{n} step (receivers, reroute_flow carve, accumulate_constant_Q with the "
+                         f"reference's rake-and-compress schedule) in {tc:.2f} s"}
+        log(f"[bench] CPU baseline {cpu}")
+        router.z.data.copy_(z_copy)
+        router.compute_receivers()
+        router.reroute_flow(carve=True)
+        router.accumulate_constant_Q(c["precipitation"], area=True)
+        torch.cuda.synchronize()
+        rg = router.receivers.to_numpy()
+        Qg = router.Q.to_numpy().astype(np.float64)
+        cnt = O.upstream_count(g, lk["rec"])
+        q0 = float(np.float32(c["precipitation"] * dx * dx))
+        ref64 = cnt.astype(np.float64) * q0
+        rel = np.abs(Qg - ref64) / ref64
+        parity = {"receivers_bit_exact": bool(np.array_equal(rg, lk["rec"])),
+                  "Q_max_rel_err_vs_f64": float(rel.max()),
+                  "Q_vs_reference_schedule_max_rel": float((np.abs(Qg - Qc) / np.abs(Qc)).max()),
+                  "reference_schedule_converged": bool(done)}
+        log(f"[bench] parity {parity}")
+
+    out = {
+        "metric": METRIC,
+        "value": round(N / (ms_step * 1e-3) / 1e6, 3),
+        "unit": "Mcells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"depressions {n}x{n} periodic_EW dx=10 m, {pits} Gaussian pits, spectral beta=2.0 "
+                               f"relief 1000 m seed 202; receivers + reroute_flow(carve) + accumulate_constant_Q(1e-5)",
+                   "cells": N, "stages": stages, "lakeflow": lake, "parity": parity},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

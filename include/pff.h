@@ -1,0 +1,164 @@
+/*
+ * pff.h -- C ABI of libpff_hip.so, the MI355X (gfx950) implementation of
+ * PyFastFlow's flow-routing and GraphFlood hot path.
+ *
+ * The reference (bgailleton/pyfastflow) has no FFI: its boundary is a Python
+ * call surface over Taichi fields (pyfastflow.flow, pyfastflow.graphflood).
+ * Each entry point below replaces one reference kernel or driver; the
+ * reference file:line is given at each declaration.  The Python drop-in
+ * (pyfastflow_amd) binds these with ctypes.
+ *
+ * Conventions
+ *  - Every array argument is a DEVICE pointer (HBM), row-major 1-D of length
+ *    N = nx*ny unless stated, owned by the caller.  The library owns only the
+ *    scratch held in a pff_ctx.
+ *  - Calls enqueue on the context's HIP stream and return without waiting,
+ *    except where a result is returned through a host pointer (stated).
+ *  - Every call returns 0 on success or a negative PFF_E* code;
+ *    pff_last_error() gives the message (per host thread).
+ *  - Grids need nx >= 2, ny >= 2 and nx*ny < 2^31.
+ */
+#ifndef PFF_H
+#define PFF_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PFF_OK 0
+#define PFF_EINVAL -1    /* bad argument (shape, null pointer, mode) */
+#define PFF_EHIP -2      /* HIP runtime error */
+#define PFF_ENOMEM -3    /* device allocation failed */
+#define PFF_ENOTCONV -4  /* an iteration cap was reached before convergence */
+
+/* Grid description; mirrors the globals constants.NX/NY/DX/BOUND_MODE and the
+   custom boundary field (constants.py:27-83). */
+typedef struct {
+  int32_t nx, ny;
+  float dx;
+  int32_t bound_mode;          /* 0 normal, 1 periodic_EW, 2 periodic_NS, 3 custom */
+  const uint8_t* boundaries;   /* device, N codes 0/1/3/7/9, mode 3 only, else NULL */
+} pff_grid;
+
+/* Hydraulic parameters; mirror constants.PREC/MANNING/EDGESW/DT_HYDRO
+   (constants.py:93-103) as set by Flooder (gf_fields.py:62-108). */
+typedef struct {
+  double prec, manning, edge_slope, dt;
+} pff_hydro;
+
+typedef struct pff_ctx pff_ctx;
+
+const char* pff_version(void);
+const char* pff_last_error(void);
+
+/* Context: device, stream and scratch.  No reference counterpart (the
+   reference keeps scratch as FlowRouter fields, flowfields.py:66-159). */
+int pff_ctx_create(pff_ctx** out, int32_t device, void* hip_stream);
+int pff_ctx_destroy(pff_ctx* ctx);
+int pff_ctx_set_stream(pff_ctx* ctx, void* hip_stream);
+int pff_ctx_synchronize(pff_ctx* ctx);
+
+/* ---- flow: receivers -------------------------------------------------- */
+/* receivers.py:15-58  compute_receivers(z, receivers, gradient) */
+int pff_compute_receivers(pff_ctx* ctx, const pff_grid* g, const float* z, int32_t* rcv, float* gradient);
+
+/* ---- flow: accumulation ------------------------------------------------ */
+/* downstream_propag.py:77-96  rcv2donor(rcv, dnr, ndnr); ndnr must be zeroed
+   by the caller as flowfields.py:275 does; dnr has 4*N slots */
+int pff_rcv2donor(pff_ctx* ctx, const pff_grid* g, const int32_t* rcv, int32_t* dnr, int32_t* ndnr);
+/* downstream_propag.py:99-179  rake_compress_accum(dnr, ndnr, p, src, dnr_, ndnr_, p_, iteration) */
+int pff_rake_compress_accum(pff_ctx* ctx, const pff_grid* g, int32_t* dnr, int32_t* ndnr, float* p, int32_t* src,
+                            int32_t* dnr_, int32_t* ndnr_, float* p_, int32_t iteration);
+/* downstream_propag.py:59-74  fuse(A, src, B, iteration) */
+int pff_fuse(pff_ctx* ctx, const pff_grid* g, float* A, const int32_t* src, const float* B, int32_t iteration);
+/* flowfields.py:254-292  FlowRouter.accumulate_constant_Q(value, area):
+   Q[i] = f32(value*dx^2 or value) * |upstream(i) + i|, computed as exact
+   integer upstream counts on the receiver forest.  *converged (host,
+   nullable) is 1 unless the receivers hold a cycle. */
+int pff_accumulate_constant_Q(pff_ctx* ctx, const pff_grid* g, const int32_t* rcv, double value, int32_t area,
+                              float* Q, int32_t* converged);
+/* Upstream cell counts (including self) as uint32; the integer core of
+   accumulate_constant_Q. */
+int pff_upstream_count(pff_ctx* ctx, const pff_grid* g, const int32_t* rcv, uint32_t* count);
+
+/* ---- flow: lakeflow ---------------------------------------------------- */
+/* lakeflow.py:24-44  depression_counter(rec) -> int (host result; syncs) */
+int pff_depression_counter(pff_ctx* ctx, const pff_grid* g, const int32_t* rcv, int64_t* ndep);
+/* lakeflow.py:46-61  basin_id_init(bid) */
+int pff_basin_id_init(pff_ctx* ctx, const pff_grid* g, int32_t* bid);
+/* lakeflow.py:63-80  propagate_basin(bid, rec_): one synchronous pointer-jumping pass */
+int pff_propagate_basin(pff_ctx* ctx, const pff_grid* g, int32_t* bid, int32_t* rec_);
+/* lakeflow.py:82-101  basin_identification(bid, rec, rec_, N): rec_ <- rec,
+   init, ceil(log2 N)+1 passes; on return bid = init(root), rec_ = roots */
+int pff_basin_identification(pff_ctx* ctx, const pff_grid* g, int32_t* bid, const int32_t* rec, int32_t* rec_);
+/* lakeflow.py:105-281  saddlesort(bid, is_border, z_prime, basin_saddle,
+   basin_saddlenode, outlet, z); per-basin arrays have N entries */
+int pff_saddlesort(pff_ctx* ctx, const pff_grid* g, const int32_t* bid, uint8_t* is_border, float* z_prime,
+                   int64_t* basin_saddle, int32_t* basin_saddlenode, int64_t* outlet, const float* z);
+/* lakeflow.py:284-304  reroute_jump(rec, outlet) */
+int pff_reroute_jump(pff_ctx* ctx, const pff_grid* g, int32_t* rec, const int64_t* outlet);
+/* lakeflow.py:405-438  reroute_carve(rec, rec_, rec__, tag, tag_, saddlenode,
+   outlet, change): rec <- carved receivers of the working copy rec_ */
+int pff_reroute_carve(pff_ctx* ctx, const pff_grid* g, int32_t* rec, int32_t* rec_, int32_t* rec__, uint8_t* tag,
+                      uint8_t* tag_, const int32_t* saddlenode, const int64_t* outlet);
+
+/* Per-call statistics of pff_reroute_flow (host struct). */
+typedef struct {
+  int64_t ndep0;      /* depressions before rerouting (lakeflow.py:493) */
+  int32_t n_loop;     /* outer loop trips run */
+  int32_t n_iter;     /* trips that rerouted (ran saddlesort) */
+  int64_t ndep[64];   /* Ndep_bis of each trip (lakeflow.py:501) */
+  int32_t carve_iters[64]; /* tag-propagation rounds of each carve */
+} pff_lake_status;
+
+/* lakeflow.py:466-544  reroute_flow(bid, rec, rec_, rec__, z, z_prime,
+   is_border, outlet, basin_saddle, basin_saddlenode, tag, tag_, change, carve).
+   Arrays as in the reference; on return rec holds the rerouted receivers
+   (carve) and rec_ the working copy (the jump-mode result), bid is reset to
+   its initial labels (lakeflow.py:537).  Optional: bid_first (device, N)
+   receives the first trip's basin labels; st (host) the statistics.  Syncs
+   once per outer trip to read the depression count. */
+int pff_reroute_flow(pff_ctx* ctx, const pff_grid* g, int32_t* bid, int32_t* rec, int32_t* rec_, int32_t* rec__,
+                     const float* z, float* z_prime, uint8_t* is_border, int64_t* outlet, int64_t* basin_saddle,
+                     int32_t* basin_saddlenode, uint8_t* tag, uint8_t* tag_, int32_t carve, int32_t* bid_first,
+                     pff_lake_status* st);
+
+/* ---- flow: fill -------------------------------------------------------- */
+/* fill_topo.py:99-125  fill_z_add_delta(zh, h, z_, receivers, receivers_, receivers__, epsilon) */
+int pff_fill_z_add_delta(pff_ctx* ctx, const pff_grid* g, float* zh, float* h, float* z_, const int32_t* rcv,
+                         int32_t* rcv_, int32_t* rcv__, double epsilon);
+/* fill_topo.py:48-78  topofill(router, epsilon, custom_z): out = filled z
+   (out may alias z: FlowRouter.fill_z, flowfields.py:295) */
+int pff_topofill(pff_ctx* ctx, const pff_grid* g, const float* z, float* out, const int32_t* rcv, int32_t* rcv_,
+                 int32_t* rcv__, double epsilon);
+
+/* ---- graphflood -------------------------------------------------------- */
+/* gf_hydrodynamics.py:23-75  diffuse_Q_constant_prec(zh, Q, temp) */
+int pff_diffuse_Q_constant_prec(pff_ctx* ctx, const pff_grid* g, const pff_hydro* p, const float* zh, float* Q,
+                                float* temp);
+/* gf_hydrodynamics.py:78-129  graphflood_core_cte_mannings(h, zh, dh, rec, Q) */
+int pff_graphflood_core_cte_mannings(pff_ctx* ctx, const pff_grid* g, const pff_hydro* p, float* h, float* zh,
+                                     float* dh, const int32_t* rec, const float* Q);
+
+/* ---- packed keys ------------------------------------------------------- */
+/* f32_i32_struct.py:123-135  pack_full_float_index(arr, ft, it) */
+int pff_pack_full_float_index(pff_ctx* ctx, int64_t n, int64_t* arr, const float* ft, const int32_t* it);
+/* f32_i32_struct.py:105-121  unpack_full_float_index(arr, ft, it) */
+int pff_unpack_full_float_index(pff_ctx* ctx, int64_t n, const int64_t* arr, float* ft, int32_t* it);
+
+/* ---- grid debug maps --------------------------------------------------- */
+/* neighbourer_flat.py:112-115  fill_edges(edges): edge class 0..8 per cell (u8) */
+int pff_fill_edges(pff_ctx* ctx, const pff_grid* g, uint8_t* edges);
+/* neighbourer_flat.py:608-611  flow_out_nodes(outnodes): can_leave_domain per cell (u8 0/1) */
+int pff_flow_out_nodes(pff_ctx* ctx, const pff_grid* g, uint8_t* outnodes);
+
+/* ---- utilities --------------------------------------------------------- */
+/* util_taichi.py:18-35 swap_arrays / :39-44 init_arange (4-byte elements) */
+int pff_swap_arrays(pff_ctx* ctx, int64_t n, void* a, void* b);
+int pff_init_arange(pff_ctx* ctx, int64_t n, int32_t* a);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

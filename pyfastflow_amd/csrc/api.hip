@@ -1,0 +1,428 @@
+// api.hip -- the extern "C" entry points of libpff_hip.so (declared in
+// include/pff.h): argument checks, the context and its scratch, and the
+// drivers that sequence the kernels (accumulate_constant_Q, reroute_flow).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "pff_internal.hpp"
+
+namespace pff {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+int check_hip(hipError_t e, const char* what) {
+  if (e == hipSuccess) return 0;
+  return fail(PFF_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int check_grid(const pff_grid* g, GridD* out) {
+  if (!g) return fail(PFF_EINVAL, "grid is NULL");
+  if (g->nx < 2 || g->ny < 2) return fail(PFF_EINVAL, "grid needs nx >= 2 and ny >= 2");
+  const int64_t N = (int64_t)g->nx * (int64_t)g->ny;
+  if (N >= (int64_t)0x7FFFFFFF) return fail(PFF_EINVAL, "grid has 2^31 cells or more");
+  if (g->bound_mode < 0 || g->bound_mode > 3) return fail(PFF_EINVAL, "bound_mode must be 0..3");
+  if (g->bound_mode == 3 && !g->boundaries) return fail(PFF_EINVAL, "custom boundary mode needs a boundaries array");
+  if (!(g->dx > 0.0f)) return fail(PFF_EINVAL, "dx must be positive");
+  out->nx = g->nx;
+  out->ny = g->ny;
+  out->N = (int32_t)N;
+  out->dx = g->dx;
+  out->mode = g->bound_mode;
+  out->B = g->boundaries;
+  return 0;
+}
+
+int scratch(pff_ctx* ctx, int slot, size_t bytes, void** out, bool zero_new) {
+  if (bytes == 0) bytes = 4;
+  if (ctx->cap[slot] < bytes) {
+    if (ctx->buf[slot]) {
+      // the old buffer may still be read by queued kernels
+      hipStreamSynchronize(ctx->stream);
+      hipFree(ctx->buf[slot]);
+      ctx->buf[slot] = nullptr;
+      ctx->cap[slot] = 0;
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess || !p) {
+      hipGetLastError();
+      return fail(PFF_ENOMEM, "device scratch allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+    ctx->buf[slot] = p;
+    ctx->cap[slot] = bytes;
+    if (zero_new) PFF_TRY(check_hip(hipMemsetAsync(p, 0, bytes, ctx->stream), "memset"));
+  }
+  *out = ctx->buf[slot];
+  return 0;
+}
+
+static int begin(pff_ctx* ctx) {
+  if (!ctx) return fail(PFF_EINVAL, "context is NULL");
+  return check_hip(hipSetDevice(ctx->device), "hipSetDevice");
+}
+
+static int need(const void* p, const char* name) {
+  if (!p) return fail(PFF_EINVAL, std::string(name) + " is NULL");
+  return 0;
+}
+
+static int ceil_log2(int64_t n) {
+  int r = 0;
+  while ((1ll << r) < n) ++r;
+  return r;
+}
+
+}  // namespace pff
+
+using namespace pff;
+
+extern "C" {
+
+const char* pff_version(void) { return "pyfastflow_amd 0.1.0 (gfx950)"; }
+const char* pff_last_error(void) { return g_err.c_str(); }
+
+int pff_ctx_create(pff_ctx** out, int32_t device, void* hip_stream) {
+  if (!out) return fail(PFF_EINVAL, "out is NULL");
+  *out = nullptr;
+  PFF_TRY(check_hip(hipSetDevice(device), "hipSetDevice"));
+  pff_ctx* c = new pff_ctx();
+  c->device = device;
+  c->stream = (hipStream_t)hip_stream;
+  if (hipHostMalloc((void**)&c->host_words, W_NWORDS * sizeof(int64_t), hipHostMallocDefault) != hipSuccess) {
+    delete c;
+    return fail(PFF_ENOMEM, "pinned host words");
+  }
+  if (hipMalloc((void**)&c->dev_words, W_NWORDS * sizeof(int64_t)) != hipSuccess) {
+    hipHostFree(c->host_words);
+    delete c;
+    return fail(PFF_ENOMEM, "device words");
+  }
+  hipMemset(c->dev_words, 0, W_NWORDS * sizeof(int64_t));
+  *out = c;
+  return 0;
+}
+
+int pff_ctx_destroy(pff_ctx* ctx) {
+  if (!ctx) return 0;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  for (int s = 0; s < pff_ctx::kSlots; ++s)
+    if (ctx->buf[s]) hipFree(ctx->buf[s]);
+  if (ctx->dev_words) hipFree(ctx->dev_words);
+  if (ctx->host_words) hipHostFree(ctx->host_words);
+  delete ctx;
+  return 0;
+}
+
+int pff_ctx_set_stream(pff_ctx* ctx, void* hip_stream) {
+  if (!ctx) return fail(PFF_EINVAL, "context is NULL");
+  ctx->stream = (hipStream_t)hip_stream;
+  return 0;
+}
+
+int pff_ctx_synchronize(pff_ctx* ctx) {
+  PFF_TRY(begin(ctx));
+  return check_hip(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+}
+
+// ---- receivers ------------------------------------------------------------
+int pff_compute_receivers(pff_ctx* ctx, const pff_grid* g, const float* z, int32_t* rcv, float* gradient) {
+  PFF_TRY(begin(ctx));
+  GridD d;
+  PFF_TRY(check_grid(g, &d));
+  PFF_TRY(need(z, "z"));
+  PFF_TRY(need(rcv, "receivers"));
+  PFF_TRY(need(gradient, "gradient"));
+  return launch_receivers(ctx, d, z, rcv, gradient);
+}
+
+// ---- accumulation ---------------------------------------------------------
+int pff_rcv2donor(pff_ctx* ctx, const pff_grid* g, const int32_t* rcv, int32_t* dnr, int32_t* ndnr) {
+  PFF_TRY(begin(ctx));
+  GridD d;
+  PFF_TRY(check_grid(g, &d));
+  PFF_TRY(need(rcv, "rcv"));
+  PFF_TRY(need(dnr, "dnr"));
+  PFF_TRY(need(ndnr, "ndnr"));
+  return launch_rcv2donor(ctx, d, rcv, dnr, ndnr);
+}
+
+int pff_rake_compress_accum(pff_ctx* ctx, const pff_grid* g, int32_t* dnr, int32_t* ndnr, float* p, int32_t* src,
+                            int32_t* dnr_, int32_t* ndnr_, float* p_, int32_t iteration) {
+  PFF_TRY(begin(ctx));
+  GridD d;
+  PFF_TRY(check_grid(g, &d));
+  if (!dnr || !ndnr || !p || !src || !dnr_ || !ndnr_ || !p_) return fail(PFF_EINVAL, "rake_compress_accum: NULL array");
+  return launch_rake_compress(ctx, d, dnr, ndnr, p, src, dnr_, ndnr_, p_, iteration);
+}
+
+int pff_fuse(pff_ctx* ctx, const pff_grid* g, float* A, const int32_t* src, const float* B, int32_t iteration) {
+  PFF_TRY(begin(ctx));
+  GridD d;
+  PFF_TRY(check_grid(g, &d));
+  if (!A || !src || !B) return fail(PFF_EINVAL, "fuse: NULL array");
+  return launch_fuse(ctx, d, A, src, B, iteration);
+}
+
+int pff_upstream_count(pff_ctx* ctx, const pff_grid* g, const int32_t* rcv, uint32_t* count) {
+  PFF_TRY(begin(ctx));
+  GridD d;
+  PFF_TRY(check_grid(g, &d));
+  PFF_TRY(need(rcv, "rcv"));
+  PFF_TRY(need(count, "count"));
+  return upstream_count(ctx, d, rcv, count, nullptr);
+}
+
+int pff_accumulate_constant_Q(pff_ctx* ctx, const pff_grid* g, const int32_t* rcv, double value, int32_t area,
+                              float* Q, int32_t* converged) {
+  PFF_TRY(begin(ctx));
+  GridD d;
+  PFF_TRY(check_grid(g, &d));
+  PFF_TRY(need(rcv, "rcv"));
+  PFF_TRY(need(Q, "Q"));
+  // Q.fill(value * (DX*DX if area else 1)) evaluates in double, stored as f32
+  // (flowfields.py:279)
+  const float q0 = (float)(value * (area ? (double)g->dx * (double)g->dx : 1.0));
+  void* cnt;
+  PFF_TRY(scratch(ctx, S_I5, (size_t)d.N * 4, &cnt));
+  int32_t cycle = 0;
+  PFF_TRY(upstream_count(ctx, d, rcv, (uint32_t*)cnt, converged ? &cycle : nullptr));
+  PFF_TRY(launch_scale_counts(ctx, d, (const uint32_t*)cnt, q0, Q));
+  if (converged) *converged = cycle ? 0 : 1;
+  return 0;
+}
+
+// ---- lakeflow -------------------------------------------------------------
+int pff_depression_counter(pff_ctx* ctx, const pff_grid* g, const int32_t* rcv, int64_t* ndep) {
+  PFF_TRY(begin(ctx));
+  GridD d;
+  PFF_TRY(check_grid(g, &d));
+  PFF_TRY(need(rcv, "rcv"));
+  PFF_TRY(need(ndep, "ndep"));
+  return depression_count(ctx, d, rcv, ndep);
+}
+
+int pff_basin_id_init(pff_ctx* ctx, const pff_grid* g, int32_t* bid) {
+  PFF_TRY(begin(ctx));
+  GridD d;
+  PFF_TRY(check_grid(g, &d));
+  PFF_TRY(need(bid, "bid"));
+  return launch_basin_id_init(ctx, d, bid);
+}
+
+int pff_propagate_basin(pff_ctx* ctx, const pff_grid* g, int32_t* bid, int32_t* rec_) {
+  PFF_TRY(begin(ctx));
+  GridD d;
+  PFF_TRY(check_grid(g, &d));
+  PFF_TRY(need(bid, "bid"));
+  PFF_TRY(need(rec_, "rec_"));
+  void *b2, *r2;
+  PFF_TRY(scratch(ctx, S_I0, (size_t)d.N * 4, &b2));
+  PFF_TRY(scratch(ctx, S_I1, (size_t)d.N * 4, &r2));
+  PFF_TRY(launch_propagate_basin(ctx, d, bid, rec_, (int32_t*)b2, (int32_t*)r2));
+  PFF_TRY(check_hip(hipMemcpyAsync(bid, b2, (size_t)d.N * 4, hipMemcpyDeviceToDevice, ctx->stream), "copy"));
+  return check_hip(hipMemcpyAsync(rec_, r2, (size_t)d.N * 4, hipMemcpyDeviceToDevice, ctx->stream), "copy");
+}
+
+int pff_basin_identification(pff_ctx* ctx, const pff_grid* g, int32_t* bid, const int32_t* rec, int32_t* rec_) {
+  PFF_TRY(begin(ctx));
+  GridD d;
+  PFF_TRY(check_grid(g, &d));
+  PFF_TRY(need(bid, "bid"));
+  PFF_TRY(need(rec, "rec"));
+  PFF_TRY(need(rec_, "rec_"));
+  return basin_labels(ctx, d, rec, bid, rec_);
+}
+
+int pff_saddlesort(pff_ctx* ctx, const pff_grid* g, const int32_t* bid, uint8_t* is_border, float* z_prime,
+                   int64_t* basin_saddle, int32_t* basin_saddlenode, int64_t* outlet, const float* z) {
+  PFF_TRY(begin(ctx));
+  GridD d;
+  PFF_TRY(check_grid(g, &d));
+  if (!bid || !is_border || !z_prime || !basin_saddle || !basin_saddlenode || !outlet || !z)
+    return fail(PFF_EINVAL, "saddlesort: NULL array");
+  return launch_saddlesort(ctx, d, bid, is_border, z_prime, basin_saddle, basin_saddlenode, outlet, z);
+}
+
+int pff_reroute_jump(pff_ctx* ctx, const pff_grid* g, int32_t* rec, const int64_t* outlet) {
+  PFF_TRY(begin(ctx));
+  GridD d;
+  PFF_TRY(check_grid(g, &d));
+  PFF_TRY(need(rec, "rec"));
+  PFF_TRY(need(outlet, "outlet"));
+  return launch_reroute_jump(ctx, d, rec, outlet);
+}
+
+int pff_reroute_carve(pff_ctx* ctx, const pff_grid* g, int32_t* rec, int32_t* rec_, int32_t* rec__, uint8_t* tag,
+                      uint8_t* tag_, const int32_t* saddlenode, const int64_t* outlet) {
+  PFF_TRY(begin(ctx));
+  GridD d;
+  PFF_TRY(check_grid(g, &d));
+  if (!rec || !rec_ || !rec__ || !saddlenode || !outlet) return fail(PFF_EINVAL, "reroute_carve: NULL array");
+  return reroute_carve(ctx, d, rec, rec_, rec__, tag, tag_, saddlenode, outlet, nullptr);
+}
+
+int pff_reroute_flow(pff_ctx* ctx, const pff_grid* g, int32_t* bid, int32_t* rec, int32_t* rec_, int32_t* rec__,
+                     const float* z, float* z_prime, uint8_t* is_border, int64_t* outlet, int64_t* basin_saddle,
+                     int32_t* basin_saddlenode, uint8_t* tag, uint8_t* tag_, int32_t carve, int32_t* bid_first,
+                     pff_lake_status* st) {
+  PFF_TRY(begin(ctx));
+  GridD d;
+  PFF_TRY(check_grid(g, &d));
+  if (!bid || !rec || !rec_ || !rec__ || !z || !z_prime || !is_border || !outlet || !basin_saddle || !basin_saddlenode)
+    return fail(PFF_EINVAL, "reroute_flow: NULL array");
+  (void)tag;
+  (void)tag_;
+  hipStream_t s = ctx->stream;
+  const size_t n4 = (size_t)d.N * 4;
+  if (st) std::memset(st, 0, sizeof(*st));
+  // rec_ <- rec (lakeflow.py:491)
+  PFF_TRY(check_hip(hipMemcpyAsync(rec_, rec, n4, hipMemcpyDeviceToDevice, s), "copy"));
+  int64_t ndep = 0;
+  PFF_TRY(depression_count(ctx, d, rec, &ndep));  // :493
+  if (st) st->ndep0 = ndep;
+  if (ndep == 0) return 0;  // :496-497 (bid untouched)
+  const int nouter = ceil_log2(ndep) + 1;  // :500
+  const int npass = npass_for(d.N);
+  int32_t *levels = nullptr, *fin = nullptr;
+  if (carve) {
+    void *lv, *fv;
+    PFF_TRY(scratch(ctx, S_LEVELS, (size_t)(npass + 1) * n4, &lv));
+    PFF_TRY(scratch(ctx, S_I4, n4, &fv));
+    levels = (int32_t*)lv;
+    fin = (int32_t*)fv;
+  }
+  for (int it = 0; it < nouter; ++it) {
+    int64_t ndb = ndep;
+    if (it > 0) PFF_TRY(depression_count(ctx, d, rec_, &ndb));  // :501
+    if (st) {
+      if (st->n_loop < 64) st->ndep[st->n_loop] = ndb;
+      st->n_loop++;
+    }
+    // basin labels on rec_ (:507-511); rec__ ends with the jumped pointers
+    if (carve) PFF_TRY(labels_with_levels(ctx, d, rec_, bid, rec__, levels, fin));
+    else PFF_TRY(basin_labels(ctx, d, rec_, bid, rec__));
+    if (it == 0 && bid_first)
+      PFF_TRY(check_hip(hipMemcpyAsync(bid_first, bid, n4, hipMemcpyDeviceToDevice, s), "copy"));
+    if (ndb == 0) break;  // :512-514
+    PFF_TRY(launch_saddlesort(ctx, d, bid, is_border, z_prime, basin_saddle, basin_saddlenode, outlet, z));  // :520
+    if (st) st->n_iter++;
+    if (carve) {
+      // rec equals rec_ here, so the carve edits rec in place and copies the
+      // edited entries back to rec_ (:525-526)
+      int32_t iters = 0;
+      PFF_TRY(carve_from_levels(ctx, d, rec, rec_, levels, fin, rec__, basin_saddlenode, outlet, rec_, nullptr, npass,
+                                &iters));
+      if (st && it < 64) st->carve_iters[it] = iters;
+    } else {
+      PFF_TRY(launch_reroute_jump(ctx, d, rec_, outlet));  // :529
+    }
+  }
+  return launch_basin_id_init(ctx, d, bid);  // :537
+}
+
+// ---- fill -------------------------------------------------------------------
+int pff_fill_z_add_delta(pff_ctx* ctx, const pff_grid* g, float* zh, float* h, float* z_, const int32_t* rcv,
+                         int32_t* rcv_, int32_t* rcv__, double epsilon) {
+  PFF_TRY(begin(ctx));
+  GridD d;
+  PFF_TRY(check_grid(g, &d));
+  if (!zh || !h || !z_ || !rcv || !rcv_ || !rcv__) return fail(PFF_EINVAL, "fill_z_add_delta: NULL array");
+  const size_t n4 = (size_t)d.N * 4;
+  hipStream_t s = ctx->stream;
+  PFF_TRY(check_hip(hipMemcpyAsync(z_, zh, n4, hipMemcpyDeviceToDevice, s), "copy"));    // :116
+  PFF_TRY(check_hip(hipMemcpyAsync(rcv_, rcv, n4, hipMemcpyDeviceToDevice, s), "copy"));  // :117
+  PFF_TRY(fill_passes(ctx, d, zh, z_, rcv_, rcv__, epsilon, -1));                         // :121-122
+  return launch_apply_fill(ctx, d, zh, h, z_);                                             // :125
+}
+
+int pff_topofill(pff_ctx* ctx, const pff_grid* g, const float* z, float* out, const int32_t* rcv, int32_t* rcv_,
+                 int32_t* rcv__, double epsilon) {
+  PFF_TRY(begin(ctx));
+  GridD d;
+  PFF_TRY(check_grid(g, &d));
+  if (!z || !out || !rcv || !rcv_ || !rcv__) return fail(PFF_EINVAL, "topofill: NULL array");
+  const size_t n4 = (size_t)d.N * 4;
+  hipStream_t s = ctx->stream;
+  void* zc;
+  PFF_TRY(scratch(ctx, S_F0, n4, &zc));
+  // the passes compare against the unfilled z; keep a copy when out aliases z
+  PFF_TRY(check_hip(hipMemcpyAsync(zc, z, n4, hipMemcpyDeviceToDevice, s), "copy"));
+  if (out != z) PFF_TRY(check_hip(hipMemcpyAsync(out, z, n4, hipMemcpyDeviceToDevice, s), "copy"));
+  PFF_TRY(check_hip(hipMemcpyAsync(rcv_, rcv, n4, hipMemcpyDeviceToDevice, s), "copy"));
+  return fill_passes(ctx, d, (const float*)zc, out, rcv_, rcv__, epsilon, 0);
+}
+
+// ---- graphflood -------------------------------------------------------------
+int pff_diffuse_Q_constant_prec(pff_ctx* ctx, const pff_grid* g, const pff_hydro* p, const float* zh, float* Q,
+                                float* temp) {
+  PFF_TRY(begin(ctx));
+  GridD d;
+  PFF_TRY(check_grid(g, &d));
+  if (!p || !zh || !Q || !temp) return fail(PFF_EINVAL, "diffuse_Q_constant_prec: NULL argument");
+  PFF_TRY(launch_diffuse(ctx, d, p, zh, Q, temp));
+  // Q = temp (gf_hydrodynamics.py:74-75)
+  return check_hip(hipMemcpyAsync(Q, temp, (size_t)d.N * 4, hipMemcpyDeviceToDevice, ctx->stream), "copy");
+}
+
+int pff_graphflood_core_cte_mannings(pff_ctx* ctx, const pff_grid* g, const pff_hydro* p, float* h, float* zh,
+                                     float* dh, const int32_t* rec, const float* Q) {
+  PFF_TRY(begin(ctx));
+  GridD d;
+  PFF_TRY(check_grid(g, &d));
+  if (!p || !h || !zh || !dh || !rec || !Q) return fail(PFF_EINVAL, "graphflood_core_cte_mannings: NULL argument");
+  return launch_manning(ctx, d, p, h, zh, dh, rec, Q);
+}
+
+// ---- packed keys and utilities ------------------------------------------------
+int pff_pack_full_float_index(pff_ctx* ctx, int64_t n, int64_t* arr, const float* ft, const int32_t* it) {
+  PFF_TRY(begin(ctx));
+  if (n < 0 || (n > 0 && (!arr || !ft || !it))) return fail(PFF_EINVAL, "pack: bad arguments");
+  if (n == 0) return 0;
+  return launch_pack(ctx, n, arr, ft, it);
+}
+
+int pff_unpack_full_float_index(pff_ctx* ctx, int64_t n, const int64_t* arr, float* ft, int32_t* it) {
+  PFF_TRY(begin(ctx));
+  if (n < 0 || (n > 0 && (!arr || !ft || !it))) return fail(PFF_EINVAL, "unpack: bad arguments");
+  if (n == 0) return 0;
+  return launch_unpack(ctx, n, arr, ft, it);
+}
+
+int pff_fill_edges(pff_ctx* ctx, const pff_grid* g, uint8_t* edges) {
+  PFF_TRY(begin(ctx));
+  GridD d;
+  PFF_TRY(check_grid(g, &d));
+  PFF_TRY(need(edges, "edges"));
+  return launch_edge_maps(ctx, d, edges, nullptr);
+}
+
+int pff_flow_out_nodes(pff_ctx* ctx, const pff_grid* g, uint8_t* outnodes) {
+  PFF_TRY(begin(ctx));
+  GridD d;
+  PFF_TRY(check_grid(g, &d));
+  PFF_TRY(need(outnodes, "outnodes"));
+  return launch_edge_maps(ctx, d, nullptr, outnodes);
+}
+
+int pff_swap_arrays(pff_ctx* ctx, int64_t n, void* a, void* b) {
+  PFF_TRY(begin(ctx));
+  if (n < 0 || (n > 0 && (!a || !b))) return fail(PFF_EINVAL, "swap: bad arguments");
+  if (n == 0) return 0;
+  return launch_swap(ctx, n, (uint32_t*)a, (uint32_t*)b);
+}
+
+int pff_init_arange(pff_ctx* ctx, int64_t n, int32_t* a) {
+  PFF_TRY(begin(ctx));
+  if (n < 0 || (n > 0 && !a)) return fail(PFF_EINVAL, "init_arange: bad arguments");
+  if (n == 0) return 0;
+  return launch_arange(ctx, n, a);
+}
+
+}  // extern "C"

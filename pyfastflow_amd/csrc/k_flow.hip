@@ -1,0 +1,342 @@
+// k_flow.hip -- receivers, donors, rake-and-compress and exact upstream
+// counts, packed-key and utility kernels for libpff_hip.so (gfx950).
+#include "pff_internal.hpp"
+
+namespace pff {
+
+// ---------------------------------------------------------------------------
+// D4 steepest-descent receivers (receivers.py:15-58).
+// 2-D launch, one lane per cell, 64 consecutive columns per wavefront so each
+// row access is one coalesced 256 B line; the vertical neighbours are rows the
+// adjacent wavefronts load, served from L2.  Ties keep the earliest direction
+// (strict >), invalid links never win, and the slope is the IEEE f32 quotient
+// (z_i - z_j) / dx, so receivers match the reference bit for bit.
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ __launch_bounds__(256) void k_receivers(GridD g, const float* __restrict__ z, int32_t* __restrict__ rcv,
+                                                   float* __restrict__ grad) {
+  const int col = blockIdx.x * 64 + threadIdx.x;
+  const int row = blockIdx.y * 4 + threadIdx.y;
+  if (col >= g.nx || row >= g.ny) return;
+  const int i = row * g.nx + col;
+  const float zi = z[i];
+  int r = i;
+  float sr = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int j = neighbour<MODE>(g, i, row, col, k);
+    if (j >= 0) {
+      const float t = (zi - z[j]) / g.dx;
+      if (t > sr) {
+        sr = t;
+        r = j;
+      }
+    }
+  }
+  rcv[i] = r;
+  grad[i] = sr;
+}
+
+int launch_receivers(pff_ctx* ctx, const GridD& g, const float* z, int32_t* rcv, float* grad) {
+  dim3 block(64, 4);
+  dim3 grid((g.nx + 63) / 64, (g.ny + 3) / 4);
+  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_receivers<M>, grid, block, 0, ctx->stream, g, z, rcv, grad));
+  return check_hip(hipGetLastError(), "receivers");
+}
+
+// ---------------------------------------------------------------------------
+// rcv2donor (downstream_propag.py:77-96): 4 donor slots per node, slot taken
+// with an atomic on the receiver's donor count.
+// ---------------------------------------------------------------------------
+__global__ void k_rcv2donor(int32_t N, const int32_t* __restrict__ rcv, int32_t* __restrict__ dnr,
+                            int32_t* __restrict__ ndnr) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N) return;
+  const int r = rcv[t];
+  if (r != t && (unsigned)r < (unsigned)N) {
+    const int old = atomicAdd(&ndnr[r], 1);
+    const int64_t did = (int64_t)r * 4 + old;
+    if (did < (int64_t)N * 4) dnr[did] = t;
+  }
+}
+
+int launch_rcv2donor(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int32_t* dnr, int32_t* ndnr) {
+  hipLaunchKernelGGL(k_rcv2donor, dim3(grid_for(g.N)), dim3(kBlock), 0, ctx->stream, g.N, rcv, dnr, ndnr);
+  return check_hip(hipGetLastError(), "rcv2donor");
+}
+
+// getSrc / updateSrc ping-pong selector (downstream_propag.py:20-56): a node
+// that already wrote in round `it` is read from its pre-round buffer.
+__device__ __forceinline__ bool get_src(const int32_t* src, int t, int it) {
+  const int e = src[t];
+  bool flip = e < 0;
+  if ((e < 0 ? -e : e) == it + 1) flip = !flip;
+  return flip;
+}
+
+// One rake-and-compress round (downstream_propag.py:99-179).
+__global__ void k_rake_compress(int32_t N, int32_t* __restrict__ dnr, int32_t* __restrict__ ndnr, float* p,
+                                int32_t* src, int32_t* __restrict__ dnr_, int32_t* __restrict__ ndnr_, float* p_,
+                                int32_t it) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N) return;
+  const bool flip = get_src(src, t, it);
+  bool worked = false;
+  int donors[4] = {-1, -1, -1, -1};
+  int todo = flip ? ndnr_[t] : ndnr[t];
+  const int64_t base = (int64_t)t * 4;
+  const int64_t lim = (int64_t)N * 4;
+  float p_added = 0.0f;
+  int i = 0;
+  while (i < todo && i < 4) {
+    if (donors[i] == -1) donors[i] = flip ? dnr_[base + i] : dnr[base + i];
+    const int did = donors[i];
+    const bool fd = get_src(src, did, it);
+    const int nd = fd ? ndnr_[did] : ndnr[did];
+    if (nd <= 1) {
+      if (!worked) p_added = flip ? p_[t] : p[t];
+      worked = true;
+      p_added += fd ? p_[did] : p[did];
+      if (nd == 0) {
+        todo -= 1;
+        if (todo > i && base + todo < lim) donors[i] = flip ? dnr_[base + todo] : dnr[base + todo];
+        i -= 1;
+      } else {
+        donors[i] = fd ? dnr_[(int64_t)did * 4] : dnr[(int64_t)did * 4];
+      }
+    }
+    i += 1;
+  }
+  if (worked) {
+    const int m = todo < 4 ? todo : 4;
+    if (flip) {
+      ndnr[t] = todo;
+      p[t] = p_added;
+      for (int j = 0; j < m; ++j) dnr[base + j] = donors[j];
+    } else {
+      ndnr_[t] = todo;
+      p_[t] = p_added;
+      for (int j = 0; j < m; ++j) dnr_[base + j] = donors[j];
+    }
+    src[t] = (flip ? 1 : -1) * (it + 1);
+  }
+}
+
+int launch_rake_compress(pff_ctx* ctx, const GridD& g, int32_t* dnr, int32_t* ndnr, float* p, int32_t* src,
+                         int32_t* dnr_, int32_t* ndnr_, float* p_, int32_t iteration) {
+  hipLaunchKernelGGL(k_rake_compress, dim3(grid_for(g.N)), dim3(kBlock), 0, ctx->stream, g.N, dnr, ndnr, p, src,
+                     dnr_, ndnr_, p_, iteration);
+  return check_hip(hipGetLastError(), "rake_compress_accum");
+}
+
+// fuse (downstream_propag.py:59-74)
+__global__ void k_fuse(int32_t N, float* A, const int32_t* src, const float* B, int32_t it) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N) return;
+  if (get_src(src, t, it)) A[t] = B[t];
+}
+
+int launch_fuse(pff_ctx* ctx, const GridD& g, float* A, const int32_t* src, const float* B, int32_t iteration) {
+  hipLaunchKernelGGL(k_fuse, dim3(grid_for(g.N)), dim3(kBlock), 0, ctx->stream, g.N, A, src, B, iteration);
+  return check_hip(hipGetLastError(), "fuse");
+}
+
+// ---------------------------------------------------------------------------
+// Exact upstream counts: pointer-jumping scatter over a compacted list of
+// live nodes.  After round k, S(i) = #{j : j reaches i in < 2^k steps};
+// round k adds S(m) into the node exactly 2^k steps below m (P(m)), then
+// P(m) <- P(P(m)).  A node leaves the list once its 2^k-step successor does
+// not exist.  Integer atomics make the result independent of the order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void wave_append(int32_t* list, int64_t* count, int32_t value, bool take) {
+  const uint64_t mask = __ballot(take);
+  if (mask == 0) return;
+  const int lane = __lane_id();
+  const int leader = __ffsll((long long)mask) - 1;
+  int base = 0;
+  if (lane == leader) base = (int)atomicAdd((unsigned long long*)count, (unsigned long long)__popcll(mask));
+  base = __shfl(base, leader);
+  if (take) {
+    const int off = __popcll(mask & ((1ull << lane) - 1ull));
+    list[base + off] = value;
+  }
+}
+
+// init: S = 1, P = rcv or -1 at roots, live list = non-roots
+__global__ void k_count_init(int32_t N, const int32_t* __restrict__ rcv, uint32_t* __restrict__ S,
+                             int32_t* __restrict__ P, int32_t* __restrict__ live, int64_t* count) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  bool take = false;
+  if (t < N) {
+    const int r = rcv[t];
+    const bool root = (r == t) || (unsigned)r >= (unsigned)N;  // out-of-range receivers act as roots
+    S[t] = 1u;
+    P[t] = root ? -1 : r;
+    take = !root;
+  }
+  wave_append(live, count, t, take);
+}
+
+// phase 1: read S and the next pointer of every live node (pre-round values)
+__global__ void k_count_gather(const int32_t* __restrict__ live, const int64_t* count, const uint32_t* __restrict__ S,
+                               const int32_t* __restrict__ P, int32_t* __restrict__ tgt, uint32_t* __restrict__ val,
+                               int32_t* __restrict__ nxt, int64_t* next_count) {
+  const int64_t n = *count;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *next_count = 0;
+  for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < n; w += (int64_t)gridDim.x * blockDim.x) {
+    const int m = live[w];
+    const int pm = P[m];
+    tgt[w] = pm;
+    val[w] = S[m];
+    nxt[w] = P[pm];
+  }
+}
+
+// phase 2: scatter-add, advance pointers, compact the list
+__global__ void k_count_scatter(const int32_t* __restrict__ live, const int64_t* count, uint32_t* S, int32_t* P,
+                                const int32_t* __restrict__ tgt, const uint32_t* __restrict__ val,
+                                const int32_t* __restrict__ nxt, int32_t* __restrict__ live_out, int64_t* next_count) {
+  const int64_t n = *count;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  // iterate in whole-wave steps so wave_append sees converged lanes
+  const int64_t nr = (n + 63) / 64 * 64;
+  for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nr; w += stride) {
+    bool take = false;
+    int m = 0;
+    if (w < n) {
+      m = live[w];
+      atomicAdd(&S[tgt[w]], val[w]);
+      const int q = nxt[w];
+      P[m] = q;
+      take = (q >= 0);
+    }
+    wave_append(live_out, next_count, m, take);
+  }
+}
+
+__global__ void k_scale_counts(int32_t N, const uint32_t* __restrict__ cnt, float q0, float* __restrict__ Q) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < N) Q[t] = q0 * (float)cnt[t];
+}
+
+int launch_scale_counts(pff_ctx* ctx, const GridD& g, const uint32_t* cnt, float q0, float* Q) {
+  hipLaunchKernelGGL(k_scale_counts, dim3(grid_for(g.N)), dim3(kBlock), 0, ctx->stream, g.N, cnt, q0, Q);
+  return check_hip(hipGetLastError(), "scale_counts");
+}
+
+int upstream_count(pff_ctx* ctx, const GridD& g, const int32_t* rcv, uint32_t* cnt, int32_t* cycle) {
+  const size_t n4 = (size_t)g.N * 4;
+  void *P, *L0, *L1, *tgt, *val, *nxt;
+  PFF_TRY(scratch(ctx, S_I0, n4, &P));
+  PFF_TRY(scratch(ctx, S_WL0, n4, &L0));
+  PFF_TRY(scratch(ctx, S_WL1, n4, &L1));
+  PFF_TRY(scratch(ctx, S_I1, n4, &tgt));
+  PFF_TRY(scratch(ctx, S_I2, n4, &val));
+  PFF_TRY(scratch(ctx, S_I3, n4, &nxt));
+  int64_t* c0 = ctx->dev_words + W_COUNT0;
+  int64_t* c1 = ctx->dev_words + W_COUNT1;
+  PFF_TRY(check_hip(hipMemsetAsync(c0, 0, sizeof(int64_t), ctx->stream), "memset"));
+  hipLaunchKernelGGL(k_count_init, dim3(grid_for(g.N)), dim3(kBlock), 0, ctx->stream, g.N, rcv, cnt, (int32_t*)P,
+                     (int32_t*)L0, c0);
+  // a path has at most N-1 links: ceil(log2 N)+1 rounds always finish a forest
+  int rounds = 1;
+  while ((1ll << (rounds - 1)) < (int64_t)g.N) ++rounds;
+  int32_t* la = (int32_t*)L0;
+  int32_t* lb = (int32_t*)L1;
+  int64_t* ca = c0;
+  int64_t* cb = c1;
+  for (int k = 0; k < rounds; ++k) {
+    hipLaunchKernelGGL(k_count_gather, dim3(stride_grid()), dim3(kBlock), 0, ctx->stream, la, ca, cnt, (int32_t*)P,
+                       (int32_t*)tgt, (uint32_t*)val, (int32_t*)nxt, cb);
+    hipLaunchKernelGGL(k_count_scatter, dim3(stride_grid()), dim3(kBlock), 0, ctx->stream, la, ca, cnt, (int32_t*)P,
+                       (int32_t*)tgt, (uint32_t*)val, (int32_t*)nxt, lb, cb);
+    int32_t* tl = la; la = lb; lb = tl;
+    int64_t* tc = ca; ca = cb; cb = tc;
+  }
+  PFF_TRY(check_hip(hipGetLastError(), "upstream_count"));
+  if (cycle) {
+    PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words, ca, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream), "d2h"));
+    PFF_TRY(check_hip(hipStreamSynchronize(ctx->stream), "sync"));
+    *cycle = ctx->host_words[0] != 0;
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// packed keys and utilities
+// ---------------------------------------------------------------------------
+__global__ void k_pack(int64_t n, int64_t* arr, const float* ft, const int32_t* it) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) arr[t] = pack_key(ft[t], it[t]);
+}
+__global__ void k_unpack(int64_t n, const int64_t* arr, float* ft, int32_t* it) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) {
+    ft[t] = key_float(arr[t]);
+    it[t] = key_index(arr[t]);
+  }
+}
+__global__ void k_swap(int64_t n, uint32_t* a, uint32_t* b) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) {
+    const uint32_t x = a[t];
+    a[t] = b[t];
+    b[t] = x;
+  }
+}
+__global__ void k_arange(int64_t n, int32_t* a) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = (int32_t)t;
+}
+__global__ void k_fill_u32(int64_t n, uint32_t* a, uint32_t v) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = v;
+}
+__global__ void k_fill_i64(int64_t n, int64_t* a, int64_t v) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = v;
+}
+
+// fill_edges / flow_out_nodes debug maps (neighbourer_flat.py:112-115, 608-611)
+template <int MODE>
+__global__ void k_edge_maps(GridD g, uint8_t* __restrict__ edges, uint8_t* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.N) return;
+  int row, col;
+  rc_of(g, i, row, col);
+  if (edges) edges[i] = (uint8_t)edge_class(g, row, col);
+  if (out) out[i] = can_leave<MODE>(g, i, row, col) ? 1 : 0;
+}
+
+int launch_edge_maps(pff_ctx* ctx, const GridD& g, uint8_t* edges, uint8_t* out) {
+  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_edge_maps<M>, dim3(grid_for(g.N)), dim3(kBlock), 0, ctx->stream, g,
+                                               edges, out));
+  return check_hip(hipGetLastError(), "edge_maps");
+}
+
+int launch_pack(pff_ctx* ctx, int64_t n, int64_t* arr, const float* ft, const int32_t* it) {
+  hipLaunchKernelGGL(k_pack, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, arr, ft, it);
+  return check_hip(hipGetLastError(), "pack");
+}
+int launch_unpack(pff_ctx* ctx, int64_t n, const int64_t* arr, float* ft, int32_t* it) {
+  hipLaunchKernelGGL(k_unpack, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, arr, ft, it);
+  return check_hip(hipGetLastError(), "unpack");
+}
+int launch_swap(pff_ctx* ctx, int64_t n, uint32_t* a, uint32_t* b) {
+  hipLaunchKernelGGL(k_swap, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, a, b);
+  return check_hip(hipGetLastError(), "swap");
+}
+int launch_arange(pff_ctx* ctx, int64_t n, int32_t* a) {
+  hipLaunchKernelGGL(k_arange, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, a);
+  return check_hip(hipGetLastError(), "arange");
+}
+int launch_fill_u32(pff_ctx* ctx, int64_t n, uint32_t* a, uint32_t v) {
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, a, v);
+  return check_hip(hipGetLastError(), "fill_u32");
+}
+int launch_fill_i64(pff_ctx* ctx, int64_t n, int64_t* a, int64_t v) {
+  hipLaunchKernelGGL(k_fill_i64, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, a, v);
+  return check_hip(hipGetLastError(), "fill_i64");
+}
+
+}  // namespace pff

@@ -1,0 +1,213 @@
+// k_gf.hip -- GraphFlood on gfx950: epsilon-fill by pointer jumping,
+// gather-form (atomic-free) discharge diffusion and the Manning depth update.
+#include "pff_internal.hpp"
+
+namespace pff {
+
+// ---------------------------------------------------------------------------
+// _topofill passes (fill_topo.py:16-45), synchronous form: every read sees the
+// pass's input (ping-pong buffers), one legal interleaving of the in-place
+// reference pass.  Pass `it` uses the increment f32(2^(it+first_exp) * eps)
+// with first_exp = -1 for fill_z_add_delta (fill_topo.py:121-122) and 0 for
+// topofill (fill_topo.py:70-72).
+// ---------------------------------------------------------------------------
+__global__ void k_topofill(int32_t N, const float* __restrict__ zh, const float* __restrict__ zin,
+                           float* __restrict__ zout, const int32_t* __restrict__ r0, int32_t* __restrict__ r1,
+                           float inc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int r = r0[i];
+  if ((unsigned)r >= (unsigned)N) r = i;
+  int rr = r0[r];
+  if ((unsigned)rr >= (unsigned)N) rr = r;
+  r1[i] = rr;
+  const float zi = zin[i];
+  if (i == r || (zi > zh[r] && rr == r)) {
+    zout[i] = zi;
+    return;
+  }
+  const float c = zin[r] + inc;
+  zout[i] = zi > c ? zi : c;
+}
+
+__global__ void k_copy_f32(int64_t n, const float* __restrict__ a, float* __restrict__ b) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) b[t] = a[t];
+}
+__global__ void k_copy_i32b(int64_t n, const int32_t* __restrict__ a, int32_t* __restrict__ b) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) b[t] = a[t];
+}
+
+// Runs ceil(log2 N) passes.  z_ holds the start elevations and receives the
+// filled ones; r0 holds the receivers and receives the final jumped pointers
+// (r1 too).
+int fill_passes(pff_ctx* ctx, const GridD& g, const float* zh, float* z_, int32_t* r0, int32_t* r1, double eps,
+                int first_exp) {
+  void* zt;
+  PFF_TRY(scratch(ctx, S_F1, (size_t)g.N * 4, &zt));
+  int nit = 0;
+  while ((1ll << nit) < (int64_t)g.N) ++nit;  // ceil(log2 N)
+  const dim3 gr(grid_for(g.N)), bl(kBlock);
+  float* za = z_;
+  float* zb = (float*)zt;
+  int32_t* ra = r0;
+  int32_t* rb = r1;
+  for (int it = 0; it < nit; ++it) {
+    const float inc = (float)(ldexp(1.0, it + first_exp) * eps);
+    hipLaunchKernelGGL(k_topofill, gr, bl, 0, ctx->stream, g.N, zh, za, zb, ra, rb, inc);
+    float* tz = za; za = zb; zb = tz;
+    int32_t* tr = ra; ra = rb; rb = tr;
+  }
+  if (za != z_) hipLaunchKernelGGL(k_copy_f32, gr, bl, 0, ctx->stream, (int64_t)g.N, za, z_);
+  // reference end state: rec_ and rec__ both hold the final jumped pointers
+  if (ra != r0) hipLaunchKernelGGL(k_copy_i32b, gr, bl, 0, ctx->stream, (int64_t)g.N, ra, r0);
+  else hipLaunchKernelGGL(k_copy_i32b, gr, bl, 0, ctx->stream, (int64_t)g.N, ra, r1);
+  return check_hip(hipGetLastError(), "topofill");
+}
+
+// _apply_fill_z_add_delta (fill_topo.py:81-97)
+__global__ void k_apply_fill(int32_t N, float* __restrict__ zh, float* __restrict__ h, const float* __restrict__ z_) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const float zf = z_[i];
+  h[i] += zf - zh[i];
+  zh[i] = zf;
+}
+
+int launch_apply_fill(pff_ctx* ctx, const GridD& g, float* zh, float* h, const float* z_) {
+  hipLaunchKernelGGL(k_apply_fill, dim3(grid_for(g.N)), dim3(kBlock), 0, ctx->stream, g.N, zh, h, z_);
+  return check_hip(hipGetLastError(), "apply_fill");
+}
+
+// ---------------------------------------------------------------------------
+// diffuse_Q_constant_prec (gf_hydrodynamics.py:23-75) in gather form.
+// Cell j (not can_leave) sends S_jk / sum_k S_jk * Q_j through each link k
+// with S_jk = max(0, (zh_j - zh_nk)/dx); can_leave cells send nothing (their
+// Q is zeroed first, :46-49).  temp_i = P*dx^2 + sum of what reaches i.
+// Instead of the reference's float atomics, each cell gathers from the cells
+// that link to it, so the sum order is fixed and the result deterministic.
+// ---------------------------------------------------------------------------
+template <int MODE>
+__device__ __forceinline__ float share_to(const GridD& g, const float* zh, const float* Q, int j, int target) {
+  int row, col;
+  rc_of(g, j, row, col);
+  if (can_leave<MODE>(g, j, row, col)) return 0.0f;
+  const float zj = zh[j];
+  float s[4];
+  int nk[4];
+  float sums = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int n = neighbour<MODE>(g, j, row, col, k);
+    nk[k] = n;
+    const float t = (n >= 0) ? (zj - zh[n]) / g.dx : 0.0f;
+    s[k] = t > 0.0f ? t : 0.0f;
+    sums += s[k];
+  }
+  if (sums == 0.0f) return 0.0f;
+  const float q = Q[j];
+  float acc = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (nk[k] == target) acc += s[k] / sums * q;
+  return acc;
+}
+
+template <int MODE>
+__global__ void k_diffuse(GridD g, float pin, const float* __restrict__ zh, const float* __restrict__ Q,
+                          float* __restrict__ temp) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.N) return;
+  int row, col;
+  rc_of(g, i, row, col);
+  float acc = pin;
+  if constexpr (MODE != 3) {
+    // modes 0-2 have symmetric links: the sources of i are its neighbours
+    int prev[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int j = neighbour<MODE>(g, i, row, col, k);
+      prev[k] = j;
+      bool dup = false;
+#pragma unroll
+      for (int m = 0; m < k; ++m) dup |= (prev[m] == j);
+      if (j >= 0 && !dup) acc += share_to<MODE>(g, zh, Q, j, i);
+    }
+  } else {
+    // custom codes can make links one-way: test every geometric neighbour
+    // (both axes wrapped) for a link into i
+    int cand[4];
+    cand[0] = row > 0 ? i - g.nx : i + g.nx * (g.ny - 1);
+    cand[1] = col > 0 ? i - 1 : i + g.nx - 1;
+    cand[2] = col < g.nx - 1 ? i + 1 : i - g.nx + 1;
+    cand[3] = row < g.ny - 1 ? i + g.nx : i - g.nx * (g.ny - 1);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      bool dup = cand[k] == i;
+      for (int m = 0; m < k; ++m) dup |= (cand[m] == cand[k]);
+      if (!dup) acc += share_to<MODE>(g, zh, Q, cand[k], i);
+    }
+  }
+  temp[i] = acc;
+}
+
+int launch_diffuse(pff_ctx* ctx, const GridD& g, const pff_hydro* p, const float* zh, const float* Q, float* temp) {
+  const float pin = (float)(p->prec * (double)g.dx * (double)g.dx);
+  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_diffuse<M>, dim3(grid_for(g.N)), dim3(kBlock), 0, ctx->stream, g,
+                                               pin, zh, Q, temp));
+  return check_hip(hipGetLastError(), "diffuse");
+}
+
+// ---------------------------------------------------------------------------
+// graphflood_core_cte_mannings (gf_hydrodynamics.py:78-129).  h receives tdh
+// twice (:121, :128) and zh once (:129), as in the reference; the second
+// kernel applies dh after every slope has been read.
+// ---------------------------------------------------------------------------
+struct ManningC {
+  float dx, edgesw, manning, dt, dx2, e53;
+};
+
+__global__ void k_manning1(int32_t N, ManningC c, float* __restrict__ h, const float* __restrict__ zh,
+                           float* __restrict__ dh, const int32_t* __restrict__ rec, const float* __restrict__ Q) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  float tS = c.edgesw;
+  const int r = rec[i];
+  if (r != i && (unsigned)r < (unsigned)N) {
+    const float s = (zh[i] - zh[r]) / c.dx;
+    tS = s > 1e-4f ? s : 1e-4f;
+  }
+  const float hi = h[i];
+  const float Qo = c.dx * powf(hi, c.e53) / c.manning * sqrtf(tS);
+  float tdh = (Q[i] - Qo) / c.dx2 * c.dt;
+  const float hn = hi + tdh;
+  if (hn < 0.0f) tdh += hn;
+  h[i] = hn;
+  dh[i] = tdh;
+}
+
+__global__ void k_manning2(int32_t N, float* __restrict__ h, float* __restrict__ zh, const float* __restrict__ dh) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const float d = dh[i];
+  h[i] += d;
+  zh[i] += d;
+}
+
+int launch_manning(pff_ctx* ctx, const GridD& g, const pff_hydro* p, float* h, float* zh, float* dh,
+                   const int32_t* rec, const float* Q) {
+  ManningC c;
+  c.dx = g.dx;
+  c.edgesw = (float)p->edge_slope;
+  c.manning = (float)p->manning;
+  c.dt = (float)p->dt;
+  c.dx2 = (float)((double)g.dx * (double)g.dx);
+  c.e53 = (float)(5.0 / 3.0);
+  const dim3 gr(grid_for(g.N)), bl(kBlock);
+  hipLaunchKernelGGL(k_manning1, gr, bl, 0, ctx->stream, g.N, c, h, zh, dh, rec, Q);
+  hipLaunchKernelGGL(k_manning2, gr, bl, 0, ctx->stream, g.N, h, zh, dh);
+  return check_hip(hipGetLastError(), "manning");
+}
+
+}  // namespace pff

@@ -1,0 +1,546 @@
+// k_lake.hip -- lakeflow on gfx950: depression count, basin labels by
+// compacted pointer jumping, the five saddlesort passes with packed 64-bit
+// atomicMin, and carving by tag doubling over the saved jump levels.
+#include "pff_internal.hpp"
+
+namespace pff {
+
+__device__ __forceinline__ void wave_append_l(int32_t* list, int64_t* count, int32_t value, bool take) {
+  const uint64_t mask = __ballot(take);
+  if (mask == 0) return;
+  const int lane = __lane_id();
+  const int leader = __ffsll((long long)mask) - 1;
+  int base = 0;
+  if (lane == leader) base = (int)atomicAdd((unsigned long long*)count, (unsigned long long)__popcll(mask));
+  base = __shfl(base, leader);
+  if (take) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = value;
+}
+
+// ---------------------------------------------------------------------------
+// depression_counter (lakeflow.py:24-44): #{i : rcv[i]==i and not can_leave(i)}
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ void k_depcount(GridD g, const int32_t* __restrict__ rcv, int64_t* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool pit = false;
+  if (i < g.N && rcv[i] == i) {
+    int row, col;
+    rc_of(g, i, row, col);
+    pit = !can_leave<MODE>(g, i, row, col);
+  }
+  const uint64_t mask = __ballot(pit);
+  if (__lane_id() == 0 && mask) atomicAdd((unsigned long long*)out, (unsigned long long)__popcll(mask));
+}
+
+int depression_count(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int64_t* out) {
+  int64_t* w = ctx->dev_words + W_COUNT2;
+  PFF_TRY(check_hip(hipMemsetAsync(w, 0, 8, ctx->stream), "memset"));
+  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_depcount<M>, dim3(grid_for(g.N)), dim3(kBlock), 0, ctx->stream, g,
+                                               rcv, w));
+  PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_COUNT2, w, 8, hipMemcpyDeviceToHost, ctx->stream), "d2h"));
+  PFF_TRY(check_hip(hipStreamSynchronize(ctx->stream), "sync"));
+  *out = ctx->host_words[W_COUNT2];
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// basin_id_init (lakeflow.py:46-61) and one synchronous propagate_basin pass
+// (lakeflow.py:63-80)
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ void k_basin_init(GridD g, int32_t* __restrict__ bid) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.N) return;
+  int row, col;
+  rc_of(g, i, row, col);
+  bid[i] = can_leave<MODE>(g, i, row, col) ? 0 : i + 1;
+}
+
+int launch_basin_id_init(pff_ctx* ctx, const GridD& g, int32_t* bid) {
+  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_basin_init<M>, dim3(grid_for(g.N)), dim3(kBlock), 0, ctx->stream, g,
+                                               bid));
+  return check_hip(hipGetLastError(), "basin_id_init");
+}
+
+__global__ void k_propagate(int32_t N, const int32_t* __restrict__ bid_in, const int32_t* __restrict__ rec_in,
+                            int32_t* __restrict__ bid_out, int32_t* __restrict__ rec_out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int r = rec_in[i];
+  if ((unsigned)r >= (unsigned)N) r = i;
+  bid_out[i] = bid_in[r];
+  int rr = rec_in[r];
+  if ((unsigned)rr >= (unsigned)N) rr = r;
+  rec_out[i] = rr;
+}
+
+int launch_propagate_basin(pff_ctx* ctx, const GridD& g, const int32_t* bid_in, const int32_t* rec_in, int32_t* bid_out,
+                           int32_t* rec_out) {
+  hipLaunchKernelGGL(k_propagate, dim3(grid_for(g.N)), dim3(kBlock), 0, ctx->stream, g.N, bid_in, rec_in, bid_out,
+                     rec_out);
+  return check_hip(hipGetLastError(), "propagate_basin");
+}
+
+// ---------------------------------------------------------------------------
+// Basin labels by compacted pointer jumping.
+// Semantics: npass = ceil(log2 N)+1 synchronous passes of
+//   bid[i] <- bid[P[i]],  P[i] <- P[P[i]]    starting from P = rec, bid = init.
+// A node whose pointer is a fixed point (a root, P[r]==r) takes the root's
+// final label in the next pass and never changes again, so it leaves the
+// list; nodes still listed get exactly the synchronous update.  Every round
+// optionally saves the jumped pointer of each listed node to LEVEL[k+1]
+// (LEVEL[0] = rec) and the finish round to fin[], which the carve uses.
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ void k_label_init(GridD g, const int32_t* __restrict__ rec, int32_t* __restrict__ bid,
+                             int32_t* __restrict__ P, int32_t* __restrict__ fin, int32_t* __restrict__ level0,
+                             int32_t* __restrict__ live, int64_t* count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool take = false;
+  if (i < g.N) {
+    int row, col;
+    rc_of(g, i, row, col);
+    bid[i] = can_leave<MODE>(g, i, row, col) ? 0 : i + 1;
+    int r = rec[i];
+    if ((unsigned)r >= (unsigned)g.N) r = i;  // out-of-range receivers act as roots
+    P[i] = r;
+    if (level0) level0[i] = r;
+    take = (r != i);
+    // finish round: 0 for roots, 'never' until a listed node finishes
+    if (fin) fin[i] = take ? 0x7fffffff : 0;
+  }
+  wave_append_l(live, count, i, take);
+}
+
+__global__ void k_label_gather(const int32_t* __restrict__ live, const int64_t* count, const int32_t* __restrict__ bid,
+                               const int32_t* __restrict__ P, int32_t* __restrict__ nb, int32_t* __restrict__ np,
+                               int64_t* next_count) {
+  const int64_t n = *count;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *next_count = 0;
+  for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < n; w += (int64_t)gridDim.x * blockDim.x) {
+    const int m = live[w];
+    const int pm = P[m];
+    nb[w] = bid[pm];
+    const int q = P[pm];
+    // still live after this pass unless pm is a fixed point (then q == pm)
+    np[w] = (q == pm) ? ~q : q;
+  }
+}
+
+__global__ void k_label_scatter(const int32_t* __restrict__ live, const int64_t* count, int32_t* bid, int32_t* P,
+                                const int32_t* __restrict__ nb, const int32_t* __restrict__ np,
+                                int32_t* __restrict__ live_out, int64_t* next_count, int32_t* __restrict__ level_next,
+                                int32_t* __restrict__ fin, int32_t round) {
+  const int64_t n = *count;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t nr = (n + 63) / 64 * 64;
+  for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nr; w += stride) {
+    bool take = false;
+    int m = 0;
+    if (w < n) {
+      m = live[w];
+      int q = np[w];
+      const bool done = q < 0;
+      if (done) q = ~q;
+      bid[m] = nb[w];
+      P[m] = q;
+      if (level_next) level_next[m] = q;
+      if (done && fin) fin[m] = round + 1;
+      take = !done;
+    }
+    wave_append_l(live_out, next_count, m, take);
+  }
+}
+
+static int label_rounds(int32_t N) {
+  int r = 0;
+  while ((1ll << r) < (int64_t)N) ++r;
+  return r + 1;  // ceil(log2 N) + 1 (lakeflow.py:510)
+}
+
+// Labels with optional saved levels.  levels: (npass+1) x N int32 or null;
+// fin: N int32 or null.  P (scratch) ends with the final pointers.
+static int labels_impl(pff_ctx* ctx, const GridD& g, const int32_t* rec, int32_t* bid, int32_t* P, int32_t* levels,
+                       int32_t* fin) {
+  const size_t n4 = (size_t)g.N * 4;
+  void *L0, *L1, *nbv, *npv;
+  PFF_TRY(scratch(ctx, S_WL0, n4, &L0));
+  PFF_TRY(scratch(ctx, S_WL1, n4, &L1));
+  PFF_TRY(scratch(ctx, S_I2, n4, &nbv));
+  PFF_TRY(scratch(ctx, S_I3, n4, &npv));
+  int64_t* c0 = ctx->dev_words + W_COUNT0;
+  int64_t* c1 = ctx->dev_words + W_COUNT1;
+  PFF_TRY(check_hip(hipMemsetAsync(c0, 0, 8, ctx->stream), "memset"));
+  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_label_init<M>, dim3(grid_for(g.N)), dim3(kBlock), 0, ctx->stream, g,
+                                               rec, bid, P, fin, levels, (int32_t*)L0, c0));
+  const int npass = label_rounds(g.N);
+  int32_t* la = (int32_t*)L0;
+  int32_t* lb = (int32_t*)L1;
+  int64_t* ca = c0;
+  int64_t* cb = c1;
+  for (int k = 0; k < npass; ++k) {
+    hipLaunchKernelGGL(k_label_gather, dim3(stride_grid()), dim3(kBlock), 0, ctx->stream, la, ca, bid, P,
+                       (int32_t*)nbv, (int32_t*)npv, cb);
+    int32_t* lvl = levels ? levels + (size_t)(k + 1) * g.N : nullptr;
+    hipLaunchKernelGGL(k_label_scatter, dim3(stride_grid()), dim3(kBlock), 0, ctx->stream, la, ca, bid, P,
+                       (int32_t*)nbv, (int32_t*)npv, lb, cb, lvl, fin, k);
+    int32_t* tl = la; la = lb; lb = tl;
+    int64_t* tc = ca; ca = cb; cb = tc;
+  }
+  return check_hip(hipGetLastError(), "basin_labels");
+}
+
+int basin_labels(pff_ctx* ctx, const GridD& g, const int32_t* rec, int32_t* bid, int32_t* rec_out) {
+  void* P;
+  if (rec_out) {
+    P = rec_out;
+  } else {
+    PFF_TRY(scratch(ctx, S_I4, (size_t)g.N * 4, &P));
+  }
+  return labels_impl(ctx, g, rec, bid, (int32_t*)P, nullptr, nullptr);
+}
+
+// ---------------------------------------------------------------------------
+// saddlesort (lakeflow.py:105-281)
+// ---------------------------------------------------------------------------
+// pass 1 (:132-157): border flag and z'
+template <int MODE>
+__global__ void k_zprime(GridD g, const int32_t* __restrict__ bid, const float* __restrict__ z,
+                         uint8_t* __restrict__ isb, float* __restrict__ zp) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.N) return;
+  int row, col;
+  rc_of(g, i, row, col);
+  if (can_leave<MODE>(g, i, row, col)) {  // is_border keeps its value (never set for these cells)
+    zp[i] = z[i];
+    return;
+  }
+  const int bi = bid[i];
+  bool border = false;
+  float zn = 1e9f;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int j = neighbour<MODE>(g, i, row, col, k);
+    if (j < 0) continue;
+    if (bid[j] != bi) {
+      border = true;
+      zn = fminf(zn, z[j]);
+    }
+  }
+  isb[i] = border ? 1 : 0;
+  zp[i] = border ? fmaxf(z[i], zn) : 1e9f;
+}
+
+// pass 2 init (:164-167)
+__global__ void k_saddle_init(int32_t N, int64_t* __restrict__ saddle, int64_t* __restrict__ outlet,
+                              int32_t* __restrict__ snode) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  saddle[i] = INVALID_KEY;
+  outlet[i] = INVALID_KEY;
+  snode[i] = -1;
+}
+
+// pass 2 (:170-194): lexicographic min of (z'_i, bid_j) per basin, packed
+// 64-bit signed atomicMin
+template <int MODE>
+__global__ void k_saddle_min(GridD g, const int32_t* __restrict__ bid, const uint8_t* __restrict__ isb,
+                             const float* __restrict__ zp, int64_t* saddle) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.N || !isb[i]) return;
+  int row, col;
+  rc_of(g, i, row, col);
+  const int tb = bid[i];
+  if (tb >= g.N) return;  // basin N (custom mode, cell N-1 interior) has no per-basin slot
+  int64_t res = INVALID_KEY;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int j = neighbour<MODE>(g, i, row, col, k);
+    if (j < 0) continue;
+    const int bj = bid[j];
+    if (bj != tb) {
+      const int64_t c = pack_key(zp[i], bj);
+      res = c < res ? c : res;
+    }
+  }
+  if (res == INVALID_KEY) return;
+  atomicMin((long long*)&saddle[tb], (long long)res);
+}
+
+// pass 3 (:199-219): saddle node; the reference's racy store is made
+// deterministic as "largest cell index" with atomicMax
+template <int MODE>
+__global__ void k_saddle_node(GridD g, const int32_t* __restrict__ bid, const uint8_t* __restrict__ isb,
+                              const float* __restrict__ zp, const int64_t* __restrict__ saddle, int32_t* snode) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.N || !isb[i]) return;
+  const int bi = bid[i];
+  if (bi == 0 || bi >= g.N) return;
+  int row, col;
+  rc_of(g, i, row, col);
+  const int64_t key = saddle[bi];
+  const float tz = key_float(key);
+  const int tb = key_index(key);
+  bool here = false;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int j = neighbour<MODE>(g, i, row, col, k);
+    if (j < 0) continue;
+    if (bid[j] == tb && zp[i] == tz) here = true;
+  }
+  if (here) atomicMax(&snode[bi], i);
+}
+
+// pass 4 (:226-250): outlet = lowest other-basin neighbour of the saddle node
+// (first direction wins ties, strict < from 1e9)
+template <int MODE>
+__global__ void k_outlet(GridD g, const int32_t* __restrict__ bid, const float* __restrict__ z,
+                         const int64_t* __restrict__ saddle, const int32_t* __restrict__ snode,
+                         int64_t* __restrict__ outlet) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < 1 || b >= g.N) return;
+  if (saddle[b] == INVALID_KEY) return;
+  const int node = snode[b];
+  if (node < 0) return;
+  int row, col;
+  rc_of(g, node, row, col);
+  float tz = 1e9f;
+  int rr = -1;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int j = neighbour<MODE>(g, node, row, col, k);
+    if (j < 0) continue;
+    if (bid[j] != b && tz > z[j]) {
+      tz = z[j];
+      rr = j;
+    }
+  }
+  if (rr > -1) {
+    const int64_t c = pack_key(tz, rr);
+    if (c < outlet[b]) outlet[b] = c;
+  }
+}
+
+// pass 5 (:255-281) in two phases.  The reference loop reads outlets other
+// basins may invalidate in the same loop; in serial ascending order that
+// read only matters for the basin holding cell 42 (the sentinel's index):
+// it is also removed when it drains into a lower-id basin that was removed.
+__global__ void k_cycle_flag(int32_t N, const int32_t* __restrict__ bid, const int64_t* __restrict__ outlet,
+                             uint8_t* __restrict__ inv) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= N) return;
+  uint8_t f = 0;
+  if (b >= 1 && outlet[b] != INVALID_KEY) {
+    const int r1 = key_index(outlet[b]);
+    const int b1 = (r1 >= 0 && r1 < N) ? bid[r1] : -1;
+    if (b1 > 0 && b1 < N) {
+      const int r2 = key_index(outlet[b1]);
+      const int b2 = (r2 >= 0 && r2 < N) ? bid[r2] : -1;
+      if (b2 == b && b1 < b) f = 1;
+    }
+  }
+  inv[b] = f;
+}
+
+__global__ void k_cycle_apply(int32_t N, const int32_t* __restrict__ bid, int64_t* outlet, int64_t* saddle,
+                              int32_t* snode, const uint8_t* __restrict__ inv) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < 1 || b >= N) return;
+  bool kill = inv[b] != 0;
+  if (!kill && N > 42 && bid[42] == b && outlet[b] != INVALID_KEY) {
+    const int r1 = key_index(outlet[b]);
+    const int b1 = (r1 >= 0 && r1 < N) ? bid[r1] : -1;
+    if (b1 > 0 && b1 < b && inv[b1]) kill = true;
+  }
+  if (kill) {
+    outlet[b] = INVALID_KEY;
+    saddle[b] = INVALID_KEY;
+    snode[b] = -1;
+  }
+}
+
+int launch_saddlesort(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t* isb, float* zp, int64_t* saddle,
+                      int32_t* snode, int64_t* outlet, const float* z) {
+  void* inv;
+  PFF_TRY(scratch(ctx, S_B0, (size_t)g.N, &inv));
+  const dim3 gr(grid_for(g.N)), bl(kBlock);
+  hipStream_t s = ctx->stream;
+  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_zprime<M>, gr, bl, 0, s, g, bid, z, isb, zp));
+  hipLaunchKernelGGL(k_saddle_init, gr, bl, 0, s, g.N, saddle, outlet, snode);
+  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_saddle_min<M>, gr, bl, 0, s, g, bid, isb, zp, saddle));
+  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_saddle_node<M>, gr, bl, 0, s, g, bid, isb, zp, saddle, snode));
+  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_outlet<M>, gr, bl, 0, s, g, bid, z, saddle, snode, outlet));
+  hipLaunchKernelGGL(k_cycle_flag, gr, bl, 0, s, g.N, bid, outlet, (uint8_t*)inv);
+  hipLaunchKernelGGL(k_cycle_apply, gr, bl, 0, s, g.N, bid, outlet, saddle, snode, (const uint8_t*)inv);
+  return check_hip(hipGetLastError(), "saddlesort");
+}
+
+// reroute_jump (lakeflow.py:284-304): rec[b-1] = outlet node of basin b
+__global__ void k_reroute_jump(int32_t N, int32_t* rec, const int64_t* __restrict__ outlet) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < 1 || b >= N) return;
+  const int64_t o = outlet[b];
+  if (o == INVALID_KEY) return;
+  rec[b - 1] = key_index(o);
+}
+
+int launch_reroute_jump(pff_ctx* ctx, const GridD& g, int32_t* rec, const int64_t* outlet) {
+  hipLaunchKernelGGL(k_reroute_jump, dim3(grid_for(g.N)), dim3(kBlock), 0, ctx->stream, g.N, rec, outlet);
+  return check_hip(hipGetLastError(), "reroute_jump");
+}
+
+// ---------------------------------------------------------------------------
+// Carve (lakeflow.py:307-438).  The reference tags, for every saddle node s,
+// the set {rec^k(s) : k < 2^t} after t rounds of "tag rec(i) for tagged i,
+// then rec <- rec o rec", until a round adds nothing; then it reverses the
+// tagged links and joins each saddle node to its outlet.  Here the tagged
+// set is a list; round t jumps every listed node by 2^t using the pointer
+// levels saved by the labelling, so the work is proportional to the tagged
+// path lengths instead of N per round.
+// ---------------------------------------------------------------------------
+
+// seed: tag every valid saddle node (init_reroute_carve, :323-330)
+__global__ void k_carve_seed(int32_t N, const int32_t* __restrict__ snode, uint32_t* tagw, int32_t* list,
+                             int64_t* count) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  bool take = false;
+  int s = 0;
+  if (b < N) {
+    s = snode[b];
+    if (s >= 0) take = atomicCAS(&tagw[s], 0u, 1u) == 0u;
+  }
+  wave_append_l(list, count, s, take);
+}
+
+// words: [0] tagged count, [1] count before this round, [2] count before the
+// previous round
+__global__ void k_carve_prep(int64_t* w) {
+  w[2] = w[1];
+  w[1] = w[0];
+}
+
+// P_t(i): LEVEL[t][i] while i was listed in the labelling, else its final
+// pointer
+__device__ __forceinline__ int jump_level(const int32_t* levels, const int32_t* fin, const int32_t* Pfinal, int N,
+                                          int t, int i) {
+  return (t <= fin[i]) ? levels[(size_t)t * N + i] : Pfinal[i];
+}
+
+// round t: every node tagged before the round tags its 2^t-th successor
+__global__ void k_carve_round(int32_t N, const int32_t* __restrict__ levels, const int32_t* __restrict__ fin,
+                              const int32_t* __restrict__ Pfinal, uint32_t* tagw, int32_t* list, int64_t* w,
+                              int32_t t) {
+  const int64_t n = w[1];
+  if (t > 0 && n == w[2]) return;  // the previous round tagged nothing: converged
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t nr = (n + 63) / 64 * 64;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nr; e += stride) {
+    bool take = false;
+    int j = 0;
+    if (e < n) {
+      const int i = list[e];
+      j = jump_level(levels, fin, Pfinal, N, t, i);
+      if (tagw[j] == 0u) take = atomicCAS(&tagw[j], 0u, 1u) == 0u;
+    }
+    wave_append_l(list, w, j, take);
+  }
+}
+
+// finalise_reroute_carve (:390-397): reverse every tagged link i -> rec_in[i]
+__global__ void k_carve_reverse(int32_t N, const int32_t* __restrict__ list, const int64_t* w,
+                                const uint32_t* __restrict__ tagw,
+                                const int32_t* __restrict__ rec_in, int32_t* rec) {
+  const int64_t n = w[0];
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = list[e];
+    const int r = rec_in[i];
+    if (r != i && (unsigned)r < (unsigned)N && tagw[r]) rec[r] = i;
+  }
+}
+
+// finalise_reroute_carve (:399-403): saddle node -> outlet node
+__global__ void k_carve_link(int32_t N, const int64_t* __restrict__ outlet, const int32_t* __restrict__ snode,
+                             int32_t* rec) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= N) return;
+  const int64_t o = outlet[b];
+  if (o != INVALID_KEY) rec[snode[b]] = key_index(o);
+}
+
+// copy the carved receivers of tagged nodes to the working copy and clear tags
+__global__ void k_carve_sync(const int32_t* __restrict__ list, const int64_t* w, uint32_t* tagw,
+                             const int32_t* __restrict__ rec, int32_t* rec_work, uint8_t* tag_bytes) {
+  const int64_t n = w[0];
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = list[e];
+    if (rec_work) rec_work[i] = rec[i];
+    if (tag_bytes) tag_bytes[i] = 1;
+    tagw[i] = 0u;
+  }
+}
+
+__global__ void k_copy_i32(int64_t n, const int32_t* __restrict__ a, int32_t* __restrict__ b) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) b[t] = a[t];
+}
+
+// Full carve for one trip.  rec must equal rec_in on entry (the driver keeps
+// them equal); on return rec holds the carved receivers and, if rec_work is
+// given, it is brought up to date too.
+int carve_from_levels(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t* rec_in, const int32_t* levels,
+                      const int32_t* fin, const int32_t* Pfinal, const int32_t* snode, const int64_t* outlet,
+                      int32_t* rec_work, uint8_t* tag_bytes, int npass, int32_t* iters) {
+  void *tagw, *list;
+  PFF_TRY(scratch(ctx, S_TAG, (size_t)g.N * 4, &tagw, true));
+  PFF_TRY(scratch(ctx, S_MISC, (size_t)g.N * 4, &list));
+  int64_t* w = ctx->dev_words + W_COUNT0;  // 3 consecutive words
+  PFF_TRY(check_hip(hipMemsetAsync(w, 0, 3 * 8, ctx->stream), "memset"));
+  hipStream_t s = ctx->stream;
+  hipLaunchKernelGGL(k_carve_seed, dim3(grid_for(g.N)), dim3(kBlock), 0, s, g.N, snode, (uint32_t*)tagw,
+                     (int32_t*)list, w);
+  for (int t = 0; t <= npass; ++t) {
+    hipLaunchKernelGGL(k_carve_prep, dim3(1), dim3(1), 0, s, w);
+    hipLaunchKernelGGL(k_carve_round, dim3(stride_grid()), dim3(kBlock), 0, s, g.N, levels, fin, Pfinal,
+                       (uint32_t*)tagw, (int32_t*)list, w, t);
+  }
+  hipLaunchKernelGGL(k_carve_reverse, dim3(stride_grid()), dim3(kBlock), 0, s, g.N, (const int32_t*)list, w,
+                     (const uint32_t*)tagw, rec_in, rec);
+  hipLaunchKernelGGL(k_carve_link, dim3(grid_for(g.N)), dim3(kBlock), 0, s, g.N, outlet, snode, rec);
+  hipLaunchKernelGGL(k_carve_sync, dim3(stride_grid()), dim3(kBlock), 0, s, (const int32_t*)list, w,
+                     (uint32_t*)tagw, rec, rec_work, tag_bytes);
+  if (iters) *iters = -1;
+  return check_hip(hipGetLastError(), "carve");
+}
+
+// Labels with the levels a carve needs.  levels: (npass+1) x N, level 0 = rec.
+int labels_with_levels(pff_ctx* ctx, const GridD& g, const int32_t* rec, int32_t* bid, int32_t* Pfinal,
+                       int32_t* levels, int32_t* fin) {
+  return labels_impl(ctx, g, rec, bid, Pfinal, levels, fin);
+}
+
+int npass_for(int32_t N) { return label_rounds(N); }
+
+// Reference-shaped carve (lakeflow.py:405-438) for the module-level API.
+// rec_ is the working copy on entry and is left holding the jumped pointers
+// (the reference clobbers it too); rec__ receives the pre-carve copy; rec
+// receives the carved receivers; tag (nullable) the tagged cells.
+int reroute_carve(pff_ctx* ctx, const GridD& g, int32_t* rec, int32_t* rec_, int32_t* rec__, uint8_t* tag,
+                  uint8_t* tag_, const int32_t* snode, const int64_t* outlet, int32_t* iters) {
+  const int npass = label_rounds(g.N);
+  void *lv, *fin, *bid;
+  PFF_TRY(scratch(ctx, S_LEVELS, (size_t)(npass + 1) * g.N * 4, &lv));
+  PFF_TRY(scratch(ctx, S_I4, (size_t)g.N * 4, &fin));
+  PFF_TRY(scratch(ctx, S_F0, (size_t)g.N * 4, &bid));
+  const dim3 gr(grid_for(g.N)), bl(kBlock);
+  hipLaunchKernelGGL(k_copy_i32, gr, bl, 0, ctx->stream, (int64_t)g.N, rec_, rec__);  // :428
+  hipLaunchKernelGGL(k_copy_i32, gr, bl, 0, ctx->stream, (int64_t)g.N, rec_, rec);    // :427
+  PFF_TRY(labels_with_levels(ctx, g, rec__, (int32_t*)bid, rec_, (int32_t*)lv, (int32_t*)fin));
+  if (tag) PFF_TRY(check_hip(hipMemsetAsync(tag, 0, (size_t)g.N, ctx->stream), "memset"));
+  PFF_TRY(carve_from_levels(ctx, g, rec, rec__, (const int32_t*)lv, (const int32_t*)fin, rec_, snode, outlet,
+                            nullptr, tag, npass, iters));
+  if (tag_ && tag)
+    PFF_TRY(check_hip(hipMemcpyAsync(tag_, tag, (size_t)g.N, hipMemcpyDeviceToDevice, ctx->stream), "copy"));
+  return 0;
+}
+
+}  // namespace pff

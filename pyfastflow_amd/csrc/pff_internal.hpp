@@ -1,0 +1,110 @@
+// pff_internal.hpp -- host-side internals of libpff_hip.so: the context, error
+// reporting and the launchers each kernel file exports to the C ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/pff.h"
+#include "pff_common.hpp"
+
+struct pff_ctx {
+  int32_t device = 0;
+  hipStream_t stream = nullptr;
+  // growable device scratch, keyed by slot
+  static constexpr int kSlots = 16;
+  void* buf[kSlots] = {};
+  size_t cap[kSlots] = {};
+  // pinned host words for counter readback
+  int64_t* host_words = nullptr;
+  // device words: counters and flags
+  int64_t* dev_words = nullptr;
+};
+
+namespace pff {
+
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int check_hip(hipError_t e, const char* what);
+int check_grid(const pff_grid* g, GridD* out);
+// device scratch slot of at least `bytes`
+int scratch(pff_ctx* ctx, int slot, size_t bytes, void** out, bool zero_new = false);
+
+// scratch slot map (each slot is one growable buffer)
+enum Slot : int {
+  S_I0 = 0, S_I1, S_I2, S_I3, S_I4, S_I5,  // int32 / uint32 N-arrays
+  S_F0, S_F1,                              // float N-arrays
+  S_WL0, S_WL1,                            // worklists
+  S_B0,                                    // byte N-array
+  S_LEVELS,                                // pointer-jumping levels
+  S_MISC,
+  S_TAG                                    // carve tag words, kept all-zero between calls
+};
+
+// device word indices (dev_words / host_words)
+enum Word : int { W_COUNT0 = 0, W_COUNT1, W_COUNT2, W_FLAG, W_FLAG2, W_CYCLE, W_NWORDS = 16 };
+
+constexpr int kBlock = 256;
+inline unsigned grid_for(int64_t n, int block = kBlock, int64_t cap = 1 << 20) {
+  int64_t b = (n + block - 1) / block;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return (unsigned)b;
+}
+// fixed grid for grid-stride kernels over a device-side count
+inline unsigned stride_grid() { return 2048; }
+
+// ---- launchers (defined in k_*.hip) -----------------------------------------
+int launch_receivers(pff_ctx* ctx, const GridD& g, const float* z, int32_t* rcv, float* grad);
+
+int launch_rcv2donor(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int32_t* dnr, int32_t* ndnr);
+int launch_rake_compress(pff_ctx* ctx, const GridD& g, int32_t* dnr, int32_t* ndnr, float* p, int32_t* src,
+                         int32_t* dnr_, int32_t* ndnr_, float* p_, int32_t iteration);
+int launch_fuse(pff_ctx* ctx, const GridD& g, float* A, const int32_t* src, const float* B, int32_t iteration);
+// exact uint32 upstream counts; *cycle (host) set when a cycle blocks completion
+int upstream_count(pff_ctx* ctx, const GridD& g, const int32_t* rcv, uint32_t* cnt, int32_t* cycle);
+int launch_scale_counts(pff_ctx* ctx, const GridD& g, const uint32_t* cnt, float q0, float* Q);
+
+int depression_count(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int64_t* out);
+int launch_basin_id_init(pff_ctx* ctx, const GridD& g, int32_t* bid);
+int launch_propagate_basin(pff_ctx* ctx, const GridD& g, const int32_t* bid_in, const int32_t* rec_in,
+                           int32_t* bid_out, int32_t* rec_out);
+// bid = init(root(rec)) with ceil(log2 N)+1 synchronous passes' semantics;
+// rec_out (nullable) receives the final jumped pointers
+int basin_labels(pff_ctx* ctx, const GridD& g, const int32_t* rec, int32_t* bid, int32_t* rec_out);
+int launch_saddlesort(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t* is_border, float* zp,
+                      int64_t* saddle, int32_t* snode, int64_t* outlet, const float* z);
+int launch_reroute_jump(pff_ctx* ctx, const GridD& g, int32_t* rec, const int64_t* outlet);
+// reference-shaped carve (lakeflow.py:405-438)
+int reroute_carve(pff_ctx* ctx, const GridD& g, int32_t* rec, int32_t* rec_, int32_t* rec__, uint8_t* tag,
+                  uint8_t* tag_, const int32_t* snode, const int64_t* outlet, int32_t* iters);
+int npass_for(int32_t N);
+int labels_with_levels(pff_ctx* ctx, const GridD& g, const int32_t* rec, int32_t* bid, int32_t* Pfinal,
+                       int32_t* levels, int32_t* fin);
+int carve_from_levels(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t* rec_in, const int32_t* levels,
+                      const int32_t* fin, const int32_t* Pfinal, const int32_t* snode, const int64_t* outlet,
+                      int32_t* rec_work, uint8_t* tag_bytes, int npass, int32_t* iters);
+
+int fill_passes(pff_ctx* ctx, const GridD& g, const float* zh, float* z_, int32_t* r0, int32_t* r1, double eps,
+                int first_exp);
+int launch_apply_fill(pff_ctx* ctx, const GridD& g, float* zh, float* h, const float* z_);
+int launch_diffuse(pff_ctx* ctx, const GridD& g, const pff_hydro* p, const float* zh, const float* Q, float* temp);
+int launch_manning(pff_ctx* ctx, const GridD& g, const pff_hydro* p, float* h, float* zh, float* dh,
+                   const int32_t* rec, const float* Q);
+
+int launch_edge_maps(pff_ctx* ctx, const GridD& g, uint8_t* edges, uint8_t* out);
+int launch_pack(pff_ctx* ctx, int64_t n, int64_t* arr, const float* ft, const int32_t* it);
+int launch_unpack(pff_ctx* ctx, int64_t n, const int64_t* arr, float* ft, int32_t* it);
+int launch_swap(pff_ctx* ctx, int64_t n, uint32_t* a, uint32_t* b);
+int launch_arange(pff_ctx* ctx, int64_t n, int32_t* a);
+int launch_fill_u32(pff_ctx* ctx, int64_t n, uint32_t* a, uint32_t v);
+int launch_fill_i64(pff_ctx* ctx, int64_t n, int64_t* a, int64_t v);
+
+}  // namespace pff
+
+#define PFF_TRY(expr)          \
+  do {                         \
+    int _rc = (expr);          \
+    if (_rc != 0) return _rc;  \
+  } while (0)

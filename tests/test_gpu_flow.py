@@ -1,0 +1,208 @@
+"""GPU parity of the flow path (receivers, accumulation, lakeflow) against the
+oracle, through the C ABI (pyfastflow_amd -> libpff_hip.so).
+
+Integer outputs must be bit-exact; Q must equal f32(q0) * exact upstream
+count, which is within 1e-5 relative of the FP64 oracle.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from pyfastflow_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+MODES = ["normal", "periodic_EW", "periodic_NS"]
+
+# (ny, nx, mode, n_pits, seed)
+SMALL = [
+    (2, 2, "normal", 0, 1),
+    (5, 7, "normal", 0, 2),
+    (7, 5, "periodic_EW", 0, 3),
+    (9, 11, "periodic_NS", 0, 4),
+    (48, 64, "normal", 3, 5),
+    (64, 64, "periodic_EW", 4, 6),
+    (65, 33, "periodic_NS", 2, 7),
+    (128, 96, "normal", 10, 8),
+    (256, 256, "periodic_EW", 40, 9),
+]
+
+
+def _dem(ny, nx, seed, pits, mode):
+    return synth.dem(ny, nx, 2.0, 200.0, seed, n_pits=pits, dx=10.0, periodic_ew=(mode == "periodic_EW"))
+
+
+def _router(pf, nx, ny, dx, mode, z, boundaries=None):
+    r = pf.flow.FlowRouter(nx, ny, dx, boundary_mode=mode, boundaries=boundaries)
+    r.set_z(z)
+    return r
+
+
+def _ograd(ora, nx, ny, dx, mode, boundaries=None):
+    return ora.Grid(nx, ny, dx, mode, boundaries)
+
+
+@pytest.mark.parametrize("ny,nx,mode,pits,seed", SMALL)
+def test_receivers_bit_exact(pf, ora, ny, nx, mode, pits, seed):
+    print(f"case receivers {ny}x{nx} {mode}", flush=True)
+    z = _dem(ny, nx, seed, pits, mode)
+    r = _router(pf, nx, ny, 10.0, mode, z)
+    r.compute_receivers()
+    g = _ograd(ora, nx, ny, 10.0, mode)
+    rcv, grad = ora.compute_receivers(g, z)
+    np.testing.assert_array_equal(r.receivers.to_numpy(), rcv)
+    np.testing.assert_array_equal(r.gradient.to_numpy().view(np.uint32), grad.view(np.uint32))
+
+
+def test_receivers_ties_and_flats(pf, ora):
+    """Flat areas drain to themselves; equal drops pick the first direction."""
+    z = np.zeros((6, 8), np.float32)
+    z[2:4, 3:5] = 1.0
+    r = _router(pf, 8, 6, 1.0, "normal", z)
+    r.compute_receivers()
+    rcv, _ = ora.compute_receivers(_ograd(ora, 8, 6, 1.0, "normal"), z)
+    np.testing.assert_array_equal(r.receivers.to_numpy(), rcv)
+
+
+def test_receivers_custom_boundaries(pf, ora):
+    ny, nx = 32, 40
+    z = _dem(ny, nx, 11, 2, "normal")
+    b = np.ones((ny, nx), np.uint8)
+    b[0, :] = 3
+    b[-1, :] = 3
+    b[:, 0] = 9
+    b[:, -1] = 9
+    b[10:14, 10:14] = 0
+    b[20, 5] = 7
+    r = _router(pf, nx, ny, 10.0, "custom", z, boundaries=b)
+    r.compute_receivers()
+    rcv, grad = ora.compute_receivers(_ograd(ora, nx, ny, 10.0, "custom", b), z)
+    np.testing.assert_array_equal(r.receivers.to_numpy(), rcv)
+    np.testing.assert_array_equal(r.gradient.to_numpy(), grad)
+
+
+@pytest.mark.parametrize("ny,nx,mode,pits,seed", SMALL)
+def test_accumulate_exact(pf, ora, ny, nx, mode, pits, seed):
+    print(f"case accumulate {ny}x{nx} {mode}", flush=True)
+    z = _dem(ny, nx, seed, pits, mode)
+    r = _router(pf, nx, ny, 10.0, mode, z)
+    r.compute_receivers()
+    assert r.accumulate_constant_Q(1e-5, area=True)
+    g = _ograd(ora, nx, ny, 10.0, mode)
+    rcv, _ = ora.compute_receivers(g, z)
+    cnt = ora.upstream_count(g, rcv)
+    q0 = np.float32(1e-5 * 10.0 * 10.0)
+    expect = (q0 * cnt.astype(np.float32)).astype(np.float32)
+    np.testing.assert_array_equal(r.Q.to_numpy(), expect)
+    ref64 = ora.accumulate_f64(g, rcv, np.full(g.N, float(q0)))
+    rel = np.abs(r.Q.to_numpy() - ref64) / np.abs(ref64)
+    assert rel.max() <= 1e-5
+
+
+def test_accumulate_unit_area_matches_reference_schedule(pf, ora):
+    """The 512^2 plain workload: Q (dx=1, value 1) is integer-valued and must
+    equal the reference's own rake-and-compress schedule exactly."""
+    c = synth.plain_small()
+    r = _router(pf, c["nx"], c["ny"], c["dx"], "normal", c["z"])
+    r.compute_receivers()
+    assert r.accumulate_constant_Q(1.0, area=True)
+    g = _ograd(ora, c["nx"], c["ny"], c["dx"], "normal")
+    rcv, _ = ora.compute_receivers(g, c["z"])
+    Qref, done = ora.accumulate_constant_Q(g, rcv, 1.0, True)
+    assert done
+    np.testing.assert_array_equal(r.Q.to_numpy(), Qref)
+    # the GPU run of the reference schedule agrees as well
+    r.accumulate_constant_Q_rake_compress(1.0, area=True)
+    np.testing.assert_array_equal(r.Q.to_numpy(), Qref)
+
+
+@pytest.mark.parametrize("ny,nx,mode,pits,seed", SMALL)
+@pytest.mark.parametrize("carve", [True, False])
+def test_reroute_flow_bit_exact(pf, ora, ny, nx, mode, pits, seed, carve):
+    print(f"case reroute {ny}x{nx} {mode} carve={carve}", flush=True)
+    z = _dem(ny, nx, seed, pits, mode)
+    r = _router(pf, nx, ny, 10.0, mode, z)
+    r.compute_receivers()
+    bf = torch.zeros(nx * ny, dtype=torch.int32, device="cuda")
+    st = r.reroute_flow(carve=carve, bid_first=bf)
+    g = _ograd(ora, nx, ny, 10.0, mode)
+    rcv, _ = ora.compute_receivers(g, z)
+    ref = ora.reroute_flow(g, z, rcv, carve=carve)
+    assert st.ndep0 == ref["ndep0"]
+    assert st.n_iter == ref["n_iter"]
+    np.testing.assert_array_equal(r.receivers.to_numpy(), ref["rec"])
+    np.testing.assert_array_equal(r.receivers_.to_numpy(), ref["rec_"])
+    if ref["ndep0"] > 0:
+        np.testing.assert_array_equal(bf.cpu().numpy(), ref["bid_first"])
+
+
+def test_saddlesort_module_bit_exact(pf, ora):
+    """Module-level saddlesort on first-pass labels: z', borders, saddle keys,
+    saddle nodes and outlets per basin."""
+    ny, nx, mode = 96, 80, "periodic_EW"
+    z = _dem(ny, nx, 21, 8, mode)
+    r = _router(pf, nx, ny, 10.0, mode, z)
+    r.compute_receivers()
+    pf.flow.basin_identification(r.bid, r.receivers, r.receivers_, nx * ny)
+    pf.flow.saddlesort(r.bid, r.is_border, r.z_, r.basin_saddle, r.basin_saddlenode, r.outlet, r.z)
+    g = _ograd(ora, nx, ny, 10.0, mode)
+    rcv, _ = ora.compute_receivers(g, z)
+    bid = ora.basin_identification(g, rcv)
+    np.testing.assert_array_equal(r.bid.to_numpy(), bid)
+    s = ora.saddlesort(g, bid, z)
+    np.testing.assert_array_equal(r.z_.to_numpy(), s["z_prime"])
+    np.testing.assert_array_equal(r.is_border.to_numpy(), s["is_border"])
+    np.testing.assert_array_equal(r.basin_saddle.to_numpy(), s["basin_saddle"])
+    np.testing.assert_array_equal(r.basin_saddlenode.to_numpy(), s["saddlenode"])
+    np.testing.assert_array_equal(r.outlet.to_numpy(), s["outlet"])
+
+
+def test_depressions_1024_full_pipeline(pf, ora):
+    """The depressions workload at 1024^2 (periodic east-west, pits):
+    receivers, lakeflow and Q against the oracle."""
+    c = synth.depressions(n=1024, n_pits=125)
+    nx, ny, dx = c["nx"], c["ny"], c["dx"]
+    r = _router(pf, nx, ny, dx, c["boundary_mode"], c["z"])
+    r.compute_receivers()
+    st = r.reroute_flow()
+    assert r.accumulate_constant_Q(c["precipitation"], area=True)
+    g = _ograd(ora, nx, ny, dx, c["boundary_mode"])
+    rcv, _ = ora.compute_receivers(g, c["z"])
+    ref = ora.reroute_flow(g, c["z"], rcv)
+    assert st.ndep0 == ref["ndep0"] and st.n_iter == ref["n_iter"]
+    np.testing.assert_array_equal(r.receivers.to_numpy(), ref["rec"])
+    assert ora.depression_counter(g, ref["rec"]) == 0
+    cnt = ora.upstream_count(g, ref["rec"])
+    q0 = np.float32(c["precipitation"] * dx * dx)
+    np.testing.assert_array_equal(r.Q.to_numpy(), q0 * cnt.astype(np.float32))
+
+
+def test_pack_unpack_gpu(pf, ora):
+    rng = np.random.default_rng(42)
+    f = rng.uniform(-1e6, 1e6, 10000).astype(np.float32)
+    f[:4] = [0.0, -0.0, np.inf, -np.inf]
+    idx = rng.integers(0, 750000, 10000).astype(np.int32)
+    tf = torch.from_numpy(f).cuda()
+    ti = torch.from_numpy(idx).cuda()
+    keys = torch.zeros(10000, dtype=torch.int64, device="cuda")
+    pf.flow.pack_full_float_index(keys, tf, ti)
+    np.testing.assert_array_equal(keys.cpu().numpy(), ora.pack_array(f, idx))
+    f2 = torch.zeros_like(tf)
+    i2 = torch.zeros_like(ti)
+    pf.flow.unpack_full_float_index(keys, f2, i2)
+    np.testing.assert_array_equal(f2.cpu().numpy().view(np.uint32), f.view(np.uint32))
+    np.testing.assert_array_equal(i2.cpu().numpy(), idx)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_edge_maps(pf, ora, mode):
+    ny, nx = 6, 9
+    pf.flow.FlowRouter(nx, ny, 1.0, boundary_mode=mode)
+    e = torch.zeros(nx * ny, dtype=torch.uint8, device="cuda")
+    o = torch.zeros_like(e)
+    pf.flow.fill_edges(e)
+    pf.flow.flow_out_nodes(o)
+    g = _ograd(ora, nx, ny, 1.0, mode)
+    np.testing.assert_array_equal(e.cpu().numpy(), [ora.which_edge(g, i) for i in range(g.N)])
+    np.testing.assert_array_equal(o.cpu().numpy(), [int(ora.can_leave(g, i)) for i in range(g.N)])
