@@ -176,7 +176,10 @@ int pff_upstream_count(pff_ctx* ctx, const pff_grid* g, const int32_t* rcv, uint
   PFF_TRY(check_grid(g, &d));
   PFF_TRY(need(rcv, "rcv"));
   PFF_TRY(need(count, "count"));
-  return upstream_count(ctx, d, rcv, count, nullptr);
+  int32_t cycle = 0;
+  PFF_TRY(tiled_upstream(ctx, d, rcv, 1, nullptr, nullptr, 0.0f, nullptr, count, nullptr, &cycle));
+  if (cycle) PFF_TRY(upstream_count(ctx, d, rcv, count, nullptr));
+  return 0;
 }
 
 int pff_accumulate_constant_Q(pff_ctx* ctx, const pff_grid* g, const int32_t* rcv, double value, int32_t area,
@@ -189,11 +192,16 @@ int pff_accumulate_constant_Q(pff_ctx* ctx, const pff_grid* g, const int32_t* rc
   // Q.fill(value * (DX*DX if area else 1)) evaluates in double, stored as f32
   // (flowfields.py:279)
   const float q0 = (float)(value * (area ? (double)g->dx * (double)g->dx : 1.0));
-  void* cnt;
-  PFF_TRY(scratch(ctx, S_I5, (size_t)d.N * 4, &cnt));
   int32_t cycle = 0;
-  PFF_TRY(upstream_count(ctx, d, rcv, (uint32_t*)cnt, converged ? &cycle : nullptr));
-  PFF_TRY(launch_scale_counts(ctx, d, (const uint32_t*)cnt, q0, Q));
+  PFF_TRY(tiled_upstream(ctx, d, rcv, 1, nullptr, nullptr, q0, Q, nullptr, nullptr, &cycle));
+  if (cycle) {
+    // not a forest: exact compacted pointer jumping, reported as unconverged
+    void* cnt;
+    PFF_TRY(scratch(ctx, S_I5, (size_t)d.N * 4, &cnt));
+    int32_t cyc2 = 0;
+    PFF_TRY(upstream_count(ctx, d, rcv, (uint32_t*)cnt, &cyc2));
+    PFF_TRY(launch_scale_counts(ctx, d, (const uint32_t*)cnt, q0, Q));
+  }
   if (converged) *converged = cycle ? 0 : 1;
   return 0;
 }
@@ -284,46 +292,79 @@ int pff_reroute_flow(pff_ctx* ctx, const pff_grid* g, int32_t* bid, int32_t* rec
   if (st) std::memset(st, 0, sizeof(*st));
   // rec_ <- rec (lakeflow.py:491)
   PFF_TRY(check_hip(hipMemcpyAsync(rec_, rec, n4, hipMemcpyDeviceToDevice, s), "copy"));
-  int64_t ndep = 0;
-  PFF_TRY(depression_count(ctx, d, rec, &ndep));  // :493
-  if (st) st->ndep0 = ndep;
-  if (ndep == 0) return 0;  // :496-497 (bid untouched)
-  const int nouter = ceil_log2(ndep) + 1;  // :500
   const int npass = npass_for(d.N);
+  // The tiled contraction is used while the receivers form a forest; if a
+  // cycle appears (possible only through rerouting, e.g. a basin-graph cycle
+  // longer than 2) every later step runs the exact synchronous semantics.
+  bool exact = !carve;  // jump mode can leave non-forest receivers: keep exact semantics
   int32_t *levels = nullptr, *fin = nullptr;
-  if (carve) {
+  auto ensure_levels = [&]() -> int {
+    if (levels) return 0;
     void *lv, *fv;
     PFF_TRY(scratch(ctx, S_LEVELS, (size_t)(npass + 1) * n4, &lv));
     PFF_TRY(scratch(ctx, S_I4, n4, &fv));
     levels = (int32_t*)lv;
     fin = (int32_t*)fv;
-  }
+    return 0;
+  };
+  int nouter = 1;
+  int64_t ndep0 = 0;
   for (int it = 0; it < nouter; ++it) {
-    int64_t ndb = ndep;
-    if (it > 0) PFF_TRY(depression_count(ctx, d, rec_, &ndb));  // :501
+    int64_t ndb = 0;
+    bool labelled = false;
+    if (!exact) {
+      int32_t cyc = 0;
+      PFF_TRY(tiled_labels(ctx, d, rec_, bid, &ndb, &cyc, /*write=*/false));
+      if (cyc) exact = true;
+    }
+    if (exact) {
+      PFF_TRY(depression_count(ctx, d, rec_, &ndb));
+    }
+    if (it == 0) {
+      ndep0 = ndb;  // lakeflow.py:493 (rec_ == rec here)
+      if (st) st->ndep0 = ndb;
+      if (ndb == 0) return 0;  // :496-497, bid untouched
+      nouter = ceil_log2(ndb) + 1;  // :500
+    }
     if (st) {
-      if (st->n_loop < 64) st->ndep[st->n_loop] = ndb;
+      if (st->n_loop < 64) st->ndep[st->n_loop] = ndb;  // :501
       st->n_loop++;
     }
-    // basin labels on rec_ (:507-511); rec__ ends with the jumped pointers
-    if (carve) PFF_TRY(labels_with_levels(ctx, d, rec_, bid, rec__, levels, fin));
-    else PFF_TRY(basin_labels(ctx, d, rec_, bid, rec__));
+    // basin labels on rec_ (:507-511)
+    if (!exact) {
+      PFF_TRY(tiled_labels_write(ctx, d, bid));
+    } else if (carve) {
+      PFF_TRY(ensure_levels());
+      PFF_TRY(labels_with_levels(ctx, d, rec_, bid, rec__, levels, fin));
+    } else {
+      PFF_TRY(basin_labels(ctx, d, rec_, bid, rec__));
+    }
+    labelled = true;
+    (void)labelled;
     if (it == 0 && bid_first)
       PFF_TRY(check_hip(hipMemcpyAsync(bid_first, bid, n4, hipMemcpyDeviceToDevice, s), "copy"));
     if (ndb == 0) break;  // :512-514
     PFF_TRY(launch_saddlesort(ctx, d, bid, is_border, z_prime, basin_saddle, basin_saddlenode, outlet, z));  // :520
     if (st) st->n_iter++;
     if (carve) {
-      // rec equals rec_ here, so the carve edits rec in place and copies the
-      // edited entries back to rec_ (:525-526)
-      int32_t iters = 0;
-      PFF_TRY(carve_from_levels(ctx, d, rec, rec_, levels, fin, rec__, basin_saddlenode, outlet, rec_, nullptr, npass,
-                                &iters));
-      if (st && it < 64) st->carve_iters[it] = iters;
+      // rec equals rec_ here: the carve edits rec and copies it to rec_ (:525-526)
+      int32_t cyc = 0;
+      if (!exact) {
+        PFF_TRY(carve_tiled(ctx, d, rec, rec_, bid, basin_saddlenode, outlet, rec_, &cyc));
+        if (cyc) exact = true;  // unreachable: labels already proved rec_ a forest
+      }
+      if (exact) {
+        PFF_TRY(ensure_levels());
+        if (cyc) PFF_TRY(labels_with_levels(ctx, d, rec_, bid, rec__, levels, fin));
+        int32_t iters = 0;
+        PFF_TRY(carve_from_levels(ctx, d, rec, rec_, levels, fin, rec__, basin_saddlenode, outlet, rec_, nullptr,
+                                  npass, &iters));
+      }
     } else {
       PFF_TRY(launch_reroute_jump(ctx, d, rec_, outlet));  // :529
     }
   }
+  (void)ndep0;
   return launch_basin_id_init(ctx, d, bid);  // :537
 }
 

@@ -141,79 +141,6 @@ int launch_fuse(pff_ctx* ctx, const GridD& g, float* A, const int32_t* src, cons
   return check_hip(hipGetLastError(), "fuse");
 }
 
-// ---------------------------------------------------------------------------
-// Exact upstream counts: pointer-jumping scatter over a compacted list of
-// live nodes.  After round k, S(i) = #{j : j reaches i in < 2^k steps};
-// round k adds S(m) into the node exactly 2^k steps below m (P(m)), then
-// P(m) <- P(P(m)).  A node leaves the list once its 2^k-step successor does
-// not exist.  Integer atomics make the result independent of the order.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void wave_append(int32_t* list, int64_t* count, int32_t value, bool take) {
-  const uint64_t mask = __ballot(take);
-  if (mask == 0) return;
-  const int lane = __lane_id();
-  const int leader = __ffsll((long long)mask) - 1;
-  int base = 0;
-  if (lane == leader) base = (int)atomicAdd((unsigned long long*)count, (unsigned long long)__popcll(mask));
-  base = __shfl(base, leader);
-  if (take) {
-    const int off = __popcll(mask & ((1ull << lane) - 1ull));
-    list[base + off] = value;
-  }
-}
-
-// init: S = 1, P = rcv or -1 at roots, live list = non-roots
-__global__ void k_count_init(int32_t N, const int32_t* __restrict__ rcv, uint32_t* __restrict__ S,
-                             int32_t* __restrict__ P, int32_t* __restrict__ live, int64_t* count) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  bool take = false;
-  if (t < N) {
-    const int r = rcv[t];
-    const bool root = (r == t) || (unsigned)r >= (unsigned)N;  // out-of-range receivers act as roots
-    S[t] = 1u;
-    P[t] = root ? -1 : r;
-    take = !root;
-  }
-  wave_append(live, count, t, take);
-}
-
-// phase 1: read S and the next pointer of every live node (pre-round values)
-__global__ void k_count_gather(const int32_t* __restrict__ live, const int64_t* count, const uint32_t* __restrict__ S,
-                               const int32_t* __restrict__ P, int32_t* __restrict__ tgt, uint32_t* __restrict__ val,
-                               int32_t* __restrict__ nxt, int64_t* next_count) {
-  const int64_t n = *count;
-  if (blockIdx.x == 0 && threadIdx.x == 0) *next_count = 0;
-  for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < n; w += (int64_t)gridDim.x * blockDim.x) {
-    const int m = live[w];
-    const int pm = P[m];
-    tgt[w] = pm;
-    val[w] = S[m];
-    nxt[w] = P[pm];
-  }
-}
-
-// phase 2: scatter-add, advance pointers, compact the list
-__global__ void k_count_scatter(const int32_t* __restrict__ live, const int64_t* count, uint32_t* S, int32_t* P,
-                                const int32_t* __restrict__ tgt, const uint32_t* __restrict__ val,
-                                const int32_t* __restrict__ nxt, int32_t* __restrict__ live_out, int64_t* next_count) {
-  const int64_t n = *count;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  // iterate in whole-wave steps so wave_append sees converged lanes
-  const int64_t nr = (n + 63) / 64 * 64;
-  for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nr; w += stride) {
-    bool take = false;
-    int m = 0;
-    if (w < n) {
-      m = live[w];
-      atomicAdd(&S[tgt[w]], val[w]);
-      const int q = nxt[w];
-      P[m] = q;
-      take = (q >= 0);
-    }
-    wave_append(live_out, next_count, m, take);
-  }
-}
-
 __global__ void k_scale_counts(int32_t N, const uint32_t* __restrict__ cnt, float q0, float* __restrict__ Q) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < N) Q[t] = q0 * (float)cnt[t];
@@ -222,44 +149,6 @@ __global__ void k_scale_counts(int32_t N, const uint32_t* __restrict__ cnt, floa
 int launch_scale_counts(pff_ctx* ctx, const GridD& g, const uint32_t* cnt, float q0, float* Q) {
   hipLaunchKernelGGL(k_scale_counts, dim3(grid_for(g.N)), dim3(kBlock), 0, ctx->stream, g.N, cnt, q0, Q);
   return check_hip(hipGetLastError(), "scale_counts");
-}
-
-int upstream_count(pff_ctx* ctx, const GridD& g, const int32_t* rcv, uint32_t* cnt, int32_t* cycle) {
-  const size_t n4 = (size_t)g.N * 4;
-  void *P, *L0, *L1, *tgt, *val, *nxt;
-  PFF_TRY(scratch(ctx, S_I0, n4, &P));
-  PFF_TRY(scratch(ctx, S_WL0, n4, &L0));
-  PFF_TRY(scratch(ctx, S_WL1, n4, &L1));
-  PFF_TRY(scratch(ctx, S_I1, n4, &tgt));
-  PFF_TRY(scratch(ctx, S_I2, n4, &val));
-  PFF_TRY(scratch(ctx, S_I3, n4, &nxt));
-  int64_t* c0 = ctx->dev_words + W_COUNT0;
-  int64_t* c1 = ctx->dev_words + W_COUNT1;
-  PFF_TRY(check_hip(hipMemsetAsync(c0, 0, sizeof(int64_t), ctx->stream), "memset"));
-  hipLaunchKernelGGL(k_count_init, dim3(grid_for(g.N)), dim3(kBlock), 0, ctx->stream, g.N, rcv, cnt, (int32_t*)P,
-                     (int32_t*)L0, c0);
-  // a path has at most N-1 links: ceil(log2 N)+1 rounds always finish a forest
-  int rounds = 1;
-  while ((1ll << (rounds - 1)) < (int64_t)g.N) ++rounds;
-  int32_t* la = (int32_t*)L0;
-  int32_t* lb = (int32_t*)L1;
-  int64_t* ca = c0;
-  int64_t* cb = c1;
-  for (int k = 0; k < rounds; ++k) {
-    hipLaunchKernelGGL(k_count_gather, dim3(stride_grid()), dim3(kBlock), 0, ctx->stream, la, ca, cnt, (int32_t*)P,
-                       (int32_t*)tgt, (uint32_t*)val, (int32_t*)nxt, cb);
-    hipLaunchKernelGGL(k_count_scatter, dim3(stride_grid()), dim3(kBlock), 0, ctx->stream, la, ca, cnt, (int32_t*)P,
-                       (int32_t*)tgt, (uint32_t*)val, (int32_t*)nxt, lb, cb);
-    int32_t* tl = la; la = lb; lb = tl;
-    int64_t* tc = ca; ca = cb; cb = tc;
-  }
-  PFF_TRY(check_hip(hipGetLastError(), "upstream_count"));
-  if (cycle) {
-    PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words, ca, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream), "d2h"));
-    PFF_TRY(check_hip(hipStreamSynchronize(ctx->stream), "sync"));
-    *cycle = ctx->host_words[0] != 0;
-  }
-  return 0;
 }
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,660 @@
+// k_forest.hip -- receiver-forest primitives on gfx950: roots (basin labels)
+// and upstream sums (drainage area, carve path tags).
+//
+// Two implementations share the same semantics on forests:
+//
+//  * global compacted pointer jumping (exact for any receiver graph, including
+//    cycles, where it reproduces ceil(log2 N)+1 synchronous passes);
+//  * tiled contraction (forests only, detects cycles): 64x64 tiles are loaded
+//    into LDS and contracted there by pointer jumping -- every cell learns the
+//    last cell of its path inside the tile ("local end") and its in-tile
+//    upstream sum.  Only local ends whose receiver leaves the tile ("exits",
+//    a few percent of the cells) are then jumped globally, and a second tile
+//    pass folds the inflow of each tile's entry cells back in.  Each cell is
+//    read and written a small constant number of times, instead of once per
+//    global pointer-jumping round.
+#include "pff_block.hpp"
+#include "pff_internal.hpp"
+
+#include <algorithm>
+
+namespace pff {
+
+// ---------------------------------------------------------------------------
+// division by the grid width with a multiply-high (Granlund & Montgomery 1994)
+// ---------------------------------------------------------------------------
+struct FastDiv {
+  uint32_t m, s;  // s >= 1 (divisor >= 2)
+};
+
+static FastDiv make_fastdiv(uint32_t d) {
+  uint32_t s = 0;
+  while ((1ull << s) < d) ++s;
+  const uint64_t m = ((1ull << 32) * ((1ull << s) - d)) / d + 1;
+  return FastDiv{(uint32_t)m, s};
+}
+
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, FastDiv f) {
+  const uint32_t t = __umulhi(n, f.m);
+  return (t + ((n - t) >> 1)) >> (f.s - 1);
+}
+
+// ---------------------------------------------------------------------------
+// Global compacted pointer jumping.  Lists are processed in chunks of
+// kChunk entries per block iteration; appends cost one atomic per chunk.
+// ---------------------------------------------------------------------------
+constexpr int kItems = 16;
+constexpr int kChunk = kItems * kBT;  // 4096
+
+// counts ---------------------------------------------------------------------
+// init from all cells: S = 1, P = rcv (or -1 at roots), list = non-roots
+__global__ __launch_bounds__(kBT) void k_count_init_all(int32_t N, const int32_t* __restrict__ rcv,
+                                                        uint32_t* __restrict__ S, int32_t* __restrict__ P,
+                                                        int32_t* __restrict__ list, int64_t* count) {
+  int32_t val[kItems];
+  bool take[kItems];
+  const int64_t base = (int64_t)blockIdx.x * kChunk;
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const int64_t t = base + j * kBT + threadIdx.x;
+    take[j] = false;
+    val[j] = (int32_t)t;
+    if (t < N) {
+      const int r = rcv[t];
+      const bool root = (r == (int)t) || (unsigned)r >= (unsigned)N;
+      S[t] = 1u;
+      P[t] = root ? -1 : r;
+      take[j] = !root;
+    }
+  }
+  block_append<kItems>(val, take, list, count);
+}
+
+// phase 1 of a round: read pre-round values of each listed node
+__global__ __launch_bounds__(kBT) void k_count_gather2(const int32_t* __restrict__ list, const int64_t* count,
+                                                       const uint32_t* __restrict__ S, const int32_t* __restrict__ P,
+                                                       int32_t* __restrict__ tgt, uint32_t* __restrict__ val,
+                                                       int32_t* __restrict__ nxt, int64_t* next_count) {
+  const int64_t n = *count;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *next_count = 0;
+  for (int64_t w = (int64_t)blockIdx.x * kBT + threadIdx.x; w < n; w += (int64_t)gridDim.x * kBT) {
+    const int m = list[w];
+    const int pm = P[m];
+    tgt[w] = pm;
+    val[w] = S[m];
+    nxt[w] = P[pm];
+  }
+}
+
+// phase 2: scatter-add into the 2^k-th successor, advance, compact
+__global__ __launch_bounds__(kBT) void k_count_scatter2(const int32_t* __restrict__ list, const int64_t* count,
+                                                        uint32_t* S, int32_t* P, const int32_t* __restrict__ tgt,
+                                                        const uint32_t* __restrict__ val,
+                                                        const int32_t* __restrict__ nxt, int32_t* __restrict__ out,
+                                                        int64_t* next_count) {
+  const int64_t n = *count;
+  for (int64_t base = (int64_t)blockIdx.x * kChunk; base < n; base += (int64_t)gridDim.x * kChunk) {
+    int32_t v[kItems];
+    bool take[kItems];
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+      const int64_t w = base + j * kBT + threadIdx.x;
+      take[j] = false;
+      v[j] = 0;
+      if (w < n) {
+        const int m = list[w];
+        atomicAdd(&S[tgt[w]], val[w]);
+        const int q = nxt[w];
+        P[m] = q;
+        v[j] = m;
+        take[j] = q >= 0;
+      }
+    }
+    block_append<kItems>(v, take, out, next_count);
+  }
+}
+
+// labels ---------------------------------------------------------------------
+template <int MODE>
+__global__ __launch_bounds__(kBT) void k_label_init_all(GridD g, const int32_t* __restrict__ rec,
+                                                        int32_t* __restrict__ bid, int32_t* __restrict__ P,
+                                                        int32_t* __restrict__ fin, int32_t* __restrict__ level0,
+                                                        int32_t* __restrict__ list, int64_t* count) {
+  int32_t val[kItems];
+  bool take[kItems];
+  const int64_t base = (int64_t)blockIdx.x * kChunk;
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const int64_t t = base + j * kBT + threadIdx.x;
+    const int i = (int)t;
+    take[j] = false;
+    val[j] = i;
+    if (t < g.N) {
+      int row, col;
+      rc_of(g, i, row, col);
+      bid[i] = can_leave<MODE>(g, i, row, col) ? 0 : i + 1;
+      int r = rec[i];
+      if ((unsigned)r >= (unsigned)g.N) r = i;  // out-of-range receivers act as roots
+      P[i] = r;
+      if (level0) level0[i] = r;
+      take[j] = r != i;
+      if (fin) fin[i] = take[j] ? 0x7fffffff : 0;
+    }
+  }
+  block_append<kItems>(val, take, list, count);
+}
+
+__global__ __launch_bounds__(kBT) void k_label_gather2(const int32_t* __restrict__ list, const int64_t* count,
+                                                       const int32_t* __restrict__ bid, const int32_t* __restrict__ P,
+                                                       int32_t* __restrict__ nb, int32_t* __restrict__ np,
+                                                       int64_t* next_count) {
+  const int64_t n = *count;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *next_count = 0;
+  for (int64_t w = (int64_t)blockIdx.x * kBT + threadIdx.x; w < n; w += (int64_t)gridDim.x * kBT) {
+    const int m = list[w];
+    const int pm = P[m];
+    nb[w] = bid[pm];
+    const int q = P[pm];
+    np[w] = (q == pm) ? ~q : q;  // pm is a fixed point: m finishes this pass
+  }
+}
+
+__global__ __launch_bounds__(kBT) void k_label_scatter2(const int32_t* __restrict__ list, const int64_t* count,
+                                                        int32_t* bid, int32_t* P, const int32_t* __restrict__ nb,
+                                                        const int32_t* __restrict__ np, int32_t* __restrict__ out,
+                                                        int64_t* next_count, int32_t* __restrict__ level_next,
+                                                        int32_t* __restrict__ fin, int32_t round) {
+  const int64_t n = *count;
+  for (int64_t base = (int64_t)blockIdx.x * kChunk; base < n; base += (int64_t)gridDim.x * kChunk) {
+    int32_t v[kItems];
+    bool take[kItems];
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+      const int64_t w = base + j * kBT + threadIdx.x;
+      take[j] = false;
+      v[j] = 0;
+      if (w < n) {
+        const int m = list[w];
+        int q = np[w];
+        const bool done = q < 0;
+        if (done) q = ~q;
+        bid[m] = nb[w];
+        P[m] = q;
+        if (level_next) level_next[m] = q;
+        if (done && fin) fin[m] = round + 1;
+        v[j] = m;
+        take[j] = !done;
+      }
+    }
+    block_append<kItems>(v, take, out, next_count);
+  }
+}
+
+// round drivers: lists ping-pong between la/lb with counts in ca/cb
+struct Lists {
+  int32_t* l[2];
+  int64_t* c[2];
+  int cur = 0;
+};
+
+static unsigned list_grid() { return 1024; }
+
+static int count_rounds(pff_ctx* ctx, Lists& L, uint32_t* S, int32_t* P, int32_t* tgt, uint32_t* val, int32_t* nxt,
+                        int rounds) {
+  for (int k = 0; k < rounds; ++k) {
+    const int a = L.cur, b = 1 - L.cur;
+    hipLaunchKernelGGL(k_count_gather2, dim3(list_grid()), dim3(kBT), 0, ctx->stream, L.l[a], L.c[a], S, P, tgt, val,
+                       nxt, L.c[b]);
+    hipLaunchKernelGGL(k_count_scatter2, dim3(list_grid()), dim3(kBT), 0, ctx->stream, L.l[a], L.c[a], S, P, tgt, val,
+                       nxt, L.l[b], L.c[b]);
+    L.cur = b;
+  }
+  return check_hip(hipGetLastError(), "count rounds");
+}
+
+static int label_rounds_run(pff_ctx* ctx, Lists& L, int32_t* bid, int32_t* P, int32_t* nb, int32_t* np,
+                            int32_t* levels, int32_t N, int32_t* fin, int first_round, int rounds) {
+  for (int k = first_round; k < first_round + rounds; ++k) {
+    const int a = L.cur, b = 1 - L.cur;
+    hipLaunchKernelGGL(k_label_gather2, dim3(list_grid()), dim3(kBT), 0, ctx->stream, L.l[a], L.c[a], bid, P, nb, np,
+                       L.c[b]);
+    int32_t* lvl = levels ? levels + (size_t)(k + 1) * N : nullptr;
+    hipLaunchKernelGGL(k_label_scatter2, dim3(list_grid()), dim3(kBT), 0, ctx->stream, L.l[a], L.c[a], bid, P, nb,
+                       np, L.l[b], L.c[b], lvl, fin, k);
+    L.cur = b;
+  }
+  return check_hip(hipGetLastError(), "label rounds");
+}
+
+static int ceil_log2i(int64_t n) {
+  int r = 0;
+  while ((1ll << r) < n) ++r;
+  return r;
+}
+
+static int read_word(pff_ctx* ctx, const int64_t* dev, int64_t* out) {
+  PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_FLAG2, dev, 8, hipMemcpyDeviceToHost, ctx->stream), "d2h"));
+  PFF_TRY(check_hip(hipStreamSynchronize(ctx->stream), "sync"));
+  *out = ctx->host_words[W_FLAG2];
+  return 0;
+}
+
+// exact upstream counts, any receiver graph; *cycle set when a cycle blocks completion
+int upstream_count(pff_ctx* ctx, const GridD& g, const int32_t* rcv, uint32_t* cnt, int32_t* cycle) {
+  const size_t n4 = (size_t)g.N * 4;
+  void *P, *L0, *L1, *tgt, *val, *nxt;
+  PFF_TRY(scratch(ctx, S_I0, n4, &P));
+  PFF_TRY(scratch(ctx, S_WL0, n4, &L0));
+  PFF_TRY(scratch(ctx, S_WL1, n4, &L1));
+  PFF_TRY(scratch(ctx, S_I1, n4, &tgt));
+  PFF_TRY(scratch(ctx, S_I2, n4, &val));
+  PFF_TRY(scratch(ctx, S_I3, n4, &nxt));
+  Lists L;
+  L.l[0] = (int32_t*)L0;
+  L.l[1] = (int32_t*)L1;
+  L.c[0] = ctx->dev_words + W_COUNT0;
+  L.c[1] = ctx->dev_words + W_COUNT1;
+  PFF_TRY(check_hip(hipMemsetAsync(L.c[0], 0, 8, ctx->stream), "memset"));
+  hipLaunchKernelGGL(k_count_init_all, dim3((g.N + kChunk - 1) / kChunk), dim3(kBT), 0, ctx->stream, g.N, rcv, cnt,
+                     (int32_t*)P, L.l[0], L.c[0]);
+  PFF_TRY(count_rounds(ctx, L, cnt, (int32_t*)P, (int32_t*)tgt, (uint32_t*)val, (int32_t*)nxt,
+                       ceil_log2i(g.N) + 1));
+  if (cycle) {
+    int64_t live = 0;
+    PFF_TRY(read_word(ctx, L.c[L.cur], &live));
+    *cycle = live != 0;
+  }
+  return 0;
+}
+
+// exact labels (ceil(log2 N)+1 synchronous passes' result), optional levels
+int labels_global(pff_ctx* ctx, const GridD& g, const int32_t* rec, int32_t* bid, int32_t* P, int32_t* levels,
+                  int32_t* fin) {
+  const size_t n4 = (size_t)g.N * 4;
+  void *L0, *L1, *nbv, *npv;
+  PFF_TRY(scratch(ctx, S_WL0, n4, &L0));
+  PFF_TRY(scratch(ctx, S_WL1, n4, &L1));
+  PFF_TRY(scratch(ctx, S_I2, n4, &nbv));
+  PFF_TRY(scratch(ctx, S_I3, n4, &npv));
+  Lists L;
+  L.l[0] = (int32_t*)L0;
+  L.l[1] = (int32_t*)L1;
+  L.c[0] = ctx->dev_words + W_COUNT0;
+  L.c[1] = ctx->dev_words + W_COUNT1;
+  PFF_TRY(check_hip(hipMemsetAsync(L.c[0], 0, 8, ctx->stream), "memset"));
+  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_label_init_all<M>, dim3((g.N + kChunk - 1) / kChunk), dim3(kBT), 0,
+                                               ctx->stream, g, rec, bid, P, fin, levels, L.l[0], L.c[0]));
+  return label_rounds_run(ctx, L, bid, P, (int32_t*)nbv, (int32_t*)npv, levels, g.N, fin, 0, ceil_log2i(g.N) + 1);
+}
+
+int basin_labels(pff_ctx* ctx, const GridD& g, const int32_t* rec, int32_t* bid, int32_t* rec_out) {
+  void* P;
+  if (rec_out) {
+    P = rec_out;
+  } else {
+    PFF_TRY(scratch(ctx, S_I4, (size_t)g.N * 4, &P));
+  }
+  return labels_global(ctx, g, rec, bid, (int32_t*)P, nullptr, nullptr);
+}
+
+int labels_with_levels(pff_ctx* ctx, const GridD& g, const int32_t* rec, int32_t* bid, int32_t* Pfinal,
+                       int32_t* levels, int32_t* fin) {
+  return labels_global(ctx, g, rec, bid, Pfinal, levels, fin);
+}
+
+int npass_for(int32_t N) { return ceil_log2i(N) + 1; }
+
+// ---------------------------------------------------------------------------
+// Tiled contraction
+// ---------------------------------------------------------------------------
+constexpr int TS = 64;               // tile side (cells)
+constexpr int TC = TS * TS;          // 4096 cells per tile
+constexpr int CPT = TC / kBT;        // 16 cells per thread
+constexpr uint16_t NONE = 0xFFFF;
+constexpr int kTileRounds = 12;      // 2^12 = TC: every acyclic in-tile path is contracted
+
+enum WMode : int { W_ROOTS = 0, W_UNIT = 1, W_SADDLE = 2 };
+
+struct TileArgs {
+  GridD g;
+  FastDiv fd;
+  const int32_t* rcv;
+  // local pass outputs
+  int32_t* E;        // global local-end of every cell
+  uint32_t* S;       // in-tile sums (later: totals at local ends)
+  uint32_t* inflow;  // inflow at entry cells (zeroed by the local pass)
+  int32_t* P2;       // reduced-graph pointer at local ends
+  int32_t* bid2;     // labels at local ends
+  int32_t* exits;
+  int64_t* n_exits;
+  int64_t* ndep;
+  int64_t* cycle;
+  // saddle weights
+  const int32_t* bid;
+  const int32_t* snode;
+  // final pass outputs
+  float q0;
+  float* Q;
+  uint32_t* cnt;
+  uint8_t* tag;
+};
+
+template <int W>
+__device__ __forceinline__ uint32_t tile_weight(const TileArgs& a, int c) {
+  if constexpr (W == W_UNIT) {
+    return 1u;
+  } else if constexpr (W == W_SADDLE) {
+    const int b = a.bid[c];
+    return (b > 0 && b < a.g.N && a.snode[b] == c) ? 1u : 0u;
+  } else {
+    return 0u;
+  }
+}
+
+// PH 1: local pass; PH 3: final pass (weights += inflow, emit results)
+template <int MODE, int PH, int W>
+__global__ __launch_bounds__(kBT) void k_tile(TileArgs a) {
+  __shared__ uint16_t P[TC];
+  __shared__ uint16_t E[(PH == 1) ? TC : 1];
+  __shared__ uint32_t S[(W != W_ROOTS) ? TC : 1];
+  const GridD& g = a.g;
+  const int x0 = blockIdx.x * TS, y0 = blockIdx.y * TS;
+  const int tw = min(TS, g.nx - x0), th = min(TS, g.ny - y0);
+  int32_t cell[CPT];
+  uint32_t rootmask = 0;
+#pragma unroll
+  for (int j = 0; j < CPT; ++j) {
+    const int li = threadIdx.x + j * kBT;
+    const int ly = li >> 6, lx = li & 63;
+    uint16_t lp = NONE;
+    cell[j] = -1;
+    uint32_t w = 0;
+    if (lx < tw && ly < th) {
+      const int c = (y0 + ly) * g.nx + x0 + lx;
+      cell[j] = c;
+      int r = a.rcv[c];
+      if ((unsigned)r >= (unsigned)g.N) r = c;
+      if (r == c) {
+        rootmask |= 1u << j;
+      } else {
+        const int rr = (int)fdiv((uint32_t)r, a.fd);
+        const int dy = rr - y0, dxx = r - rr * g.nx - x0;
+        if ((unsigned)dy < (unsigned)th && (unsigned)dxx < (unsigned)tw) lp = (uint16_t)(dy * TS + dxx);
+      }
+      if constexpr (W != W_ROOTS) {
+        w = tile_weight<W>(a, c);
+        if constexpr (PH == 3) w += a.inflow[c];
+      }
+    }
+    P[li] = lp;
+    if constexpr (PH == 1) E[li] = (lp == NONE) ? (uint16_t)li : lp;
+    if constexpr (W != W_ROOTS) S[li] = w;
+  }
+  __syncthreads();
+  bool cyc = true;
+  for (int k = 0; k < kTileRounds; ++k) {
+    uint16_t pm[CPT], pp[CPT], ee[CPT];
+    uint32_t sv[CPT];
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+      const int li = threadIdx.x + j * kBT;
+      pm[j] = P[li];
+      pp[j] = NONE;
+      sv[j] = 0;
+      if (pm[j] != NONE) {
+        pp[j] = P[pm[j]];
+        if constexpr (W != W_ROOTS) sv[j] = S[li];
+      }
+      if constexpr (PH == 1) ee[j] = E[E[li]];
+    }
+    __syncthreads();
+    int any = 0;
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+      const int li = threadIdx.x + j * kBT;
+      if (pm[j] != NONE) {
+        if constexpr (W != W_ROOTS) atomicAdd(&S[pm[j]], sv[j]);
+        P[li] = pp[j];
+        any |= (pp[j] != NONE);
+      }
+      if constexpr (PH == 1) E[li] = ee[j];
+    }
+    if (!__syncthreads_or(any)) {
+      cyc = false;
+      break;
+    }
+  }
+  if constexpr (PH == 1) {
+    int32_t ex[CPT];
+    bool take[CPT];
+    int pits = 0;
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+      const int li = threadIdx.x + j * kBT;
+      take[j] = false;
+      ex[j] = cell[j];
+      const int c = cell[j];
+      if (c < 0) continue;
+      const int e = E[li];
+      a.E[c] = (y0 + (e >> 6)) * g.nx + x0 + (e & 63);
+      if constexpr (W != W_ROOTS) {
+        a.S[c] = S[li];
+        a.inflow[c] = 0u;
+      }
+      if (e == li) {  // local end
+        if (rootmask & (1u << j)) {
+          if constexpr (W == W_ROOTS) {
+            const int row = y0 + (li >> 6), col = x0 + (li & 63);
+            const bool out = can_leave<MODE>(g, c, row, col);
+            a.P2[c] = c;
+            a.bid2[c] = out ? 0 : c + 1;
+            pits += out ? 0 : 1;
+          } else {
+            a.P2[c] = -1;
+          }
+        } else {
+          take[j] = true;  // exit: its receiver is in another tile
+        }
+      }
+    }
+    if (cyc && threadIdx.x == 0) atomicOr((unsigned long long*)a.cycle, 1ull);
+    if constexpr (W == W_ROOTS) block_count(pits, a.ndep);
+    block_append<CPT>(ex, take, a.exits, a.n_exits);
+  } else {
+    if (cyc && threadIdx.x == 0) atomicOr((unsigned long long*)a.cycle, 1ull);
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+      const int li = threadIdx.x + j * kBT;
+      const int c = cell[j];
+      if (c < 0) continue;
+      if constexpr (W == W_UNIT) {
+        if (a.cnt) a.cnt[c] = S[li];
+        if (a.Q) a.Q[c] = a.q0 * (float)S[li];
+      } else if constexpr (W == W_SADDLE) {
+        a.tag[c] = S[li] ? 1 : 0;
+      }
+    }
+  }
+}
+
+// reduced graph: exit x points to the local end of the cell it drains into
+__global__ __launch_bounds__(kBT) void k_exit_link(int32_t N, const int32_t* __restrict__ exits, const int64_t* count,
+                                                   const int32_t* __restrict__ rcv, const int32_t* __restrict__ E,
+                                                   int32_t* __restrict__ P2, int32_t* __restrict__ bid2, int copy_list,
+                                                   int32_t* __restrict__ list_out) {
+  const int64_t n = *count;
+  for (int64_t w = (int64_t)blockIdx.x * kBT + threadIdx.x; w < n; w += (int64_t)gridDim.x * kBT) {
+    const int x = exits[w];
+    P2[x] = E[rcv[x]];
+    if (bid2) bid2[x] = 0;
+    if (copy_list) list_out[w] = x;
+  }
+}
+
+// inflow of entry cells: sum of the totals of the exits draining into them
+__global__ __launch_bounds__(kBT) void k_exit_inflow(const int32_t* __restrict__ exits, const int64_t* count,
+                                                     const int32_t* __restrict__ rcv, const uint32_t* __restrict__ S,
+                                                     uint32_t* inflow) {
+  const int64_t n = *count;
+  for (int64_t w = (int64_t)blockIdx.x * kBT + threadIdx.x; w < n; w += (int64_t)gridDim.x * kBT) {
+    const int x = exits[w];
+    atomicAdd(&inflow[rcv[x]], S[x]);
+  }
+}
+
+// final labels: every cell takes the label of its local end
+__global__ __launch_bounds__(kBT) void k_label_from_end(int32_t N, const int32_t* __restrict__ E,
+                                                        const int32_t* __restrict__ bid2, int32_t* __restrict__ bid) {
+  const int i = blockIdx.x * kBT + threadIdx.x;
+  if (i < N) bid[i] = bid2[E[i]];
+}
+
+struct TileBufs {
+  int32_t *E, *P2, *bid2, *exits, *L0, *L1, *t0, *t1, *t2;
+  uint32_t *S, *inflow;
+};
+
+static int tile_bufs(pff_ctx* ctx, const GridD& g, TileBufs& b) {
+  const size_t n4 = (size_t)g.N * 4;
+  void* p;
+  PFF_TRY(scratch(ctx, S_TE, n4, &p)); b.E = (int32_t*)p;
+  PFF_TRY(scratch(ctx, S_TP2, n4, &p)); b.P2 = (int32_t*)p;
+  PFF_TRY(scratch(ctx, S_TBID, n4, &p)); b.bid2 = (int32_t*)p;
+  PFF_TRY(scratch(ctx, S_TEXIT, n4, &p)); b.exits = (int32_t*)p;
+  PFF_TRY(scratch(ctx, S_WL0, n4, &p)); b.L0 = (int32_t*)p;
+  PFF_TRY(scratch(ctx, S_WL1, n4, &p)); b.L1 = (int32_t*)p;
+  PFF_TRY(scratch(ctx, S_I1, n4, &p)); b.t0 = (int32_t*)p;
+  PFF_TRY(scratch(ctx, S_I2, n4, &p)); b.t1 = (int32_t*)p;
+  PFF_TRY(scratch(ctx, S_I3, n4, &p)); b.t2 = (int32_t*)p;
+  PFF_TRY(scratch(ctx, S_TS, n4, &p)); b.S = (uint32_t*)p;
+  PFF_TRY(scratch(ctx, S_TIN, n4, &p)); b.inflow = (uint32_t*)p;
+  return 0;
+}
+
+static TileArgs tile_args(const GridD& g, const TileBufs& b, pff_ctx* ctx, const int32_t* rcv) {
+  TileArgs a{};
+  a.g = g;
+  a.fd = make_fastdiv((uint32_t)g.nx);
+  a.rcv = rcv;
+  a.E = b.E;
+  a.S = b.S;
+  a.inflow = b.inflow;
+  a.P2 = b.P2;
+  a.bid2 = b.bid2;
+  a.exits = b.exits;
+  a.n_exits = ctx->dev_words + W_NEXIT;
+  a.ndep = ctx->dev_words + W_NDEP;
+  a.cycle = ctx->dev_words + W_CYCLE;
+  return a;
+}
+
+static dim3 tile_grid(const GridD& g) { return dim3((g.nx + TS - 1) / TS, (g.ny + TS - 1) / TS); }
+
+// Upstream sums with the tiled contraction.  W_UNIT: counts -> cnt and/or
+// Q = q0*count; W_SADDLE: tag[c] = 1 where a saddle node lies upstream.
+// Returns 1 in *cycle (host) when the graph is not a forest (outputs are then
+// not meaningful and the caller must use the exact path).  Syncs once.
+int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, const int32_t* bid,
+                   const int32_t* snode, float q0, float* Q, uint32_t* cnt, uint8_t* tag, int32_t* cycle) {
+  TileBufs b;
+  PFF_TRY(tile_bufs(ctx, g, b));
+  TileArgs a = tile_args(g, b, ctx, rcv);
+  a.bid = bid;
+  a.snode = snode;
+  a.q0 = q0;
+  a.Q = Q;
+  a.cnt = cnt;
+  a.tag = tag;
+  hipStream_t s = ctx->stream;
+  PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_NEXIT, 0, 8, s), "memset"));
+  PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_CYCLE, 0, 8, s), "memset"));
+  const dim3 tg = tile_grid(g);
+  if (wmode == W_UNIT) hipLaunchKernelGGL((k_tile<0, 1, W_UNIT>), tg, dim3(kBT), 0, s, a);
+  else hipLaunchKernelGGL((k_tile<0, 1, W_SADDLE>), tg, dim3(kBT), 0, s, a);
+  // reduced graph over the exits: P2 (roots = -1), S holds the in-tile sums
+  Lists L;
+  L.l[0] = b.L0;
+  L.l[1] = b.L1;
+  L.c[0] = ctx->dev_words + W_COUNT0;
+  L.c[1] = ctx->dev_words + W_COUNT1;
+  PFF_TRY(check_hip(hipMemcpyAsync(L.c[0], a.n_exits, 8, hipMemcpyDeviceToDevice, s), "copy"));
+  hipLaunchKernelGGL(k_exit_link, dim3(list_grid()), dim3(kBT), 0, s, g.N, b.exits, a.n_exits, rcv, b.E, b.P2,
+                     (int32_t*)nullptr, 1, b.L0);
+  const int cap = ceil_log2i(g.N) + 1;
+  int done_rounds = 0;
+  int64_t live = 1;
+  while (done_rounds < cap) {
+    const int r = std::min(done_rounds == 0 ? 10 : 6, cap - done_rounds);
+    PFF_TRY(count_rounds(ctx, L, b.S, b.P2, b.t0, (uint32_t*)b.t1, b.t2, r));
+    done_rounds += r;
+    PFF_TRY(read_word(ctx, L.c[L.cur], &live));
+    if (live == 0) break;
+  }
+  int64_t cyc = 0;
+  PFF_TRY(read_word(ctx, a.cycle, &cyc));
+  if (live != 0 || cyc != 0) {
+    *cycle = 1;
+    return 0;
+  }
+  *cycle = 0;
+  hipLaunchKernelGGL(k_exit_inflow, dim3(list_grid()), dim3(kBT), 0, s, b.exits, a.n_exits, rcv, b.S, b.inflow);
+  if (wmode == W_UNIT) hipLaunchKernelGGL((k_tile<0, 3, W_UNIT>), tg, dim3(kBT), 0, s, a);
+  else hipLaunchKernelGGL((k_tile<0, 3, W_SADDLE>), tg, dim3(kBT), 0, s, a);
+  return check_hip(hipGetLastError(), "tiled_upstream");
+}
+
+// Labels bid = init(root) with the tiled contraction, plus the depression
+// count (roots that cannot leave).  *ndep, *cycle are host outputs; syncs once
+// or twice.  On *cycle = 1 bid is not written.
+int tiled_labels(pff_ctx* ctx, const GridD& g, const int32_t* rec, int32_t* bid, int64_t* ndep, int32_t* cycle,
+                 bool write) {
+  TileBufs b;
+  PFF_TRY(tile_bufs(ctx, g, b));
+  TileArgs a = tile_args(g, b, ctx, rec);
+  hipStream_t s = ctx->stream;
+  PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_NEXIT, 0, 8, s), "memset"));
+  PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_CYCLE, 0, 8, s), "memset"));
+  PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_NDEP, 0, 8, s), "memset"));
+  const dim3 tg = tile_grid(g);
+  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL((k_tile<M, 1, W_ROOTS>), tg, dim3(kBT), 0, s, a));
+  Lists L;
+  L.l[0] = b.L0;
+  L.l[1] = b.L1;
+  L.c[0] = ctx->dev_words + W_COUNT0;
+  L.c[1] = ctx->dev_words + W_COUNT1;
+  PFF_TRY(check_hip(hipMemcpyAsync(L.c[0], a.n_exits, 8, hipMemcpyDeviceToDevice, s), "copy"));
+  hipLaunchKernelGGL(k_exit_link, dim3(list_grid()), dim3(kBT), 0, s, g.N, b.exits, a.n_exits, rec, b.E, b.P2,
+                     b.bid2, 1, b.L0);
+  const int cap = ceil_log2i(g.N) + 1;
+  int done_rounds = 0;
+  int64_t live = 1;
+  while (done_rounds < cap) {
+    const int r = std::min(done_rounds == 0 ? 10 : 6, cap - done_rounds);
+    PFF_TRY(label_rounds_run(ctx, L, b.bid2, b.P2, b.t0, b.t1, nullptr, g.N, nullptr, done_rounds, r));
+    done_rounds += r;
+    // one readback for the live count, the cycle flag and the depression count
+    PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_FLAG2, L.c[L.cur], 8, hipMemcpyDeviceToHost, s), "d2h"));
+    PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_CYCLE, a.cycle, 8, hipMemcpyDeviceToHost, s), "d2h"));
+    PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_NDEP, a.ndep, 8, hipMemcpyDeviceToHost, s), "d2h"));
+    PFF_TRY(check_hip(hipStreamSynchronize(s), "sync"));
+    live = ctx->host_words[W_FLAG2];
+    if (live == 0 || ctx->host_words[W_CYCLE] != 0) break;
+  }
+  *ndep = ctx->host_words[W_NDEP];
+  if (live != 0 || ctx->host_words[W_CYCLE] != 0) {
+    *cycle = 1;
+    return 0;
+  }
+  *cycle = 0;
+  if (write) return tiled_labels_write(ctx, g, bid);
+  return 0;
+}
+
+int tiled_labels_write(pff_ctx* ctx, const GridD& g, int32_t* bid) {
+  TileBufs b;
+  PFF_TRY(tile_bufs(ctx, g, b));
+  hipLaunchKernelGGL(k_label_from_end, dim3(grid_for(g.N)), dim3(kBT), 0, ctx->stream, g.N, b.E, b.bid2, bid);
+  return check_hip(hipGetLastError(), "tiled_labels_write");
+}
+
+}  // namespace pff

@@ -1,35 +1,24 @@
 // k_lake.hip -- lakeflow on gfx950: depression count, basin labels by
 // compacted pointer jumping, the five saddlesort passes with packed 64-bit
 // atomicMin, and carving by tag doubling over the saved jump levels.
+#include "pff_block.hpp"
 #include "pff_internal.hpp"
 
 namespace pff {
-
-__device__ __forceinline__ void wave_append_l(int32_t* list, int64_t* count, int32_t value, bool take) {
-  const uint64_t mask = __ballot(take);
-  if (mask == 0) return;
-  const int lane = __lane_id();
-  const int leader = __ffsll((long long)mask) - 1;
-  int base = 0;
-  if (lane == leader) base = (int)atomicAdd((unsigned long long*)count, (unsigned long long)__popcll(mask));
-  base = __shfl(base, leader);
-  if (take) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = value;
-}
 
 // ---------------------------------------------------------------------------
 // depression_counter (lakeflow.py:24-44): #{i : rcv[i]==i and not can_leave(i)}
 // ---------------------------------------------------------------------------
 template <int MODE>
-__global__ void k_depcount(GridD g, const int32_t* __restrict__ rcv, int64_t* out) {
+__global__ __launch_bounds__(kBT) void k_depcount(GridD g, const int32_t* __restrict__ rcv, int64_t* out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  bool pit = false;
+  int pit = 0;
   if (i < g.N && rcv[i] == i) {
     int row, col;
     rc_of(g, i, row, col);
-    pit = !can_leave<MODE>(g, i, row, col);
+    pit = can_leave<MODE>(g, i, row, col) ? 0 : 1;
   }
-  const uint64_t mask = __ballot(pit);
-  if (__lane_id() == 0 && mask) atomicAdd((unsigned long long*)out, (unsigned long long)__popcll(mask));
+  block_count(pit, out);
 }
 
 int depression_count(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int64_t* out) {
@@ -79,125 +68,6 @@ int launch_propagate_basin(pff_ctx* ctx, const GridD& g, const int32_t* bid_in, 
   hipLaunchKernelGGL(k_propagate, dim3(grid_for(g.N)), dim3(kBlock), 0, ctx->stream, g.N, bid_in, rec_in, bid_out,
                      rec_out);
   return check_hip(hipGetLastError(), "propagate_basin");
-}
-
-// ---------------------------------------------------------------------------
-// Basin labels by compacted pointer jumping.
-// Semantics: npass = ceil(log2 N)+1 synchronous passes of
-//   bid[i] <- bid[P[i]],  P[i] <- P[P[i]]    starting from P = rec, bid = init.
-// A node whose pointer is a fixed point (a root, P[r]==r) takes the root's
-// final label in the next pass and never changes again, so it leaves the
-// list; nodes still listed get exactly the synchronous update.  Every round
-// optionally saves the jumped pointer of each listed node to LEVEL[k+1]
-// (LEVEL[0] = rec) and the finish round to fin[], which the carve uses.
-// ---------------------------------------------------------------------------
-template <int MODE>
-__global__ void k_label_init(GridD g, const int32_t* __restrict__ rec, int32_t* __restrict__ bid,
-                             int32_t* __restrict__ P, int32_t* __restrict__ fin, int32_t* __restrict__ level0,
-                             int32_t* __restrict__ live, int64_t* count) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  bool take = false;
-  if (i < g.N) {
-    int row, col;
-    rc_of(g, i, row, col);
-    bid[i] = can_leave<MODE>(g, i, row, col) ? 0 : i + 1;
-    int r = rec[i];
-    if ((unsigned)r >= (unsigned)g.N) r = i;  // out-of-range receivers act as roots
-    P[i] = r;
-    if (level0) level0[i] = r;
-    take = (r != i);
-    // finish round: 0 for roots, 'never' until a listed node finishes
-    if (fin) fin[i] = take ? 0x7fffffff : 0;
-  }
-  wave_append_l(live, count, i, take);
-}
-
-__global__ void k_label_gather(const int32_t* __restrict__ live, const int64_t* count, const int32_t* __restrict__ bid,
-                               const int32_t* __restrict__ P, int32_t* __restrict__ nb, int32_t* __restrict__ np,
-                               int64_t* next_count) {
-  const int64_t n = *count;
-  if (blockIdx.x == 0 && threadIdx.x == 0) *next_count = 0;
-  for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < n; w += (int64_t)gridDim.x * blockDim.x) {
-    const int m = live[w];
-    const int pm = P[m];
-    nb[w] = bid[pm];
-    const int q = P[pm];
-    // still live after this pass unless pm is a fixed point (then q == pm)
-    np[w] = (q == pm) ? ~q : q;
-  }
-}
-
-__global__ void k_label_scatter(const int32_t* __restrict__ live, const int64_t* count, int32_t* bid, int32_t* P,
-                                const int32_t* __restrict__ nb, const int32_t* __restrict__ np,
-                                int32_t* __restrict__ live_out, int64_t* next_count, int32_t* __restrict__ level_next,
-                                int32_t* __restrict__ fin, int32_t round) {
-  const int64_t n = *count;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  const int64_t nr = (n + 63) / 64 * 64;
-  for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nr; w += stride) {
-    bool take = false;
-    int m = 0;
-    if (w < n) {
-      m = live[w];
-      int q = np[w];
-      const bool done = q < 0;
-      if (done) q = ~q;
-      bid[m] = nb[w];
-      P[m] = q;
-      if (level_next) level_next[m] = q;
-      if (done && fin) fin[m] = round + 1;
-      take = !done;
-    }
-    wave_append_l(live_out, next_count, m, take);
-  }
-}
-
-static int label_rounds(int32_t N) {
-  int r = 0;
-  while ((1ll << r) < (int64_t)N) ++r;
-  return r + 1;  // ceil(log2 N) + 1 (lakeflow.py:510)
-}
-
-// Labels with optional saved levels.  levels: (npass+1) x N int32 or null;
-// fin: N int32 or null.  P (scratch) ends with the final pointers.
-static int labels_impl(pff_ctx* ctx, const GridD& g, const int32_t* rec, int32_t* bid, int32_t* P, int32_t* levels,
-                       int32_t* fin) {
-  const size_t n4 = (size_t)g.N * 4;
-  void *L0, *L1, *nbv, *npv;
-  PFF_TRY(scratch(ctx, S_WL0, n4, &L0));
-  PFF_TRY(scratch(ctx, S_WL1, n4, &L1));
-  PFF_TRY(scratch(ctx, S_I2, n4, &nbv));
-  PFF_TRY(scratch(ctx, S_I3, n4, &npv));
-  int64_t* c0 = ctx->dev_words + W_COUNT0;
-  int64_t* c1 = ctx->dev_words + W_COUNT1;
-  PFF_TRY(check_hip(hipMemsetAsync(c0, 0, 8, ctx->stream), "memset"));
-  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_label_init<M>, dim3(grid_for(g.N)), dim3(kBlock), 0, ctx->stream, g,
-                                               rec, bid, P, fin, levels, (int32_t*)L0, c0));
-  const int npass = label_rounds(g.N);
-  int32_t* la = (int32_t*)L0;
-  int32_t* lb = (int32_t*)L1;
-  int64_t* ca = c0;
-  int64_t* cb = c1;
-  for (int k = 0; k < npass; ++k) {
-    hipLaunchKernelGGL(k_label_gather, dim3(stride_grid()), dim3(kBlock), 0, ctx->stream, la, ca, bid, P,
-                       (int32_t*)nbv, (int32_t*)npv, cb);
-    int32_t* lvl = levels ? levels + (size_t)(k + 1) * g.N : nullptr;
-    hipLaunchKernelGGL(k_label_scatter, dim3(stride_grid()), dim3(kBlock), 0, ctx->stream, la, ca, bid, P,
-                       (int32_t*)nbv, (int32_t*)npv, lb, cb, lvl, fin, k);
-    int32_t* tl = la; la = lb; lb = tl;
-    int64_t* tc = ca; ca = cb; cb = tc;
-  }
-  return check_hip(hipGetLastError(), "basin_labels");
-}
-
-int basin_labels(pff_ctx* ctx, const GridD& g, const int32_t* rec, int32_t* bid, int32_t* rec_out) {
-  void* P;
-  if (rec_out) {
-    P = rec_out;
-  } else {
-    PFF_TRY(scratch(ctx, S_I4, (size_t)g.N * 4, &P));
-  }
-  return labels_impl(ctx, g, rec, bid, (int32_t*)P, nullptr, nullptr);
 }
 
 // ---------------------------------------------------------------------------
@@ -400,16 +270,24 @@ int launch_reroute_jump(pff_ctx* ctx, const GridD& g, int32_t* rec, const int64_
 // ---------------------------------------------------------------------------
 
 // seed: tag every valid saddle node (init_reroute_carve, :323-330)
-__global__ void k_carve_seed(int32_t N, const int32_t* __restrict__ snode, uint32_t* tagw, int32_t* list,
-                             int64_t* count) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  bool take = false;
-  int s = 0;
-  if (b < N) {
-    s = snode[b];
-    if (s >= 0) take = atomicCAS(&tagw[s], 0u, 1u) == 0u;
+__global__ __launch_bounds__(kBT) void k_carve_seed(int32_t N, const int32_t* __restrict__ snode, uint32_t* tagw,
+                                                    int32_t* list, int64_t* count) {
+  constexpr int IT = 16;
+  int32_t v[IT];
+  bool take[IT];
+  const int64_t base = (int64_t)blockIdx.x * IT * kBT;
+#pragma unroll
+  for (int j = 0; j < IT; ++j) {
+    const int64_t b = base + j * kBT + threadIdx.x;
+    take[j] = false;
+    v[j] = 0;
+    if (b < N) {
+      const int s = snode[b];
+      v[j] = s;
+      if (s >= 0) take[j] = atomicCAS(&tagw[s], 0u, 1u) == 0u;
+    }
   }
-  wave_append_l(list, count, s, take);
+  block_append<IT>(v, take, list, count);
 }
 
 // words: [0] tagged count, [1] count before this round, [2] count before the
@@ -427,22 +305,28 @@ __device__ __forceinline__ int jump_level(const int32_t* levels, const int32_t* 
 }
 
 // round t: every node tagged before the round tags its 2^t-th successor
-__global__ void k_carve_round(int32_t N, const int32_t* __restrict__ levels, const int32_t* __restrict__ fin,
-                              const int32_t* __restrict__ Pfinal, uint32_t* tagw, int32_t* list, int64_t* w,
-                              int32_t t) {
+__global__ __launch_bounds__(kBT) void k_carve_round(int32_t N, const int32_t* __restrict__ levels,
+                                                     const int32_t* __restrict__ fin, const int32_t* __restrict__ Pfinal,
+                                                     uint32_t* tagw, int32_t* list, int64_t* w, int32_t t) {
+  constexpr int IT = 16;
   const int64_t n = w[1];
   if (t > 0 && n == w[2]) return;  // the previous round tagged nothing: converged
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  const int64_t nr = (n + 63) / 64 * 64;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nr; e += stride) {
-    bool take = false;
-    int j = 0;
-    if (e < n) {
-      const int i = list[e];
-      j = jump_level(levels, fin, Pfinal, N, t, i);
-      if (tagw[j] == 0u) take = atomicCAS(&tagw[j], 0u, 1u) == 0u;
+  for (int64_t base = (int64_t)blockIdx.x * IT * kBT; base < n; base += (int64_t)gridDim.x * IT * kBT) {
+    int32_t v[IT];
+    bool take[IT];
+#pragma unroll
+    for (int j = 0; j < IT; ++j) {
+      const int64_t e = base + j * kBT + threadIdx.x;
+      take[j] = false;
+      v[j] = 0;
+      if (e < n) {
+        const int i = list[e];
+        const int jj = jump_level(levels, fin, Pfinal, N, t, i);
+        v[j] = jj;
+        if (tagw[jj] == 0u) take[j] = atomicCAS(&tagw[jj], 0u, 1u) == 0u;
+      }
     }
-    wave_append_l(list, w, j, take);
+    block_append<IT>(v, take, list, w);
   }
 }
 
@@ -496,11 +380,11 @@ int carve_from_levels(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t*
   int64_t* w = ctx->dev_words + W_COUNT0;  // 3 consecutive words
   PFF_TRY(check_hip(hipMemsetAsync(w, 0, 3 * 8, ctx->stream), "memset"));
   hipStream_t s = ctx->stream;
-  hipLaunchKernelGGL(k_carve_seed, dim3(grid_for(g.N)), dim3(kBlock), 0, s, g.N, snode, (uint32_t*)tagw,
+  hipLaunchKernelGGL(k_carve_seed, dim3((g.N + 16 * kBT - 1) / (16 * kBT)), dim3(kBT), 0, s, g.N, snode, (uint32_t*)tagw,
                      (int32_t*)list, w);
   for (int t = 0; t <= npass; ++t) {
     hipLaunchKernelGGL(k_carve_prep, dim3(1), dim3(1), 0, s, w);
-    hipLaunchKernelGGL(k_carve_round, dim3(stride_grid()), dim3(kBlock), 0, s, g.N, levels, fin, Pfinal,
+    hipLaunchKernelGGL(k_carve_round, dim3(stride_grid()), dim3(kBT), 0, s, g.N, levels, fin, Pfinal,
                        (uint32_t*)tagw, (int32_t*)list, w, t);
   }
   hipLaunchKernelGGL(k_carve_reverse, dim3(stride_grid()), dim3(kBlock), 0, s, g.N, (const int32_t*)list, w,
@@ -512,13 +396,34 @@ int carve_from_levels(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t*
   return check_hip(hipGetLastError(), "carve");
 }
 
-// Labels with the levels a carve needs.  levels: (npass+1) x N, level 0 = rec.
-int labels_with_levels(pff_ctx* ctx, const GridD& g, const int32_t* rec, int32_t* bid, int32_t* Pfinal,
-                       int32_t* levels, int32_t* fin) {
-  return labels_impl(ctx, g, rec, bid, Pfinal, levels, fin);
+// finalise for the tiled carve: reverse tagged links (lakeflow.py:394-397)
+__global__ __launch_bounds__(kBT) void k_carve_reverse_all(int32_t N, const uint8_t* __restrict__ tag,
+                                                           const int32_t* __restrict__ rec_in, int32_t* rec) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N || !tag[i]) return;
+  const int r = rec_in[i];
+  if (r != i && (unsigned)r < (unsigned)N && tag[r]) rec[r] = i;
 }
 
-int npass_for(int32_t N) { return label_rounds(N); }
+// Carve for one trip with the tiled contraction: the cells the reference
+// tags (every cell on the path from a saddle node to its pit) are exactly the
+// cells with a saddle node upstream, an upstream sum of saddle indicators.
+// rec equals rec_in on entry; on return rec holds the carved receivers and
+// rec_work (if given) a copy.  *cycle (host) = 1 if rec_in is not a forest
+// (nothing is written then).
+int carve_tiled(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t* rec_in, const int32_t* bid,
+                const int32_t* snode, const int64_t* outlet, int32_t* rec_work, int32_t* cycle) {
+  void* tag;
+  PFF_TRY(scratch(ctx, S_B1, (size_t)g.N, &tag));
+  PFF_TRY(tiled_upstream(ctx, g, rec_in, 2, bid, snode, 0.0f, nullptr, nullptr, (uint8_t*)tag, cycle));
+  if (*cycle) return 0;
+  const dim3 gr(grid_for(g.N)), bl(kBlock);
+  hipLaunchKernelGGL(k_carve_reverse_all, gr, bl, 0, ctx->stream, g.N, (const uint8_t*)tag, rec_in, rec);
+  hipLaunchKernelGGL(k_carve_link, gr, bl, 0, ctx->stream, g.N, outlet, snode, rec);
+  if (rec_work)
+    PFF_TRY(check_hip(hipMemcpyAsync(rec_work, rec, (size_t)g.N * 4, hipMemcpyDeviceToDevice, ctx->stream), "copy"));
+  return check_hip(hipGetLastError(), "carve_tiled");
+}
 
 // Reference-shaped carve (lakeflow.py:405-438) for the module-level API.
 // rec_ is the working copy on entry and is left holding the jumped pointers
@@ -526,7 +431,7 @@ int npass_for(int32_t N) { return label_rounds(N); }
 // receives the carved receivers; tag (nullable) the tagged cells.
 int reroute_carve(pff_ctx* ctx, const GridD& g, int32_t* rec, int32_t* rec_, int32_t* rec__, uint8_t* tag,
                   uint8_t* tag_, const int32_t* snode, const int64_t* outlet, int32_t* iters) {
-  const int npass = label_rounds(g.N);
+  const int npass = npass_for(g.N);
   void *lv, *fin, *bid;
   PFF_TRY(scratch(ctx, S_LEVELS, (size_t)(npass + 1) * g.N * 4, &lv));
   PFF_TRY(scratch(ctx, S_I4, (size_t)g.N * 4, &fin));

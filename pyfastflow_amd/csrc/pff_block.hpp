@@ -1,0 +1,64 @@
+// pff_block.hpp -- block-level helpers (256-thread blocks, 64-wide waves):
+// exclusive scan and list append with one global atomic per block.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pff {
+
+constexpr int kBT = 256;  // threads per block for block helpers
+
+// Exclusive prefix of v over the block; *total receives the block sum.
+// Must be reached by every thread of the block.
+__device__ __forceinline__ int block_exclusive_scan(int v, int* total) {
+  __shared__ int wsum[kBT / 64];
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[wid] = x;
+  __syncthreads();
+  int woff = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kBT / 64; ++w) {
+    const int s = wsum[w];
+    woff += (w < wid) ? s : 0;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return woff + x - v;
+}
+
+// Append the taken values of every thread (up to ITEMS each) to list[*count..]
+// with a single atomicAdd per block.  Must be reached by every thread.
+template <int ITEMS>
+__device__ __forceinline__ void block_append(const int32_t (&val)[ITEMS], const bool (&take)[ITEMS], int32_t* list,
+                                             int64_t* count) {
+  __shared__ int64_t base_s;
+  int mine = 0;
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) mine += take[j] ? 1 : 0;
+  int total;
+  const int off = block_exclusive_scan(mine, &total);
+  if (threadIdx.x == 0) base_s = total ? (int64_t)atomicAdd((unsigned long long*)count, (unsigned long long)total) : 0;
+  __syncthreads();
+  int64_t p = base_s + off;
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j)
+    if (take[j]) list[p++] = val[j];
+  __syncthreads();
+}
+
+// Block-wide sum with one atomicAdd on a 64-bit counter.
+__device__ __forceinline__ void block_count(int mine, int64_t* counter) {
+  int total;
+  (void)block_exclusive_scan(mine, &total);
+  if (threadIdx.x == 0 && total) atomicAdd((unsigned long long*)counter, (unsigned long long)total);
+}
+
+}  // namespace pff

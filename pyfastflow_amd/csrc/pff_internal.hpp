@@ -13,7 +13,7 @@ struct pff_ctx {
   int32_t device = 0;
   hipStream_t stream = nullptr;
   // growable device scratch, keyed by slot
-  static constexpr int kSlots = 16;
+  static constexpr int kSlots = 32;
   void* buf[kSlots] = {};
   size_t cap[kSlots] = {};
   // pinned host words for counter readback
@@ -39,11 +39,13 @@ enum Slot : int {
   S_B0,                                    // byte N-array
   S_LEVELS,                                // pointer-jumping levels
   S_MISC,
-  S_TAG                                    // carve tag words, kept all-zero between calls
+  S_TAG,                                   // carve tag words, kept all-zero between calls
+  S_B1,                                    // byte N-array (tiled carve tags)
+  S_TE, S_TP2, S_TBID, S_TEXIT, S_TS, S_TIN  // tiled contraction buffers
 };
 
 // device word indices (dev_words / host_words)
-enum Word : int { W_COUNT0 = 0, W_COUNT1, W_COUNT2, W_FLAG, W_FLAG2, W_CYCLE, W_NWORDS = 16 };
+enum Word : int { W_COUNT0 = 0, W_COUNT1, W_COUNT2, W_FLAG, W_FLAG2, W_CYCLE, W_NEXIT, W_NDEP, W_NWORDS = 16 };
 
 constexpr int kBlock = 256;
 inline unsigned grid_for(int64_t n, int block = kBlock, int64_t cap = 1 << 20) {
@@ -75,6 +77,15 @@ int launch_propagate_basin(pff_ctx* ctx, const GridD& g, const int32_t* bid_in, 
 int basin_labels(pff_ctx* ctx, const GridD& g, const int32_t* rec, int32_t* bid, int32_t* rec_out);
 int launch_saddlesort(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t* is_border, float* zp,
                       int64_t* saddle, int32_t* snode, int64_t* outlet, const float* z);
+// tiled contraction (forests; *cycle = 1 when the receivers are not a forest)
+int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, const int32_t* bid,
+                   const int32_t* snode, float q0, float* Q, uint32_t* cnt, uint8_t* tag, int32_t* cycle);
+int tiled_labels(pff_ctx* ctx, const GridD& g, const int32_t* rec, int32_t* bid, int64_t* ndep, int32_t* cycle,
+                 bool write = true);
+// writes the labels prepared by the last tiled_labels(write=false) call
+int tiled_labels_write(pff_ctx* ctx, const GridD& g, int32_t* bid);
+int carve_tiled(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t* rec_in, const int32_t* bid,
+                const int32_t* snode, const int64_t* outlet, int32_t* rec_work, int32_t* cycle);
 int launch_reroute_jump(pff_ctx* ctx, const GridD& g, int32_t* rec, const int64_t* outlet);
 // reference-shaped carve (lakeflow.py:405-438)
 int reroute_carve(pff_ctx* ctx, const GridD& g, int32_t* rec, int32_t* rec_, int32_t* rec__, uint8_t* tag,
