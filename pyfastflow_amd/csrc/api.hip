@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <utility>
 
 #include "pff_internal.hpp"
 
@@ -177,7 +178,7 @@ int pff_upstream_count(pff_ctx* ctx, const pff_grid* g, const int32_t* rcv, uint
   PFF_TRY(need(rcv, "rcv"));
   PFF_TRY(need(count, "count"));
   int32_t cycle = 0;
-  PFF_TRY(tiled_upstream(ctx, d, rcv, 1, nullptr, nullptr, 0.0f, nullptr, count, nullptr, &cycle));
+  PFF_TRY(tiled_upstream(ctx, d, rcv, 1, nullptr, nullptr, 0.0f, nullptr, count, nullptr, nullptr, &cycle));
   if (cycle) PFF_TRY(upstream_count(ctx, d, rcv, count, nullptr));
   return 0;
 }
@@ -193,7 +194,7 @@ int pff_accumulate_constant_Q(pff_ctx* ctx, const pff_grid* g, const int32_t* rc
   // (flowfields.py:279)
   const float q0 = (float)(value * (area ? (double)g->dx * (double)g->dx : 1.0));
   int32_t cycle = 0;
-  PFF_TRY(tiled_upstream(ctx, d, rcv, 1, nullptr, nullptr, q0, Q, nullptr, nullptr, &cycle));
+  PFF_TRY(tiled_upstream(ctx, d, rcv, 1, nullptr, nullptr, q0, Q, nullptr, nullptr, nullptr, &cycle));
   if (cycle) {
     // not a forest: exact compacted pointer jumping, reported as unconverged
     void* cnt;
@@ -307,21 +308,22 @@ int pff_reroute_flow(pff_ctx* ctx, const pff_grid* g, int32_t* bid, int32_t* rec
     fin = (int32_t*)fv;
     return 0;
   };
+  // tile activity flags: trip t labels, sorts and carves only the tiles that
+  // held a closed basin after trip t-1 (all tiles in the first trip)
+  uint32_t *flag_a = nullptr, *flag_b = nullptr;
+  PFF_TRY(tile_flags(ctx, d, &flag_a, &flag_b));
+  const uint32_t* tact = nullptr;
   int nouter = 1;
-  int64_t ndep0 = 0;
   for (int it = 0; it < nouter; ++it) {
     int64_t ndb = 0;
-    bool labelled = false;
     if (!exact) {
       int32_t cyc = 0;
-      PFF_TRY(tiled_labels(ctx, d, rec_, bid, &ndb, &cyc, /*write=*/false));
+      PFF_TRY(tiled_labels(ctx, d, rec_, tact, &ndb, &cyc));
       if (cyc) exact = true;
     }
-    if (exact) {
-      PFF_TRY(depression_count(ctx, d, rec_, &ndb));
-    }
+    if (exact) PFF_TRY(depression_count(ctx, d, rec_, &ndb));
     if (it == 0) {
-      ndep0 = ndb;  // lakeflow.py:493 (rec_ == rec here)
+      // lakeflow.py:493 (rec_ == rec here)
       if (st) st->ndep0 = ndb;
       if (ndb == 0) return 0;  // :496-497, bid untouched
       nouter = ceil_log2(ndb) + 1;  // :500
@@ -332,39 +334,47 @@ int pff_reroute_flow(pff_ctx* ctx, const pff_grid* g, int32_t* bid, int32_t* rec
     }
     // basin labels on rec_ (:507-511)
     if (!exact) {
-      PFF_TRY(tiled_labels_write(ctx, d, bid));
+      PFF_TRY(tiled_labels_write(ctx, d, bid, tact, flag_a));
+      tact = flag_a;
+      std::swap(flag_a, flag_b);
     } else if (carve) {
       PFF_TRY(ensure_levels());
       PFF_TRY(labels_with_levels(ctx, d, rec_, bid, rec__, levels, fin));
     } else {
       PFF_TRY(basin_labels(ctx, d, rec_, bid, rec__));
     }
-    labelled = true;
-    (void)labelled;
     if (it == 0 && bid_first)
       PFF_TRY(check_hip(hipMemcpyAsync(bid_first, bid, n4, hipMemcpyDeviceToDevice, s), "copy"));
     if (ndb == 0) break;  // :512-514
-    PFF_TRY(launch_saddlesort(ctx, d, bid, is_border, z_prime, basin_saddle, basin_saddlenode, outlet, z));  // :520
-    if (st) st->n_iter++;
-    if (carve) {
-      // rec equals rec_ here: the carve edits rec and copies it to rec_ (:525-526)
+    // saddles and outlets (:520)
+    if (!exact) {
+      int32_t* pits = nullptr;
+      int64_t* npits = nullptr;
+      PFF_TRY(tiled_pits(ctx, d, &pits, &npits));
+      PFF_TRY(saddlesort_active(ctx, d, bid, is_border, z_prime, basin_saddle, basin_saddlenode, outlet, z, tact, pits,
+                                npits));
+      if (st) st->n_iter++;
+      // carve (:525-526): rec equals rec_ here; rec is edited, then copied to rec_
       int32_t cyc = 0;
-      if (!exact) {
-        PFF_TRY(carve_tiled(ctx, d, rec, rec_, bid, basin_saddlenode, outlet, rec_, &cyc));
-        if (cyc) exact = true;  // unreachable: labels already proved rec_ a forest
-      }
-      if (exact) {
-        PFF_TRY(ensure_levels());
-        if (cyc) PFF_TRY(labels_with_levels(ctx, d, rec_, bid, rec__, levels, fin));
-        int32_t iters = 0;
-        PFF_TRY(carve_from_levels(ctx, d, rec, rec_, levels, fin, rec__, basin_saddlenode, outlet, rec_, nullptr,
-                                  npass, &iters));
-      }
+      PFF_TRY(carve_tiled(ctx, d, rec, rec_, bid, basin_saddlenode, outlet, rec_, tact, pits, npits, &cyc));
+      if (!cyc) continue;
+      // not reachable (the labels proved rec_ a forest); fall back regardless
+      exact = true;
+      PFF_TRY(ensure_levels());
+      PFF_TRY(labels_with_levels(ctx, d, rec_, bid, rec__, levels, fin));
+      PFF_TRY(launch_saddlesort(ctx, d, bid, is_border, z_prime, basin_saddle, basin_saddlenode, outlet, z));
+    } else {
+      PFF_TRY(launch_saddlesort(ctx, d, bid, is_border, z_prime, basin_saddle, basin_saddlenode, outlet, z));
+      if (st) st->n_iter++;
+    }
+    if (carve) {
+      int32_t iters = 0;
+      PFF_TRY(carve_from_levels(ctx, d, rec, rec_, levels, fin, rec__, basin_saddlenode, outlet, rec_, nullptr, npass,
+                                &iters));
     } else {
       PFF_TRY(launch_reroute_jump(ctx, d, rec_, outlet));  // :529
     }
   }
-  (void)ndep0;
   return launch_basin_id_init(ctx, d, bid);  // :537
 }
 

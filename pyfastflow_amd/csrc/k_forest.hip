@@ -318,7 +318,9 @@ enum WMode : int { W_ROOTS = 0, W_UNIT = 1, W_SADDLE = 2 };
 struct TileArgs {
   GridD g;
   FastDiv fd;
+  int tiles_x;
   const int32_t* rcv;
+  const uint32_t* tact;  // per-tile "active" flags; null = every tile active
   // local pass outputs
   int32_t* E;        // global local-end of every cell
   uint32_t* S;       // in-tile sums (later: totals at local ends)
@@ -327,7 +329,8 @@ struct TileArgs {
   int32_t* bid2;     // labels at local ends
   int32_t* exits;
   int64_t* n_exits;
-  int64_t* ndep;
+  int32_t* pits;     // basin ids (cell+1) of interior roots
+  int64_t* n_pits;
   int64_t* cycle;
   // saddle weights
   const int32_t* bid;
@@ -351,12 +354,17 @@ __device__ __forceinline__ uint32_t tile_weight(const TileArgs& a, int c) {
   }
 }
 
+__device__ __forceinline__ bool tile_active(const uint32_t* tact, int tiles_x, int bx, int by) {
+  return !tact || tact[by * tiles_x + bx] != 0u;
+}
+
 // PH 1: local pass; PH 3: final pass (weights += inflow, emit results)
 template <int MODE, int PH, int W>
 __global__ __launch_bounds__(kBT) void k_tile(TileArgs a) {
   __shared__ uint16_t P[TC];
   __shared__ uint16_t E[(PH == 1) ? TC : 1];
   __shared__ uint32_t S[(W != W_ROOTS) ? TC : 1];
+  if (!tile_active(a.tact, a.tiles_x, blockIdx.x, blockIdx.y)) return;  // uniform per block
   const GridD& g = a.g;
   const int x0 = blockIdx.x * TS, y0 = blockIdx.y * TS;
   const int tw = min(TS, g.nx - x0), th = min(TS, g.ny - y0);
@@ -424,15 +432,17 @@ __global__ __launch_bounds__(kBT) void k_tile(TileArgs a) {
       break;
     }
   }
+  if (cyc && threadIdx.x == 0) atomicOr((unsigned long long*)a.cycle, 1ull);
   if constexpr (PH == 1) {
-    int32_t ex[CPT];
-    bool take[CPT];
-    int pits = 0;
+    int32_t ex[CPT], pv[CPT];
+    bool take[CPT], tpit[CPT];
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
       const int li = threadIdx.x + j * kBT;
       take[j] = false;
+      tpit[j] = false;
       ex[j] = cell[j];
+      pv[j] = cell[j] + 1;
       const int c = cell[j];
       if (c < 0) continue;
       const int e = E[li];
@@ -448,20 +458,23 @@ __global__ __launch_bounds__(kBT) void k_tile(TileArgs a) {
             const bool out = can_leave<MODE>(g, c, row, col);
             a.P2[c] = c;
             a.bid2[c] = out ? 0 : c + 1;
-            pits += out ? 0 : 1;
+            tpit[j] = !out;
           } else {
             a.P2[c] = -1;
           }
+        } else if constexpr (W == W_SADDLE) {
+          // only paths inside closed basins can carry a saddle tag
+          const int b = a.bid[c];
+          if (b != 0) take[j] = true;
+          else a.P2[c] = -1;
         } else {
           take[j] = true;  // exit: its receiver is in another tile
         }
       }
     }
-    if (cyc && threadIdx.x == 0) atomicOr((unsigned long long*)a.cycle, 1ull);
-    if constexpr (W == W_ROOTS) block_count(pits, a.ndep);
+    if constexpr (W == W_ROOTS) block_append<CPT>(pv, tpit, a.pits, a.n_pits);
     block_append<CPT>(ex, take, a.exits, a.n_exits);
   } else {
-    if (cyc && threadIdx.x == 0) atomicOr((unsigned long long*)a.cycle, 1ull);
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
       const int li = threadIdx.x + j * kBT;
@@ -477,17 +490,28 @@ __global__ __launch_bounds__(kBT) void k_tile(TileArgs a) {
   }
 }
 
-// reduced graph: exit x points to the local end of the cell it drains into
-__global__ __launch_bounds__(kBT) void k_exit_link(int32_t N, const int32_t* __restrict__ exits, const int64_t* count,
-                                                   const int32_t* __restrict__ rcv, const int32_t* __restrict__ E,
-                                                   int32_t* __restrict__ P2, int32_t* __restrict__ bid2, int copy_list,
-                                                   int32_t* __restrict__ list_out) {
+// reduced graph: exit x points to the local end of the cell it drains into.
+// For labels, a target in an inactive tile drains to an open edge (label 0):
+// the exit becomes a terminal labelled 0.
+__global__ __launch_bounds__(kBT) void k_exit_link(TileArgs a, const int32_t* __restrict__ exits,
+                                                   const int64_t* count, int labels, int32_t* __restrict__ list_out) {
   const int64_t n = *count;
   for (int64_t w = (int64_t)blockIdx.x * kBT + threadIdx.x; w < n; w += (int64_t)gridDim.x * kBT) {
     const int x = exits[w];
-    P2[x] = E[rcv[x]];
-    if (bid2) bid2[x] = 0;
-    if (copy_list) list_out[w] = x;
+    const int r = a.rcv[x];
+    if (labels && a.tact) {
+      const int rr = (int)fdiv((uint32_t)r, a.fd);
+      const int rc = r - rr * a.g.nx;
+      if (!tile_active(a.tact, a.tiles_x, rc / TS, rr / TS)) {
+        a.P2[x] = x;
+        a.bid2[x] = 0;
+        list_out[w] = x;
+        continue;
+      }
+    }
+    a.P2[x] = a.E[r];
+    if (labels) a.bid2[x] = 0;
+    list_out[w] = x;
   }
 }
 
@@ -502,16 +526,38 @@ __global__ __launch_bounds__(kBT) void k_exit_inflow(const int32_t* __restrict__
   }
 }
 
-// final labels: every cell takes the label of its local end
-__global__ __launch_bounds__(kBT) void k_label_from_end(int32_t N, const int32_t* __restrict__ E,
-                                                        const int32_t* __restrict__ bid2, int32_t* __restrict__ bid) {
-  const int i = blockIdx.x * kBT + threadIdx.x;
-  if (i < N) bid[i] = bid2[E[i]];
+// final labels on active tiles: bid = label of the local end; the tile stays
+// active for the next trip if any of its cells is in a closed basin
+__global__ __launch_bounds__(kBT) void k_label_tiles(TileArgs a, int32_t* __restrict__ bid,
+                                                     uint32_t* __restrict__ tact_out) {
+  const int bx = blockIdx.x, by = blockIdx.y;
+  const int t = by * a.tiles_x + bx;
+  if (!tile_active(a.tact, a.tiles_x, bx, by)) {
+    if (threadIdx.x == 0) tact_out[t] = 0u;
+    return;
+  }
+  const GridD& g = a.g;
+  const int x0 = bx * TS, y0 = by * TS;
+  const int tw = min(TS, g.nx - x0), th = min(TS, g.ny - y0);
+  int any = 0;
+#pragma unroll
+  for (int j = 0; j < CPT; ++j) {
+    const int li = threadIdx.x + j * kBT;
+    const int ly = li >> 6, lx = li & 63;
+    if (lx < tw && ly < th) {
+      const int c = (y0 + ly) * g.nx + x0 + lx;
+      const int b = a.bid2[a.E[c]];
+      bid[c] = b;
+      any |= (b != 0);
+    }
+  }
+  any = __syncthreads_or(any);
+  if (threadIdx.x == 0) tact_out[t] = any ? 1u : 0u;
 }
 
 struct TileBufs {
-  int32_t *E, *P2, *bid2, *exits, *L0, *L1, *t0, *t1, *t2;
-  uint32_t *S, *inflow;
+  int32_t *E, *P2, *bid2, *exits, *L0, *L1, *t0, *t1, *t2, *pits;
+  uint32_t *S, *inflow, *tact;
 };
 
 static int tile_bufs(pff_ctx* ctx, const GridD& g, TileBufs& b) {
@@ -528,6 +574,9 @@ static int tile_bufs(pff_ctx* ctx, const GridD& g, TileBufs& b) {
   PFF_TRY(scratch(ctx, S_I3, n4, &p)); b.t2 = (int32_t*)p;
   PFF_TRY(scratch(ctx, S_TS, n4, &p)); b.S = (uint32_t*)p;
   PFF_TRY(scratch(ctx, S_TIN, n4, &p)); b.inflow = (uint32_t*)p;
+  PFF_TRY(scratch(ctx, S_TPIT, n4, &p)); b.pits = (int32_t*)p;
+  const size_t nt = (size_t)((g.nx + TS - 1) / TS) * ((g.ny + TS - 1) / TS);
+  PFF_TRY(scratch(ctx, S_TACT, nt * 4 * 2, &p)); b.tact = (uint32_t*)p;
   return 0;
 }
 
@@ -535,6 +584,7 @@ static TileArgs tile_args(const GridD& g, const TileBufs& b, pff_ctx* ctx, const
   TileArgs a{};
   a.g = g;
   a.fd = make_fastdiv((uint32_t)g.nx);
+  a.tiles_x = (g.nx + TS - 1) / TS;
   a.rcv = rcv;
   a.E = b.E;
   a.S = b.S;
@@ -543,19 +593,25 @@ static TileArgs tile_args(const GridD& g, const TileBufs& b, pff_ctx* ctx, const
   a.bid2 = b.bid2;
   a.exits = b.exits;
   a.n_exits = ctx->dev_words + W_NEXIT;
-  a.ndep = ctx->dev_words + W_NDEP;
+  a.pits = b.pits;
+  a.n_pits = ctx->dev_words + W_NPIT;
   a.cycle = ctx->dev_words + W_CYCLE;
   return a;
 }
 
 static dim3 tile_grid(const GridD& g) { return dim3((g.nx + TS - 1) / TS, (g.ny + TS - 1) / TS); }
 
+int tiles_count(const GridD& g) { return ((g.nx + TS - 1) / TS) * ((g.ny + TS - 1) / TS); }
+
 // Upstream sums with the tiled contraction.  W_UNIT: counts -> cnt and/or
-// Q = q0*count; W_SADDLE: tag[c] = 1 where a saddle node lies upstream.
-// Returns 1 in *cycle (host) when the graph is not a forest (outputs are then
-// not meaningful and the caller must use the exact path).  Syncs once.
+// Q = q0*count; W_SADDLE: tag[c] = 1 where a saddle node lies upstream
+// (tiles flagged inactive in tact are skipped: they hold no closed basin).
+// *cycle (host) = 1 when the graph is not a forest (outputs are then not
+// meaningful and the caller must use the exact path).  Syncs once per six
+// reduced-graph rounds (usually once).
 int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, const int32_t* bid,
-                   const int32_t* snode, float q0, float* Q, uint32_t* cnt, uint8_t* tag, int32_t* cycle) {
+                   const int32_t* snode, float q0, float* Q, uint32_t* cnt, uint8_t* tag, const uint32_t* tact,
+                   int32_t* cycle) {
   TileBufs b;
   PFF_TRY(tile_bufs(ctx, g, b));
   TileArgs a = tile_args(g, b, ctx, rcv);
@@ -565,6 +621,7 @@ int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, 
   a.Q = Q;
   a.cnt = cnt;
   a.tag = tag;
+  a.tact = tact;
   hipStream_t s = ctx->stream;
   PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_NEXIT, 0, 8, s), "memset"));
   PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_CYCLE, 0, 8, s), "memset"));
@@ -578,8 +635,7 @@ int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, 
   L.c[0] = ctx->dev_words + W_COUNT0;
   L.c[1] = ctx->dev_words + W_COUNT1;
   PFF_TRY(check_hip(hipMemcpyAsync(L.c[0], a.n_exits, 8, hipMemcpyDeviceToDevice, s), "copy"));
-  hipLaunchKernelGGL(k_exit_link, dim3(list_grid()), dim3(kBT), 0, s, g.N, b.exits, a.n_exits, rcv, b.E, b.P2,
-                     (int32_t*)nullptr, 1, b.L0);
+  hipLaunchKernelGGL(k_exit_link, dim3(list_grid()), dim3(kBT), 0, s, a, b.exits, a.n_exits, 0, b.L0);
   const int cap = ceil_log2i(g.N) + 1;
   int done_rounds = 0;
   int64_t live = 1;
@@ -587,12 +643,13 @@ int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, 
     const int r = std::min(done_rounds == 0 ? 10 : 6, cap - done_rounds);
     PFF_TRY(count_rounds(ctx, L, b.S, b.P2, b.t0, (uint32_t*)b.t1, b.t2, r));
     done_rounds += r;
-    PFF_TRY(read_word(ctx, L.c[L.cur], &live));
-    if (live == 0) break;
+    PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_FLAG2, L.c[L.cur], 8, hipMemcpyDeviceToHost, s), "d2h"));
+    PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_CYCLE, a.cycle, 8, hipMemcpyDeviceToHost, s), "d2h"));
+    PFF_TRY(check_hip(hipStreamSynchronize(s), "sync"));
+    live = ctx->host_words[W_FLAG2];
+    if (live == 0 || ctx->host_words[W_CYCLE] != 0) break;
   }
-  int64_t cyc = 0;
-  PFF_TRY(read_word(ctx, a.cycle, &cyc));
-  if (live != 0 || cyc != 0) {
+  if (live != 0 || ctx->host_words[W_CYCLE] != 0) {
     *cycle = 1;
     return 0;
   }
@@ -604,17 +661,22 @@ int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, 
 }
 
 // Labels bid = init(root) with the tiled contraction, plus the depression
-// count (roots that cannot leave).  *ndep, *cycle are host outputs; syncs once
-// or twice.  On *cycle = 1 bid is not written.
-int tiled_labels(pff_ctx* ctx, const GridD& g, const int32_t* rec, int32_t* bid, int64_t* ndep, int32_t* cycle,
-                 bool write) {
+// count and the list of pits (basin ids of interior roots).  tact_in: tiles
+// to label (null = all); cells of other tiles keep their labels, which must
+// be 0 (they drain to an open edge).  The labels are written by
+// tiled_labels_write, which also flags the tiles that still hold a closed
+// basin.  *ndep, *npits, *cycle are host outputs; syncs once (per six
+// reduced-graph rounds).
+int tiled_labels(pff_ctx* ctx, const GridD& g, const int32_t* rec, const uint32_t* tact_in, int64_t* ndep,
+                 int32_t* cycle) {
   TileBufs b;
   PFF_TRY(tile_bufs(ctx, g, b));
   TileArgs a = tile_args(g, b, ctx, rec);
+  a.tact = tact_in;
   hipStream_t s = ctx->stream;
   PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_NEXIT, 0, 8, s), "memset"));
   PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_CYCLE, 0, 8, s), "memset"));
-  PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_NDEP, 0, 8, s), "memset"));
+  PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_NPIT, 0, 8, s), "memset"));
   const dim3 tg = tile_grid(g);
   PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL((k_tile<M, 1, W_ROOTS>), tg, dim3(kBT), 0, s, a));
   Lists L;
@@ -623,8 +685,7 @@ int tiled_labels(pff_ctx* ctx, const GridD& g, const int32_t* rec, int32_t* bid,
   L.c[0] = ctx->dev_words + W_COUNT0;
   L.c[1] = ctx->dev_words + W_COUNT1;
   PFF_TRY(check_hip(hipMemcpyAsync(L.c[0], a.n_exits, 8, hipMemcpyDeviceToDevice, s), "copy"));
-  hipLaunchKernelGGL(k_exit_link, dim3(list_grid()), dim3(kBT), 0, s, g.N, b.exits, a.n_exits, rec, b.E, b.P2,
-                     b.bid2, 1, b.L0);
+  hipLaunchKernelGGL(k_exit_link, dim3(list_grid()), dim3(kBT), 0, s, a, b.exits, a.n_exits, 1, b.L0);
   const int cap = ceil_log2i(g.N) + 1;
   int done_rounds = 0;
   int64_t live = 1;
@@ -632,29 +693,43 @@ int tiled_labels(pff_ctx* ctx, const GridD& g, const int32_t* rec, int32_t* bid,
     const int r = std::min(done_rounds == 0 ? 10 : 6, cap - done_rounds);
     PFF_TRY(label_rounds_run(ctx, L, b.bid2, b.P2, b.t0, b.t1, nullptr, g.N, nullptr, done_rounds, r));
     done_rounds += r;
-    // one readback for the live count, the cycle flag and the depression count
+    // one readback for the live count, the cycle flag and the pit count
     PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_FLAG2, L.c[L.cur], 8, hipMemcpyDeviceToHost, s), "d2h"));
     PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_CYCLE, a.cycle, 8, hipMemcpyDeviceToHost, s), "d2h"));
-    PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_NDEP, a.ndep, 8, hipMemcpyDeviceToHost, s), "d2h"));
+    PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_NPIT, a.n_pits, 8, hipMemcpyDeviceToHost, s), "d2h"));
     PFF_TRY(check_hip(hipStreamSynchronize(s), "sync"));
     live = ctx->host_words[W_FLAG2];
     if (live == 0 || ctx->host_words[W_CYCLE] != 0) break;
   }
-  *ndep = ctx->host_words[W_NDEP];
-  if (live != 0 || ctx->host_words[W_CYCLE] != 0) {
-    *cycle = 1;
-    return 0;
-  }
-  *cycle = 0;
-  if (write) return tiled_labels_write(ctx, g, bid);
+  *ndep = ctx->host_words[W_NPIT];
+  *cycle = (live != 0 || ctx->host_words[W_CYCLE] != 0) ? 1 : 0;
   return 0;
 }
 
-int tiled_labels_write(pff_ctx* ctx, const GridD& g, int32_t* bid) {
+int tiled_labels_write(pff_ctx* ctx, const GridD& g, int32_t* bid, const uint32_t* tact_in, uint32_t* tact_out) {
   TileBufs b;
   PFF_TRY(tile_bufs(ctx, g, b));
-  hipLaunchKernelGGL(k_label_from_end, dim3(grid_for(g.N)), dim3(kBT), 0, ctx->stream, g.N, b.E, b.bid2, bid);
+  TileArgs a = tile_args(g, b, ctx, nullptr);
+  a.tact = tact_in;
+  hipLaunchKernelGGL(k_label_tiles, tile_grid(g), dim3(kBT), 0, ctx->stream, a, bid, tact_out);
   return check_hip(hipGetLastError(), "tiled_labels_write");
+}
+
+// device list of pits from the last tiled_labels call
+int tiled_pits(pff_ctx* ctx, const GridD& g, int32_t** pits, int64_t** npits) {
+  TileBufs b;
+  PFF_TRY(tile_bufs(ctx, g, b));
+  *pits = b.pits;
+  *npits = ctx->dev_words + W_NPIT;
+  return 0;
+}
+
+int tile_flags(pff_ctx* ctx, const GridD& g, uint32_t** flags_a, uint32_t** flags_b) {
+  TileBufs b;
+  PFF_TRY(tile_bufs(ctx, g, b));
+  *flags_a = b.tact;
+  *flags_b = b.tact + tiles_count(g);
+  return 0;
 }
 
 }  // namespace pff

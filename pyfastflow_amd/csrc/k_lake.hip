@@ -396,32 +396,273 @@ int carve_from_levels(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t*
   return check_hip(hipGetLastError(), "carve");
 }
 
-// finalise for the tiled carve: reverse tagged links (lakeflow.py:394-397)
-__global__ __launch_bounds__(kBT) void k_carve_reverse_all(int32_t N, const uint8_t* __restrict__ tag,
-                                                           const int32_t* __restrict__ rec_in, int32_t* rec) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N || !tag[i]) return;
-  const int r = rec_in[i];
-  if (r != i && (unsigned)r < (unsigned)N && tag[r]) rec[r] = i;
+// ---------------------------------------------------------------------------
+// Active-region versions used by the reroute_flow driver.  A tile is active
+// while it holds a cell of a closed basin (label != 0); cells draining to an
+// open edge keep label 0 and their receivers for the rest of the call, so
+// their tiles are skipped.  Per-basin passes run over the list of pits
+// (basin id = pit cell + 1) instead of all N basin slots.
+// ---------------------------------------------------------------------------
+constexpr int TSL = 64;  // tile side, as in the tiled contraction
+
+#define PFF_TILE_PROLOGUE(tact, tiles_x)                                          \
+  const int bx_ = blockIdx.x, by_ = blockIdx.y;                                  \
+  if (tact && tact[by_ * (tiles_x) + bx_] == 0u) return;                         \
+  const int x0_ = bx_ * TSL, y0_ = by_ * TSL;                                    \
+  const int tw_ = min(TSL, g.nx - x0_), th_ = min(TSL, g.ny - y0_);
+
+template <typename F>
+__device__ __forceinline__ void tile_for(const GridD& g, int x0, int y0, int tw, int th, F&& f) {
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int li = threadIdx.x + j * 256;
+    const int ly = li >> 6, lx = li & 63;
+    if (lx < tw && ly < th) {
+      const int row = y0 + ly, col = x0 + lx;
+      f(row * g.nx + col, row, col);
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBT) void k_saddle_init_list(const int32_t* __restrict__ pits, const int64_t* np,
+                                                          int64_t* saddle, int64_t* outlet, int32_t* snode) {
+  const int64_t n = *np;
+  for (int64_t w = (int64_t)blockIdx.x * kBT + threadIdx.x; w < n; w += (int64_t)gridDim.x * kBT) {
+    const int b = pits[w];
+    saddle[b] = INVALID_KEY;
+    outlet[b] = INVALID_KEY;
+    snode[b] = -1;
+  }
+}
+
+// passes 1+2 fused (lakeflow.py:132-194): z', border flag and the packed
+// atomicMin of (z'_i, bid_j); a plain read first skips atomics that cannot
+// lower the current minimum
+template <int MODE>
+__global__ __launch_bounds__(kBT) void k_zprime_min_t(GridD g, int tiles_x, const uint32_t* __restrict__ tact,
+                                                      const int32_t* __restrict__ bid, const float* __restrict__ z,
+                                                      uint8_t* __restrict__ isb, float* __restrict__ zp,
+                                                      int64_t* saddle) {
+  PFF_TILE_PROLOGUE(tact, tiles_x)
+  tile_for(g, x0_, y0_, tw_, th_, [&](int i, int row, int col) {
+    const float zi = z[i];
+    if (can_leave<MODE>(g, i, row, col)) {
+      zp[i] = zi;
+    } else {
+      const int bi = bid[i];
+      bool border = false;
+      float zn = 1e9f;
+      int nbj[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int jn = neighbour<MODE>(g, i, row, col, k);
+        nbj[k] = -1;
+        if (jn < 0) continue;
+        const int bj = bid[jn];
+        if (bj != bi) {
+          border = true;
+          zn = fminf(zn, z[jn]);
+          nbj[k] = bj;
+        }
+      }
+      isb[i] = border ? 1 : 0;
+      const float zpi = border ? fmaxf(zi, zn) : 1e9f;
+      zp[i] = zpi;
+      if (border && bi != 0 && bi < g.N) {
+        int64_t res = INVALID_KEY;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          if (nbj[k] < 0) continue;
+          const int64_t c = pack_key(zpi, nbj[k]);
+          res = c < res ? c : res;
+        }
+        if (res != INVALID_KEY && res < saddle[bi]) atomicMin((long long*)&saddle[bi], (long long)res);
+      }
+    }
+  });
+}
+
+// pass 3 (:199-219), largest cell index wins
+template <int MODE>
+__global__ __launch_bounds__(kBT) void k_saddle_node_t(GridD g, int tiles_x, const uint32_t* __restrict__ tact,
+                                                       const int32_t* __restrict__ bid, const uint8_t* __restrict__ isb,
+                                                       const float* __restrict__ zp, const int64_t* __restrict__ saddle,
+                                                       int32_t* snode) {
+  PFF_TILE_PROLOGUE(tact, tiles_x)
+  tile_for(g, x0_, y0_, tw_, th_, [&](int i, int row, int col) {
+    const int bi = bid[i];
+    if (bi != 0 && bi < g.N && !can_leave<MODE>(g, i, row, col) && isb[i]) {
+      const int64_t key = saddle[bi];
+      const float tz = key_float(key);
+      const int tb = key_index(key);
+      bool here = false;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int jn = neighbour<MODE>(g, i, row, col, k);
+        if (jn < 0) continue;
+        if (bid[jn] == tb && zp[i] == tz) here = true;
+      }
+      if (here) atomicMax(&snode[bi], i);
+    }
+  });
+}
+
+// pass 4 (:226-250) over the pits
+template <int MODE>
+__global__ __launch_bounds__(kBT) void k_outlet_list(GridD g, const int32_t* __restrict__ pits, const int64_t* np,
+                                                     const int32_t* __restrict__ bid, const float* __restrict__ z,
+                                                     const int64_t* __restrict__ saddle,
+                                                     const int32_t* __restrict__ snode, int64_t* __restrict__ outlet) {
+  const int64_t n = *np;
+  for (int64_t w = (int64_t)blockIdx.x * kBT + threadIdx.x; w < n; w += (int64_t)gridDim.x * kBT) {
+    const int b = pits[w];
+    if (b >= g.N || saddle[b] == INVALID_KEY) continue;
+    const int node = snode[b];
+    if (node < 0) continue;
+    int row, col;
+    rc_of(g, node, row, col);
+    float tz = 1e9f;
+    int rr = -1;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int jn = neighbour<MODE>(g, node, row, col, k);
+      if (jn < 0) continue;
+      if (bid[jn] != b && tz > z[jn]) {
+        tz = z[jn];
+        rr = jn;
+      }
+    }
+    if (rr > -1) {
+      const int64_t c = pack_key(tz, rr);
+      if (c < outlet[b]) outlet[b] = c;
+    }
+  }
+}
+
+// pass 5 (:255-281) over the pits, in the two phases of k_cycle_flag/apply
+__global__ __launch_bounds__(kBT) void k_cycle_flag_list(int32_t N, const int32_t* __restrict__ pits, const int64_t* np,
+                                                         const int32_t* __restrict__ bid,
+                                                         const int64_t* __restrict__ outlet, uint8_t* __restrict__ inv) {
+  const int64_t n = *np;
+  for (int64_t w = (int64_t)blockIdx.x * kBT + threadIdx.x; w < n; w += (int64_t)gridDim.x * kBT) {
+    const int b = pits[w];
+    if (b >= N) continue;
+    uint8_t f = 0;
+    if (outlet[b] != INVALID_KEY) {
+      const int r1 = key_index(outlet[b]);
+      const int b1 = (r1 >= 0 && r1 < N) ? bid[r1] : -1;
+      if (b1 > 0 && b1 < N) {
+        const int r2 = key_index(outlet[b1]);
+        const int b2 = (r2 >= 0 && r2 < N) ? bid[r2] : -1;
+        if (b2 == b && b1 < b) f = 1;
+      }
+    }
+    inv[b] = f;
+  }
+}
+
+__global__ __launch_bounds__(kBT) void k_cycle_apply_list(int32_t N, const int32_t* __restrict__ pits,
+                                                          const int64_t* np, const int32_t* __restrict__ bid,
+                                                          int64_t* outlet, int64_t* saddle, int32_t* snode,
+                                                          const uint8_t* __restrict__ inv) {
+  const int64_t n = *np;
+  for (int64_t w = (int64_t)blockIdx.x * kBT + threadIdx.x; w < n; w += (int64_t)gridDim.x * kBT) {
+    const int b = pits[w];
+    if (b >= N) continue;
+    bool kill = inv[b] != 0;
+    if (!kill && N > 42 && bid[42] == b && outlet[b] != INVALID_KEY) {
+      const int r1 = key_index(outlet[b]);
+      const int b1 = (r1 >= 0 && r1 < N) ? bid[r1] : -1;
+      if (b1 > 0 && b1 < b && inv[b1]) kill = true;
+    }
+    if (kill) {
+      outlet[b] = INVALID_KEY;
+      saddle[b] = INVALID_KEY;
+      snode[b] = -1;
+    }
+  }
+}
+
+int saddlesort_active(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t* isb, float* zp, int64_t* saddle,
+                      int32_t* snode, int64_t* outlet, const float* z, const uint32_t* tact, const int32_t* pits,
+                      const int64_t* npits) {
+  void* inv;
+  PFF_TRY(scratch(ctx, S_B0, (size_t)g.N, &inv));
+  hipStream_t s = ctx->stream;
+  const int tiles_x = (g.nx + TSL - 1) / TSL;
+  const dim3 tg(tiles_x, (g.ny + TSL - 1) / TSL), lg(1024), bl(kBT);
+  hipLaunchKernelGGL(k_saddle_init_list, lg, bl, 0, s, pits, npits, saddle, outlet, snode);
+  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_zprime_min_t<M>, tg, bl, 0, s, g, tiles_x, tact, bid, z, isb, zp,
+                                               saddle));
+  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_saddle_node_t<M>, tg, bl, 0, s, g, tiles_x, tact, bid,
+                                               (const uint8_t*)isb, (const float*)zp, (const int64_t*)saddle, snode));
+  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_outlet_list<M>, lg, bl, 0, s, g, pits, npits, bid, z,
+                                               (const int64_t*)saddle, (const int32_t*)snode, outlet));
+  hipLaunchKernelGGL(k_cycle_flag_list, lg, bl, 0, s, g.N, pits, npits, bid, (const int64_t*)outlet, (uint8_t*)inv);
+  hipLaunchKernelGGL(k_cycle_apply_list, lg, bl, 0, s, g.N, pits, npits, bid, outlet, saddle, snode,
+                     (const uint8_t*)inv);
+  return check_hip(hipGetLastError(), "saddlesort_active");
+}
+
+// tiled carve finalise (lakeflow.py:390-397): reverse tagged links
+__global__ __launch_bounds__(kBT) void k_carve_reverse_t(GridD g, int tiles_x, const uint32_t* __restrict__ tact,
+                                                         const uint8_t* __restrict__ tag,
+                                                         const int32_t* __restrict__ rec_in, int32_t* rec) {
+  PFF_TILE_PROLOGUE(tact, tiles_x)
+  tile_for(g, x0_, y0_, tw_, th_, [&](int i, int row, int col) {
+    (void)row;
+    (void)col;
+    if (tag[i]) {
+      const int r = rec_in[i];
+      if (r != i && (unsigned)r < (unsigned)g.N && tag[r]) rec[r] = i;
+    }
+  });
+}
+
+// saddle node -> outlet (lakeflow.py:399-403) over the pits
+__global__ __launch_bounds__(kBT) void k_carve_link_list(const int32_t* __restrict__ pits, const int64_t* np,
+                                                         const int64_t* __restrict__ outlet,
+                                                         const int32_t* __restrict__ snode, int32_t* rec) {
+  const int64_t n = *np;
+  for (int64_t w = (int64_t)blockIdx.x * kBT + threadIdx.x; w < n; w += (int64_t)gridDim.x * kBT) {
+    const int b = pits[w];
+    const int64_t o = outlet[b];
+    if (o != INVALID_KEY) rec[snode[b]] = key_index(o);
+  }
+}
+
+// every changed receiver belongs to a tagged cell: copy those to the working copy
+__global__ __launch_bounds__(kBT) void k_carve_sync_t(GridD g, int tiles_x, const uint32_t* __restrict__ tact,
+                                                      const uint8_t* __restrict__ tag, const int32_t* __restrict__ rec,
+                                                      int32_t* __restrict__ rec_work) {
+  PFF_TILE_PROLOGUE(tact, tiles_x)
+  tile_for(g, x0_, y0_, tw_, th_, [&](int i, int row, int col) {
+    (void)row;
+    (void)col;
+    if (tag[i]) rec_work[i] = rec[i];
+  });
 }
 
 // Carve for one trip with the tiled contraction: the cells the reference
 // tags (every cell on the path from a saddle node to its pit) are exactly the
 // cells with a saddle node upstream, an upstream sum of saddle indicators.
 // rec equals rec_in on entry; on return rec holds the carved receivers and
-// rec_work (if given) a copy.  *cycle (host) = 1 if rec_in is not a forest
+// rec_work (if given) the same.  *cycle (host) = 1 if rec_in is not a forest
 // (nothing is written then).
 int carve_tiled(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t* rec_in, const int32_t* bid,
-                const int32_t* snode, const int64_t* outlet, int32_t* rec_work, int32_t* cycle) {
+                const int32_t* snode, const int64_t* outlet, int32_t* rec_work, const uint32_t* tact,
+                const int32_t* pits, const int64_t* npits, int32_t* cycle) {
   void* tag;
   PFF_TRY(scratch(ctx, S_B1, (size_t)g.N, &tag));
-  PFF_TRY(tiled_upstream(ctx, g, rec_in, 2, bid, snode, 0.0f, nullptr, nullptr, (uint8_t*)tag, cycle));
+  PFF_TRY(tiled_upstream(ctx, g, rec_in, 2, bid, snode, 0.0f, nullptr, nullptr, (uint8_t*)tag, tact, cycle));
   if (*cycle) return 0;
-  const dim3 gr(grid_for(g.N)), bl(kBlock);
-  hipLaunchKernelGGL(k_carve_reverse_all, gr, bl, 0, ctx->stream, g.N, (const uint8_t*)tag, rec_in, rec);
-  hipLaunchKernelGGL(k_carve_link, gr, bl, 0, ctx->stream, g.N, outlet, snode, rec);
+  const int tiles_x = (g.nx + TSL - 1) / TSL;
+  const dim3 tg(tiles_x, (g.ny + TSL - 1) / TSL), bl(kBT);
+  hipLaunchKernelGGL(k_carve_reverse_t, tg, bl, 0, ctx->stream, g, tiles_x, tact, (const uint8_t*)tag, rec_in, rec);
+  hipLaunchKernelGGL(k_carve_link_list, dim3(1024), bl, 0, ctx->stream, pits, npits, outlet, snode, rec);
   if (rec_work)
-    PFF_TRY(check_hip(hipMemcpyAsync(rec_work, rec, (size_t)g.N * 4, hipMemcpyDeviceToDevice, ctx->stream), "copy"));
+    hipLaunchKernelGGL(k_carve_sync_t, tg, bl, 0, ctx->stream, g, tiles_x, tact, (const uint8_t*)tag,
+                       (const int32_t*)rec, rec_work);
   return check_hip(hipGetLastError(), "carve_tiled");
 }
 

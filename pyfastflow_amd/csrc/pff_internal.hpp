@@ -41,11 +41,11 @@ enum Slot : int {
   S_MISC,
   S_TAG,                                   // carve tag words, kept all-zero between calls
   S_B1,                                    // byte N-array (tiled carve tags)
-  S_TE, S_TP2, S_TBID, S_TEXIT, S_TS, S_TIN  // tiled contraction buffers
+  S_TE, S_TP2, S_TBID, S_TEXIT, S_TS, S_TIN, S_TPIT, S_TACT  // tiled contraction buffers
 };
 
 // device word indices (dev_words / host_words)
-enum Word : int { W_COUNT0 = 0, W_COUNT1, W_COUNT2, W_FLAG, W_FLAG2, W_CYCLE, W_NEXIT, W_NDEP, W_NWORDS = 16 };
+enum Word : int { W_COUNT0 = 0, W_COUNT1, W_COUNT2, W_FLAG, W_FLAG2, W_CYCLE, W_NEXIT, W_NDEP, W_NPIT, W_NWORDS = 16 };
 
 constexpr int kBlock = 256;
 inline unsigned grid_for(int64_t n, int block = kBlock, int64_t cap = 1 << 20) {
@@ -79,13 +79,20 @@ int launch_saddlesort(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t*
                       int64_t* saddle, int32_t* snode, int64_t* outlet, const float* z);
 // tiled contraction (forests; *cycle = 1 when the receivers are not a forest)
 int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, const int32_t* bid,
-                   const int32_t* snode, float q0, float* Q, uint32_t* cnt, uint8_t* tag, int32_t* cycle);
-int tiled_labels(pff_ctx* ctx, const GridD& g, const int32_t* rec, int32_t* bid, int64_t* ndep, int32_t* cycle,
-                 bool write = true);
-// writes the labels prepared by the last tiled_labels(write=false) call
-int tiled_labels_write(pff_ctx* ctx, const GridD& g, int32_t* bid);
+                   const int32_t* snode, float q0, float* Q, uint32_t* cnt, uint8_t* tag, const uint32_t* tact,
+                   int32_t* cycle);
+int tiled_labels(pff_ctx* ctx, const GridD& g, const int32_t* rec, const uint32_t* tact_in, int64_t* ndep,
+                 int32_t* cycle);
+int tiled_labels_write(pff_ctx* ctx, const GridD& g, int32_t* bid, const uint32_t* tact_in, uint32_t* tact_out);
+int tiled_pits(pff_ctx* ctx, const GridD& g, int32_t** pits, int64_t** npits);
+int tile_flags(pff_ctx* ctx, const GridD& g, uint32_t** flags_a, uint32_t** flags_b);
+int tiles_count(const GridD& g);
 int carve_tiled(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t* rec_in, const int32_t* bid,
-                const int32_t* snode, const int64_t* outlet, int32_t* rec_work, int32_t* cycle);
+                const int32_t* snode, const int64_t* outlet, int32_t* rec_work, const uint32_t* tact,
+                const int32_t* pits, const int64_t* npits, int32_t* cycle);
+int saddlesort_active(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t* isb, float* zp, int64_t* saddle,
+                      int32_t* snode, int64_t* outlet, const float* z, const uint32_t* tact, const int32_t* pits,
+                      const int64_t* npits);
 int launch_reroute_jump(pff_ctx* ctx, const GridD& g, int32_t* rec, const int64_t* outlet);
 // reference-shaped carve (lakeflow.py:405-438)
 int reroute_carve(pff_ctx* ctx, const GridD& g, int32_t* rec, int32_t* rec_, int32_t* rec__, uint8_t* tag,
