@@ -313,12 +313,13 @@ int pff_reroute_flow(pff_ctx* ctx, const pff_grid* g, int32_t* bid, int32_t* rec
   uint32_t *flag_a = nullptr, *flag_b = nullptr;
   PFF_TRY(tile_flags(ctx, d, &flag_a, &flag_b));
   const uint32_t* tact = nullptr;
+  int label_rounds = 0;
   int nouter = 1;
   for (int it = 0; it < nouter; ++it) {
     int64_t ndb = 0;
     if (!exact) {
       int32_t cyc = 0;
-      PFF_TRY(tiled_labels(ctx, d, rec_, tact, &ndb, &cyc));
+      PFF_TRY(tiled_labels(ctx, d, rec_, tact, &ndb, &cyc, &label_rounds));
       if (cyc) exact = true;
     }
     if (exact) PFF_TRY(depression_count(ctx, d, rec_, &ndb));
@@ -356,7 +357,8 @@ int pff_reroute_flow(pff_ctx* ctx, const pff_grid* g, int32_t* bid, int32_t* rec
       if (st) st->n_iter++;
       // carve (:525-526): rec equals rec_ here; rec is edited, then copied to rec_
       int32_t cyc = 0;
-      PFF_TRY(carve_tiled(ctx, d, rec, rec_, bid, basin_saddlenode, outlet, rec_, tact, pits, npits, &cyc));
+      PFF_TRY(carve_tiled(ctx, d, rec, rec_, bid, basin_saddlenode, outlet, rec_, tact, pits, npits, label_rounds,
+                          &cyc));
       if (!cyc) continue;
       // not reachable (the labels proved rec_ a forest); fall back regardless
       exact = true;

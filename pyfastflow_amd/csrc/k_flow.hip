@@ -15,8 +15,10 @@ namespace pff {
 template <int MODE>
 __global__ __launch_bounds__(256) void k_receivers(GridD g, const float* __restrict__ z, int32_t* __restrict__ rcv,
                                                    float* __restrict__ grad) {
-  const int col = blockIdx.x * 64 + threadIdx.x;
-  const int row = blockIdx.y * 4 + threadIdx.y;
+  int bx, by;
+  xcd_block(bx, by);
+  const int col = bx * 64 + threadIdx.x;
+  const int row = by * 4 + threadIdx.y;
   if (col >= g.nx || row >= g.ny) return;
   const int i = row * g.nx + col;
   const float zi = z[i];

@@ -147,9 +147,12 @@ __global__ __launch_bounds__(kBT) void k_label_init_all(GridD g, const int32_t* 
 __global__ __launch_bounds__(kBT) void k_label_gather2(const int32_t* __restrict__ list, const int64_t* count,
                                                        const int32_t* __restrict__ bid, const int32_t* __restrict__ P,
                                                        int32_t* __restrict__ nb, int32_t* __restrict__ np,
-                                                       int64_t* next_count) {
+                                                       int64_t* next_count, int64_t* first_empty, int32_t round) {
   const int64_t n = *count;
-  if (blockIdx.x == 0 && threadIdx.x == 0) *next_count = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *next_count = 0;
+    if (n == 0 && first_empty) atomicMin((long long*)first_empty, (long long)round);
+  }
   for (int64_t w = (int64_t)blockIdx.x * kBT + threadIdx.x; w < n; w += (int64_t)gridDim.x * kBT) {
     const int m = list[w];
     const int pm = P[m];
@@ -217,7 +220,7 @@ static int label_rounds_run(pff_ctx* ctx, Lists& L, int32_t* bid, int32_t* P, in
   for (int k = first_round; k < first_round + rounds; ++k) {
     const int a = L.cur, b = 1 - L.cur;
     hipLaunchKernelGGL(k_label_gather2, dim3(list_grid()), dim3(kBT), 0, ctx->stream, L.l[a], L.c[a], bid, P, nb, np,
-                       L.c[b]);
+                       L.c[b], ctx->dev_words + W_EMPTY, k);
     int32_t* lvl = levels ? levels + (size_t)(k + 1) * N : nullptr;
     hipLaunchKernelGGL(k_label_scatter2, dim3(list_grid()), dim3(kBT), 0, ctx->stream, L.l[a], L.c[a], bid, P, nb,
                        np, L.l[b], L.c[b], lvl, fin, k);
@@ -309,7 +312,8 @@ int npass_for(int32_t N) { return ceil_log2i(N) + 1; }
 // ---------------------------------------------------------------------------
 constexpr int TS = 64;               // tile side (cells)
 constexpr int TC = TS * TS;          // 4096 cells per tile
-constexpr int CPT = TC / kBT;        // 16 cells per thread
+constexpr int kTB = 1024;            // threads per tile block
+constexpr int CPT = TC / kTB;        // 4 cells per thread
 constexpr uint16_t NONE = 0xFFFF;
 constexpr int kTileRounds = 12;      // 2^12 = TC: every acyclic in-tile path is contracted
 
@@ -360,7 +364,7 @@ __device__ __forceinline__ bool tile_active(const uint32_t* tact, int tiles_x, i
 
 // PH 1: local pass; PH 3: final pass (weights += inflow, emit results)
 template <int MODE, int PH, int W>
-__global__ __launch_bounds__(kBT) void k_tile(TileArgs a) {
+__global__ __launch_bounds__(kTB) void k_tile(TileArgs a) {
   __shared__ uint16_t P[TC];
   __shared__ uint16_t E[(PH == 1) ? TC : 1];
   __shared__ uint32_t S[(W != W_ROOTS) ? TC : 1];
@@ -372,7 +376,7 @@ __global__ __launch_bounds__(kBT) void k_tile(TileArgs a) {
   uint32_t rootmask = 0;
 #pragma unroll
   for (int j = 0; j < CPT; ++j) {
-    const int li = threadIdx.x + j * kBT;
+    const int li = threadIdx.x + j * kTB;
     const int ly = li >> 6, lx = li & 63;
     uint16_t lp = NONE;
     cell[j] = -1;
@@ -405,7 +409,7 @@ __global__ __launch_bounds__(kBT) void k_tile(TileArgs a) {
     uint32_t sv[CPT];
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
-      const int li = threadIdx.x + j * kBT;
+      const int li = threadIdx.x + j * kTB;
       pm[j] = P[li];
       pp[j] = NONE;
       sv[j] = 0;
@@ -419,7 +423,7 @@ __global__ __launch_bounds__(kBT) void k_tile(TileArgs a) {
     int any = 0;
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
-      const int li = threadIdx.x + j * kBT;
+      const int li = threadIdx.x + j * kTB;
       if (pm[j] != NONE) {
         if constexpr (W != W_ROOTS) atomicAdd(&S[pm[j]], sv[j]);
         P[li] = pp[j];
@@ -438,7 +442,7 @@ __global__ __launch_bounds__(kBT) void k_tile(TileArgs a) {
     bool take[CPT], tpit[CPT];
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
-      const int li = threadIdx.x + j * kBT;
+      const int li = threadIdx.x + j * kTB;
       take[j] = false;
       tpit[j] = false;
       ex[j] = cell[j];
@@ -472,12 +476,12 @@ __global__ __launch_bounds__(kBT) void k_tile(TileArgs a) {
         }
       }
     }
-    if constexpr (W == W_ROOTS) block_append<CPT>(pv, tpit, a.pits, a.n_pits);
-    block_append<CPT>(ex, take, a.exits, a.n_exits);
+    if constexpr (W == W_ROOTS) block_append<CPT, kTB>(pv, tpit, a.pits, a.n_pits);
+    block_append<CPT, kTB>(ex, take, a.exits, a.n_exits);
   } else {
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
-      const int li = threadIdx.x + j * kBT;
+      const int li = threadIdx.x + j * kTB;
       const int c = cell[j];
       if (c < 0) continue;
       if constexpr (W == W_UNIT) {
@@ -528,31 +532,24 @@ __global__ __launch_bounds__(kBT) void k_exit_inflow(const int32_t* __restrict__
 
 // final labels on active tiles: bid = label of the local end; the tile stays
 // active for the next trip if any of its cells is in a closed basin
+// (tact_out zeroed beforehand; one lane per cell, 16 blocks per tile)
 __global__ __launch_bounds__(kBT) void k_label_tiles(TileArgs a, int32_t* __restrict__ bid,
                                                      uint32_t* __restrict__ tact_out) {
-  const int bx = blockIdx.x, by = blockIdx.y;
-  const int t = by * a.tiles_x + bx;
-  if (!tile_active(a.tact, a.tiles_x, bx, by)) {
-    if (threadIdx.x == 0) tact_out[t] = 0u;
-    return;
-  }
-  const GridD& g = a.g;
-  const int x0 = bx * TS, y0 = by * TS;
-  const int tw = min(TS, g.nx - x0), th = min(TS, g.ny - y0);
+  int bx, by;
+  xcd_block(bx, by);
+  const int t = (by >> 4) * a.tiles_x + bx;
+  if (a.tact && a.tact[t] == 0u) return;  // uniform per block
+  const int col = bx * 64 + threadIdx.x;
+  const int row = by * 4 + threadIdx.y;
   int any = 0;
-#pragma unroll
-  for (int j = 0; j < CPT; ++j) {
-    const int li = threadIdx.x + j * kBT;
-    const int ly = li >> 6, lx = li & 63;
-    if (lx < tw && ly < th) {
-      const int c = (y0 + ly) * g.nx + x0 + lx;
-      const int b = a.bid2[a.E[c]];
-      bid[c] = b;
-      any |= (b != 0);
-    }
+  if (col < a.g.nx && row < a.g.ny) {
+    const int c = row * a.g.nx + col;
+    const int b = a.bid2[a.E[c]];
+    bid[c] = b;
+    any = b != 0;
   }
   any = __syncthreads_or(any);
-  if (threadIdx.x == 0) tact_out[t] = any ? 1u : 0u;
+  if (any && threadIdx.x == 0 && threadIdx.y == 0) atomicOr(&tact_out[t], 1u);
 }
 
 struct TileBufs {
@@ -606,12 +603,14 @@ int tiles_count(const GridD& g) { return ((g.nx + TS - 1) / TS) * ((g.ny + TS - 
 // Upstream sums with the tiled contraction.  W_UNIT: counts -> cnt and/or
 // Q = q0*count; W_SADDLE: tag[c] = 1 where a saddle node lies upstream
 // (tiles flagged inactive in tact are skipped: they hold no closed basin).
-// *cycle (host) = 1 when the graph is not a forest (outputs are then not
-// meaningful and the caller must use the exact path).  Syncs once per six
-// reduced-graph rounds (usually once).
+// rounds > 0 with cycle null: the graph is known to be a forest whose reduced
+// depth needs at most `rounds` pointer-jumping rounds (e.g. the label rounds
+// of the same graph); nothing syncs.  Otherwise the rounds run until the list
+// empties, checked on the host every few rounds, and *cycle (host) reports a
+// graph that is not a forest (outputs are then not meaningful).
 int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, const int32_t* bid,
                    const int32_t* snode, float q0, float* Q, uint32_t* cnt, uint8_t* tag, const uint32_t* tact,
-                   int32_t* cycle) {
+                   int32_t* cycle, int rounds) {
   TileBufs b;
   PFF_TRY(tile_bufs(ctx, g, b));
   TileArgs a = tile_args(g, b, ctx, rcv);
@@ -626,8 +625,8 @@ int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, 
   PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_NEXIT, 0, 8, s), "memset"));
   PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_CYCLE, 0, 8, s), "memset"));
   const dim3 tg = tile_grid(g);
-  if (wmode == W_UNIT) hipLaunchKernelGGL((k_tile<0, 1, W_UNIT>), tg, dim3(kBT), 0, s, a);
-  else hipLaunchKernelGGL((k_tile<0, 1, W_SADDLE>), tg, dim3(kBT), 0, s, a);
+  if (wmode == W_UNIT) hipLaunchKernelGGL((k_tile<0, 1, W_UNIT>), tg, dim3(kTB), 0, s, a);
+  else hipLaunchKernelGGL((k_tile<0, 1, W_SADDLE>), tg, dim3(kTB), 0, s, a);
   // reduced graph over the exits: P2 (roots = -1), S holds the in-tile sums
   Lists L;
   L.l[0] = b.L0;
@@ -636,27 +635,32 @@ int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, 
   L.c[1] = ctx->dev_words + W_COUNT1;
   PFF_TRY(check_hip(hipMemcpyAsync(L.c[0], a.n_exits, 8, hipMemcpyDeviceToDevice, s), "copy"));
   hipLaunchKernelGGL(k_exit_link, dim3(list_grid()), dim3(kBT), 0, s, a, b.exits, a.n_exits, 0, b.L0);
-  const int cap = ceil_log2i(g.N) + 1;
-  int done_rounds = 0;
-  int64_t live = 1;
-  while (done_rounds < cap) {
-    const int r = std::min(done_rounds == 0 ? 10 : 6, cap - done_rounds);
-    PFF_TRY(count_rounds(ctx, L, b.S, b.P2, b.t0, (uint32_t*)b.t1, b.t2, r));
-    done_rounds += r;
-    PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_FLAG2, L.c[L.cur], 8, hipMemcpyDeviceToHost, s), "d2h"));
-    PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_CYCLE, a.cycle, 8, hipMemcpyDeviceToHost, s), "d2h"));
-    PFF_TRY(check_hip(hipStreamSynchronize(s), "sync"));
-    live = ctx->host_words[W_FLAG2];
-    if (live == 0 || ctx->host_words[W_CYCLE] != 0) break;
+  if (rounds > 0 && !cycle) {
+    // the caller knows the depth bound (the labels of the same graph)
+    PFF_TRY(count_rounds(ctx, L, b.S, b.P2, b.t0, (uint32_t*)b.t1, b.t2, rounds));
+  } else {
+    const int cap = ceil_log2i(g.N) + 1;
+    int done_rounds = 0;
+    int64_t live = 1;
+    int step = rounds > 0 ? rounds : 8;
+    while (done_rounds < cap) {
+      const int r = std::min(step, cap - done_rounds);
+      PFF_TRY(count_rounds(ctx, L, b.S, b.P2, b.t0, (uint32_t*)b.t1, b.t2, r));
+      done_rounds += r;
+      step = 4;
+      PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_LIVE, L.c[L.cur], 8, hipMemcpyDeviceToHost, s), "d2h"));
+      PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_CYCLE, a.cycle, 8, hipMemcpyDeviceToHost, s), "d2h"));
+      PFF_TRY(check_hip(hipStreamSynchronize(s), "sync"));
+      live = ctx->host_words[W_LIVE];
+      if (live == 0 || ctx->host_words[W_CYCLE] != 0) break;
+    }
+    const int cyc = (live != 0 || ctx->host_words[W_CYCLE] != 0) ? 1 : 0;
+    if (cycle) *cycle = cyc;
+    if (cyc) return cycle ? 0 : fail(PFF_ENOTCONV, "tiled_upstream: receivers are not a forest");
   }
-  if (live != 0 || ctx->host_words[W_CYCLE] != 0) {
-    *cycle = 1;
-    return 0;
-  }
-  *cycle = 0;
   hipLaunchKernelGGL(k_exit_inflow, dim3(list_grid()), dim3(kBT), 0, s, b.exits, a.n_exits, rcv, b.S, b.inflow);
-  if (wmode == W_UNIT) hipLaunchKernelGGL((k_tile<0, 3, W_UNIT>), tg, dim3(kBT), 0, s, a);
-  else hipLaunchKernelGGL((k_tile<0, 3, W_SADDLE>), tg, dim3(kBT), 0, s, a);
+  if (wmode == W_UNIT) hipLaunchKernelGGL((k_tile<0, 3, W_UNIT>), tg, dim3(kTB), 0, s, a);
+  else hipLaunchKernelGGL((k_tile<0, 3, W_SADDLE>), tg, dim3(kTB), 0, s, a);
   return check_hip(hipGetLastError(), "tiled_upstream");
 }
 
@@ -668,7 +672,7 @@ int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, 
 // basin.  *ndep, *npits, *cycle are host outputs; syncs once (per six
 // reduced-graph rounds).
 int tiled_labels(pff_ctx* ctx, const GridD& g, const int32_t* rec, const uint32_t* tact_in, int64_t* ndep,
-                 int32_t* cycle) {
+                 int32_t* cycle, int* rounds_used) {
   TileBufs b;
   PFF_TRY(tile_bufs(ctx, g, b));
   TileArgs a = tile_args(g, b, ctx, rec);
@@ -678,7 +682,7 @@ int tiled_labels(pff_ctx* ctx, const GridD& g, const int32_t* rec, const uint32_
   PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_CYCLE, 0, 8, s), "memset"));
   PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_NPIT, 0, 8, s), "memset"));
   const dim3 tg = tile_grid(g);
-  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL((k_tile<M, 1, W_ROOTS>), tg, dim3(kBT), 0, s, a));
+  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL((k_tile<M, 1, W_ROOTS>), tg, dim3(kTB), 0, s, a));
   Lists L;
   L.l[0] = b.L0;
   L.l[1] = b.L1;
@@ -686,23 +690,35 @@ int tiled_labels(pff_ctx* ctx, const GridD& g, const int32_t* rec, const uint32_
   L.c[1] = ctx->dev_words + W_COUNT1;
   PFF_TRY(check_hip(hipMemcpyAsync(L.c[0], a.n_exits, 8, hipMemcpyDeviceToDevice, s), "copy"));
   hipLaunchKernelGGL(k_exit_link, dim3(list_grid()), dim3(kBT), 0, s, a, b.exits, a.n_exits, 1, b.L0);
+  // start from the previous call's round count, then confirm on the host
   const int cap = ceil_log2i(g.N) + 1;
+  PFF_TRY(launch_fill_i64(ctx, 1, ctx->dev_words + W_EMPTY, (int64_t)1 << 40));
   int done_rounds = 0;
   int64_t live = 1;
+  int step = std::max(2, std::min(ctx->last_label_rounds + 1, cap));
   while (done_rounds < cap) {
-    const int r = std::min(done_rounds == 0 ? 10 : 6, cap - done_rounds);
+    const int r = std::min(step, cap - done_rounds);
     PFF_TRY(label_rounds_run(ctx, L, b.bid2, b.P2, b.t0, b.t1, nullptr, g.N, nullptr, done_rounds, r));
     done_rounds += r;
-    // one readback for the live count, the cycle flag and the pit count
-    PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_FLAG2, L.c[L.cur], 8, hipMemcpyDeviceToHost, s), "d2h"));
+    step = 3;
+    // one readback: live count (a cycle across tiles), in-tile cycle flag, pits
+    PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_LIVE, L.c[L.cur], 8, hipMemcpyDeviceToHost, s), "d2h"));
     PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_CYCLE, a.cycle, 8, hipMemcpyDeviceToHost, s), "d2h"));
     PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_NPIT, a.n_pits, 8, hipMemcpyDeviceToHost, s), "d2h"));
+    PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_EMPTY, ctx->dev_words + W_EMPTY, 8, hipMemcpyDeviceToHost, s),
+                      "d2h"));
     PFF_TRY(check_hip(hipStreamSynchronize(s), "sync"));
-    live = ctx->host_words[W_FLAG2];
+    live = ctx->host_words[W_LIVE];
     if (live == 0 || ctx->host_words[W_CYCLE] != 0) break;
   }
+  // rounds that actually had work (the first empty round index, or all)
+  const int64_t fe = ctx->host_words[W_EMPTY];
+  const int needed = (live == 0 && fe < done_rounds) ? (int)fe : done_rounds;
+  ctx->last_label_rounds = std::max(needed, 1);
+  done_rounds = ctx->last_label_rounds + 1;
   *ndep = ctx->host_words[W_NPIT];
   *cycle = (live != 0 || ctx->host_words[W_CYCLE] != 0) ? 1 : 0;
+  if (rounds_used) *rounds_used = done_rounds;
   return 0;
 }
 
@@ -711,7 +727,9 @@ int tiled_labels_write(pff_ctx* ctx, const GridD& g, int32_t* bid, const uint32_
   PFF_TRY(tile_bufs(ctx, g, b));
   TileArgs a = tile_args(g, b, ctx, nullptr);
   a.tact = tact_in;
-  hipLaunchKernelGGL(k_label_tiles, tile_grid(g), dim3(kBT), 0, ctx->stream, a, bid, tact_out);
+  PFF_TRY(check_hip(hipMemsetAsync(tact_out, 0, (size_t)tiles_count(g) * 4, ctx->stream), "memset"));
+  hipLaunchKernelGGL(k_label_tiles, dim3((g.nx + 63) / 64, (g.ny + 3) / 4), dim3(64, 4), 0, ctx->stream, a, bid,
+                     tact_out);
   return check_hip(hipGetLastError(), "tiled_labels_write");
 }
 

@@ -405,24 +405,18 @@ int carve_from_levels(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t*
 // ---------------------------------------------------------------------------
 constexpr int TSL = 64;  // tile side, as in the tiled contraction
 
-#define PFF_TILE_PROLOGUE(tact, tiles_x)                                          \
-  const int bx_ = blockIdx.x, by_ = blockIdx.y;                                  \
-  if (tact && tact[by_ * (tiles_x) + bx_] == 0u) return;                         \
-  const int x0_ = bx_ * TSL, y0_ = by_ * TSL;                                    \
-  const int tw_ = min(TSL, g.nx - x0_), th_ = min(TSL, g.ny - y0_);
+// One cell per lane: 64x4 blocks, 16 blocks per 64x64 tile; a block whose
+// tile is inactive returns at once.
+#define PFF_CELL_PROLOGUE(tact, tiles_x)                                       \
+  int bx_, by_;                                                                \
+  xcd_block(bx_, by_);                                                         \
+  if (tact && tact[(by_ >> 4) * (tiles_x) + bx_] == 0u) return;                \
+  const int col = bx_ * 64 + threadIdx.x;                                      \
+  const int row = by_ * 4 + threadIdx.y;                                       \
+  if (col >= g.nx || row >= g.ny) return;                                      \
+  const int i = row * g.nx + col;
 
-template <typename F>
-__device__ __forceinline__ void tile_for(const GridD& g, int x0, int y0, int tw, int th, F&& f) {
-#pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    const int li = threadIdx.x + j * 256;
-    const int ly = li >> 6, lx = li & 63;
-    if (lx < tw && ly < th) {
-      const int row = y0 + ly, col = x0 + lx;
-      f(row * g.nx + col, row, col);
-    }
-  }
-}
+static dim3 cell_grid(const GridD& g) { return dim3((g.nx + 63) / 64, (g.ny + 3) / 4); }
 
 __global__ __launch_bounds__(kBT) void k_saddle_init_list(const int32_t* __restrict__ pits, const int64_t* np,
                                                           int64_t* saddle, int64_t* outlet, int32_t* snode) {
@@ -443,8 +437,17 @@ __global__ __launch_bounds__(kBT) void k_zprime_min_t(GridD g, int tiles_x, cons
                                                       const int32_t* __restrict__ bid, const float* __restrict__ z,
                                                       uint8_t* __restrict__ isb, float* __restrict__ zp,
                                                       int64_t* saddle) {
-  PFF_TILE_PROLOGUE(tact, tiles_x)
-  tile_for(g, x0_, y0_, tw_, th_, [&](int i, int row, int col) {
+  // no early return: the whole wave takes part in the segmented minimum
+  int bx_, by_;
+  xcd_block(bx_, by_);
+  if (tact && tact[(by_ >> 4) * tiles_x + bx_] == 0u) return;  // uniform per block
+  const int col = bx_ * 64 + threadIdx.x;
+  const int row = by_ * 4 + threadIdx.y;
+  const bool inside = col < g.nx && row < g.ny;
+  const int i = inside ? row * g.nx + col : 0;
+  int keyb = -1 - (int)threadIdx.x;  // lanes without a candidate never match another lane
+  int64_t res = INVALID_KEY;
+  if (inside) {
     const float zi = z[i];
     if (can_leave<MODE>(g, i, row, col)) {
       zp[i] = zi;
@@ -469,17 +472,29 @@ __global__ __launch_bounds__(kBT) void k_zprime_min_t(GridD g, int tiles_x, cons
       const float zpi = border ? fmaxf(zi, zn) : 1e9f;
       zp[i] = zpi;
       if (border && bi != 0 && bi < g.N) {
-        int64_t res = INVALID_KEY;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
           if (nbj[k] < 0) continue;
           const int64_t c = pack_key(zpi, nbj[k]);
           res = c < res ? c : res;
         }
-        if (res != INVALID_KEY && res < saddle[bi]) atomicMin((long long*)&saddle[bi], (long long)res);
+        if (res != INVALID_KEY) keyb = bi;
       }
     }
-  });
+  }
+  // segmented minimum over runs of lanes of the same basin (64 lanes = 64
+  // consecutive cells of a row); only the first lane of a run issues the
+  // atomic, after a plain read shows it can lower the stored minimum
+  const int lane = threadIdx.x;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int ob = __shfl_down(keyb, d);
+    const int64_t ov = __shfl_down(res, d);
+    if (lane + d < 64 && ob == keyb && ov < res) res = ov;
+  }
+  const int prevb = __shfl_up(keyb, 1);
+  const bool head = keyb >= 0 && (lane == 0 || prevb != keyb);
+  if (head && res < saddle[keyb]) atomicMin((long long*)&saddle[keyb], (long long)res);
 }
 
 // pass 3 (:199-219), largest cell index wins
@@ -488,23 +503,21 @@ __global__ __launch_bounds__(kBT) void k_saddle_node_t(GridD g, int tiles_x, con
                                                        const int32_t* __restrict__ bid, const uint8_t* __restrict__ isb,
                                                        const float* __restrict__ zp, const int64_t* __restrict__ saddle,
                                                        int32_t* snode) {
-  PFF_TILE_PROLOGUE(tact, tiles_x)
-  tile_for(g, x0_, y0_, tw_, th_, [&](int i, int row, int col) {
-    const int bi = bid[i];
-    if (bi != 0 && bi < g.N && !can_leave<MODE>(g, i, row, col) && isb[i]) {
-      const int64_t key = saddle[bi];
-      const float tz = key_float(key);
-      const int tb = key_index(key);
-      bool here = false;
+  PFF_CELL_PROLOGUE(tact, tiles_x)
+  const int bi = bid[i];
+  if (bi == 0 || bi >= g.N || can_leave<MODE>(g, i, row, col) || !isb[i]) return;
+  const int64_t key = saddle[bi];
+  const float tz = key_float(key);
+  const int tb = key_index(key);
+  if (zp[i] != tz) return;
+  bool here = false;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int jn = neighbour<MODE>(g, i, row, col, k);
-        if (jn < 0) continue;
-        if (bid[jn] == tb && zp[i] == tz) here = true;
-      }
-      if (here) atomicMax(&snode[bi], i);
-    }
-  });
+  for (int k = 0; k < 4; ++k) {
+    const int jn = neighbour<MODE>(g, i, row, col, k);
+    if (jn < 0) continue;
+    if (bid[jn] == tb) here = true;
+  }
+  if (here) atomicMax(&snode[bi], i);
 }
 
 // pass 4 (:226-250) over the pits
@@ -590,11 +603,11 @@ int saddlesort_active(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t*
   PFF_TRY(scratch(ctx, S_B0, (size_t)g.N, &inv));
   hipStream_t s = ctx->stream;
   const int tiles_x = (g.nx + TSL - 1) / TSL;
-  const dim3 tg(tiles_x, (g.ny + TSL - 1) / TSL), lg(1024), bl(kBT);
+  const dim3 tg = cell_grid(g), lg(1024), bl(kBT), cb(64, 4);
   hipLaunchKernelGGL(k_saddle_init_list, lg, bl, 0, s, pits, npits, saddle, outlet, snode);
-  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_zprime_min_t<M>, tg, bl, 0, s, g, tiles_x, tact, bid, z, isb, zp,
+  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_zprime_min_t<M>, tg, cb, 0, s, g, tiles_x, tact, bid, z, isb, zp,
                                                saddle));
-  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_saddle_node_t<M>, tg, bl, 0, s, g, tiles_x, tact, bid,
+  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_saddle_node_t<M>, tg, cb, 0, s, g, tiles_x, tact, bid,
                                                (const uint8_t*)isb, (const float*)zp, (const int64_t*)saddle, snode));
   PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_outlet_list<M>, lg, bl, 0, s, g, pits, npits, bid, z,
                                                (const int64_t*)saddle, (const int32_t*)snode, outlet));
@@ -608,15 +621,11 @@ int saddlesort_active(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t*
 __global__ __launch_bounds__(kBT) void k_carve_reverse_t(GridD g, int tiles_x, const uint32_t* __restrict__ tact,
                                                          const uint8_t* __restrict__ tag,
                                                          const int32_t* __restrict__ rec_in, int32_t* rec) {
-  PFF_TILE_PROLOGUE(tact, tiles_x)
-  tile_for(g, x0_, y0_, tw_, th_, [&](int i, int row, int col) {
-    (void)row;
-    (void)col;
-    if (tag[i]) {
-      const int r = rec_in[i];
-      if (r != i && (unsigned)r < (unsigned)g.N && tag[r]) rec[r] = i;
-    }
-  });
+  PFF_CELL_PROLOGUE(tact, tiles_x)
+  if (tag[i]) {
+    const int r = rec_in[i];
+    if (r != i && (unsigned)r < (unsigned)g.N && tag[r]) rec[r] = i;
+  }
 }
 
 // saddle node -> outlet (lakeflow.py:399-403) over the pits
@@ -635,12 +644,8 @@ __global__ __launch_bounds__(kBT) void k_carve_link_list(const int32_t* __restri
 __global__ __launch_bounds__(kBT) void k_carve_sync_t(GridD g, int tiles_x, const uint32_t* __restrict__ tact,
                                                       const uint8_t* __restrict__ tag, const int32_t* __restrict__ rec,
                                                       int32_t* __restrict__ rec_work) {
-  PFF_TILE_PROLOGUE(tact, tiles_x)
-  tile_for(g, x0_, y0_, tw_, th_, [&](int i, int row, int col) {
-    (void)row;
-    (void)col;
-    if (tag[i]) rec_work[i] = rec[i];
-  });
+  PFF_CELL_PROLOGUE(tact, tiles_x)
+  if (tag[i]) rec_work[i] = rec[i];
 }
 
 // Carve for one trip with the tiled contraction: the cells the reference
@@ -651,17 +656,19 @@ __global__ __launch_bounds__(kBT) void k_carve_sync_t(GridD g, int tiles_x, cons
 // (nothing is written then).
 int carve_tiled(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t* rec_in, const int32_t* bid,
                 const int32_t* snode, const int64_t* outlet, int32_t* rec_work, const uint32_t* tact,
-                const int32_t* pits, const int64_t* npits, int32_t* cycle) {
+                const int32_t* pits, const int64_t* npits, int rounds, int32_t* cycle) {
   void* tag;
   PFF_TRY(scratch(ctx, S_B1, (size_t)g.N, &tag));
-  PFF_TRY(tiled_upstream(ctx, g, rec_in, 2, bid, snode, 0.0f, nullptr, nullptr, (uint8_t*)tag, tact, cycle));
-  if (*cycle) return 0;
+  // rec_in was proved a forest by this trip's labels: no cycle check needed
+  PFF_TRY(tiled_upstream(ctx, g, rec_in, 2, bid, snode, 0.0f, nullptr, nullptr, (uint8_t*)tag, tact, nullptr,
+                         rounds));
+  *cycle = 0;
   const int tiles_x = (g.nx + TSL - 1) / TSL;
-  const dim3 tg(tiles_x, (g.ny + TSL - 1) / TSL), bl(kBT);
-  hipLaunchKernelGGL(k_carve_reverse_t, tg, bl, 0, ctx->stream, g, tiles_x, tact, (const uint8_t*)tag, rec_in, rec);
+  const dim3 tg = cell_grid(g), bl(kBT), cb(64, 4);
+  hipLaunchKernelGGL(k_carve_reverse_t, tg, cb, 0, ctx->stream, g, tiles_x, tact, (const uint8_t*)tag, rec_in, rec);
   hipLaunchKernelGGL(k_carve_link_list, dim3(1024), bl, 0, ctx->stream, pits, npits, outlet, snode, rec);
   if (rec_work)
-    hipLaunchKernelGGL(k_carve_sync_t, tg, bl, 0, ctx->stream, g, tiles_x, tact, (const uint8_t*)tag,
+    hipLaunchKernelGGL(k_carve_sync_t, tg, cb, 0, ctx->stream, g, tiles_x, tact, (const uint8_t*)tag,
                        (const int32_t*)rec, rec_work);
   return check_hip(hipGetLastError(), "carve_tiled");
 }

@@ -8,12 +8,14 @@ namespace pff {
 
 constexpr int kBT = 256;  // threads per block for block helpers
 
-// Exclusive prefix of v over the block; *total receives the block sum.
-// Must be reached by every thread of the block.
+// Exclusive prefix of v over a block of BT threads (1-D or flattened);
+// *total receives the block sum.  Must be reached by every thread.
+template <int BT = kBT>
 __device__ __forceinline__ int block_exclusive_scan(int v, int* total) {
-  __shared__ int wsum[kBT / 64];
-  const int lane = threadIdx.x & 63;
-  const int wid = threadIdx.x >> 6;
+  __shared__ int wsum[BT / 64];
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  const int lane = tid & 63;
+  const int wid = tid >> 6;
   int x = v;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
@@ -24,7 +26,7 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* total) {
   __syncthreads();
   int woff = 0, tot = 0;
 #pragma unroll
-  for (int w = 0; w < kBT / 64; ++w) {
+  for (int w = 0; w < BT / 64; ++w) {
     const int s = wsum[w];
     woff += (w < wid) ? s : 0;
     tot += s;
@@ -36,7 +38,7 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* total) {
 
 // Append the taken values of every thread (up to ITEMS each) to list[*count..]
 // with a single atomicAdd per block.  Must be reached by every thread.
-template <int ITEMS>
+template <int ITEMS, int BT = kBT>
 __device__ __forceinline__ void block_append(const int32_t (&val)[ITEMS], const bool (&take)[ITEMS], int32_t* list,
                                              int64_t* count) {
   __shared__ int64_t base_s;
@@ -44,8 +46,8 @@ __device__ __forceinline__ void block_append(const int32_t (&val)[ITEMS], const 
 #pragma unroll
   for (int j = 0; j < ITEMS; ++j) mine += take[j] ? 1 : 0;
   int total;
-  const int off = block_exclusive_scan(mine, &total);
-  if (threadIdx.x == 0) base_s = total ? (int64_t)atomicAdd((unsigned long long*)count, (unsigned long long)total) : 0;
+  const int off = block_exclusive_scan<BT>(mine, &total);
+  if (threadIdx.x == 0 && threadIdx.y == 0) base_s = total ? (int64_t)atomicAdd((unsigned long long*)count, (unsigned long long)total) : 0;
   __syncthreads();
   int64_t p = base_s + off;
 #pragma unroll
@@ -55,10 +57,11 @@ __device__ __forceinline__ void block_append(const int32_t (&val)[ITEMS], const 
 }
 
 // Block-wide sum with one atomicAdd on a 64-bit counter.
+template <int BT = kBT>
 __device__ __forceinline__ void block_count(int mine, int64_t* counter) {
   int total;
-  (void)block_exclusive_scan(mine, &total);
-  if (threadIdx.x == 0 && total) atomicAdd((unsigned long long*)counter, (unsigned long long)total);
+  (void)block_exclusive_scan<BT>(mine, &total);
+  if (threadIdx.x == 0 && threadIdx.y == 0 && total) atomicAdd((unsigned long long*)counter, (unsigned long long)total);
 }
 
 }  // namespace pff

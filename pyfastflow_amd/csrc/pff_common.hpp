@@ -129,6 +129,22 @@ __device__ __forceinline__ int32_t key_index(int64_t key) { return (int32_t)(uin
 // (lakeflow.py:127)
 constexpr int64_t INVALID_KEY = (int64_t)0x4CBEBC200000002Aull;
 
+// XCD-aware block order.  Workgroups are dispatched round-robin over the 8
+// XCDs (each with its own L2) in linear block order; this remaps the linear
+// id so that XCD k processes the k-th contiguous eighth of the 2-D grid in
+// row-major order.  A stencil block then finds the rows of its vertical
+// neighbours in its own L2.
+constexpr int kXCD = 8;
+__device__ __forceinline__ void xcd_block(int& bx, int& by) {
+  const int gx = gridDim.x;
+  const int total = gx * gridDim.y;
+  const int p = blockIdx.y * gx + blockIdx.x;
+  const int per = total / kXCD;
+  const int l = (p < per * kXCD) ? (p % kXCD) * per + p / kXCD : p;
+  bx = l % gx;
+  by = l / gx;
+}
+
 __device__ __forceinline__ void rc_of(const GridD& g, int i, int& row, int& col) {
   row = i / g.nx;
   col = i - row * g.nx;

@@ -20,6 +20,8 @@ struct pff_ctx {
   int64_t* host_words = nullptr;
   // device words: counters and flags
   int64_t* dev_words = nullptr;
+  // reduced-graph rounds the last tiled labelling needed (a start estimate)
+  int last_label_rounds = 8;
 };
 
 namespace pff {
@@ -45,7 +47,7 @@ enum Slot : int {
 };
 
 // device word indices (dev_words / host_words)
-enum Word : int { W_COUNT0 = 0, W_COUNT1, W_COUNT2, W_FLAG, W_FLAG2, W_CYCLE, W_NEXIT, W_NDEP, W_NPIT, W_NWORDS = 16 };
+enum Word : int { W_COUNT0 = 0, W_COUNT1, W_COUNT2, W_FLAG, W_FLAG2, W_CYCLE, W_NEXIT, W_NDEP, W_NPIT, W_LIVE, W_EMPTY, W_NWORDS = 16 };
 
 constexpr int kBlock = 256;
 inline unsigned grid_for(int64_t n, int block = kBlock, int64_t cap = 1 << 20) {
@@ -80,16 +82,16 @@ int launch_saddlesort(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t*
 // tiled contraction (forests; *cycle = 1 when the receivers are not a forest)
 int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, const int32_t* bid,
                    const int32_t* snode, float q0, float* Q, uint32_t* cnt, uint8_t* tag, const uint32_t* tact,
-                   int32_t* cycle);
+                   int32_t* cycle, int rounds = 0);
 int tiled_labels(pff_ctx* ctx, const GridD& g, const int32_t* rec, const uint32_t* tact_in, int64_t* ndep,
-                 int32_t* cycle);
+                 int32_t* cycle, int* rounds_used = nullptr);
 int tiled_labels_write(pff_ctx* ctx, const GridD& g, int32_t* bid, const uint32_t* tact_in, uint32_t* tact_out);
 int tiled_pits(pff_ctx* ctx, const GridD& g, int32_t** pits, int64_t** npits);
 int tile_flags(pff_ctx* ctx, const GridD& g, uint32_t** flags_a, uint32_t** flags_b);
 int tiles_count(const GridD& g);
 int carve_tiled(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t* rec_in, const int32_t* bid,
                 const int32_t* snode, const int64_t* outlet, int32_t* rec_work, const uint32_t* tact,
-                const int32_t* pits, const int64_t* npits, int32_t* cycle);
+                const int32_t* pits, const int64_t* npits, int rounds, int32_t* cycle);
 int saddlesort_active(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t* isb, float* zp, int64_t* saddle,
                       int32_t* snode, int64_t* outlet, const float* z, const uint32_t* tact, const int32_t* pits,
                       const int64_t* npits);

@@ -1,0 +1,65 @@
+"""compute_receivers of the oracle (receivers.py:15-58) against a vectorised
+numpy closed form and hand cases."""
+
+import numpy as np
+import pytest
+
+from oracle_util import receivers_closed_form
+from pyfastflow_amd import synth
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("ny,nx,seed", [(2, 2, 1), (5, 7, 2), (33, 17, 3), (64, 64, 4), (100, 80, 5)])
+def test_closed_form(ora, ny, nx, seed, mode):
+    z = synth.dem(ny, nx, 2.0, 300.0, seed, n_pits=2, dx=10.0)
+    rcv, grad = ora.compute_receivers(ora.Grid(nx, ny, 10.0, mode), z)
+    r2, g2 = receivers_closed_form(z, ny, nx, mode, 10.0)
+    np.testing.assert_array_equal(rcv, r2)
+    np.testing.assert_array_equal(grad.view(np.uint32), g2.view(np.uint32))
+
+
+def test_tilted_plane_drains_south(ora):
+    ny, nx = 6, 5
+    z = (np.arange(ny)[::-1, None] * 1.0 + np.zeros((1, nx))).astype(np.float32)
+    rcv, grad = ora.compute_receivers(ora.Grid(nx, ny, 1.0, 0), z)
+    i = np.arange(ny * nx)
+    south = i + nx
+    interior_rows = i // nx < ny - 1
+    np.testing.assert_array_equal(rcv[interior_rows], south[interior_rows])
+    np.testing.assert_array_equal(rcv[~interior_rows], i[~interior_rows])
+    assert np.all(grad[interior_rows] == 1.0)
+
+
+def test_flat_cells_are_their_own_receiver(ora):
+    z = np.full((4, 4), 3.0, np.float32)
+    rcv, grad = ora.compute_receivers(ora.Grid(4, 4, 1.0, 0), z)
+    np.testing.assert_array_equal(rcv, np.arange(16))
+    assert np.all(grad == 0)
+
+
+def test_equal_drops_take_the_first_direction(ora):
+    """top (0) before left (1) before right (2) before bottom (3)."""
+    z = np.full((3, 3), 5.0, np.float32)
+    z[1, 1] = 6.0  # centre; all four neighbours 1 lower
+    rcv, _ = ora.compute_receivers(ora.Grid(3, 3, 1.0, 0), z)
+    assert rcv[4] == 1  # top
+    z[0, 1] = 5.5  # top now shallower: left wins
+    rcv, _ = ora.compute_receivers(ora.Grid(3, 3, 1.0, 0), z)
+    assert rcv[4] == 3
+
+
+def test_periodic_wrap_column(ora):
+    z = np.full((3, 4), 10.0, np.float32)
+    z[1, 3] = 1.0  # east column, lowest
+    z[1, 0] = 5.0  # west column: its left neighbour wraps to the east column
+    rcv, _ = ora.compute_receivers(ora.Grid(4, 3, 1.0, 1), z)
+    assert rcv[1 * 4 + 0] == 1 * 4 + 3
+    rcv0, _ = ora.compute_receivers(ora.Grid(4, 3, 1.0, 0), z)
+    assert rcv0[1 * 4 + 0] == 1 * 4 + 0  # no wrap in normal mode: self
+
+
+def test_division_ambiguity_is_counted(ora):
+    """The oracle reports cells where (zi-zj)/dx and (zi-zj)*fl(1/dx) pick
+    different receivers; for the 512^2 plain workload (dx=1) there are none."""
+    c = synth.plain_small(n=128)
+    assert ora.receivers_div_ambiguity(ora.Grid(128, 128, 1.0, 0), c["z"]) == 0
