@@ -38,6 +38,9 @@ def parse():
     p.add_argument("--size", type=int, default=4096)
     p.add_argument("--pits", type=int, default=2000)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and full-size parity check")
+    p.add_argument("--pmc-calibrate", action="store_true",
+                   help="after the steps, run a 1 GiB read + 1 GiB write elementwise pass 3 times "
+                        "(scale for rocprofv3 FETCH/WRITE_SIZE)")
     return p.parse_args()
 
 
@@ -115,12 +118,47 @@ def main():
     stages = {"receivers_ms": round(t_rcv, 4), "lakeflow_ms": round(t_lake, 4), "accumulate_ms": round(t_acc, 4)}
     log(f"[bench] stages: {stages}")
 
-    # ---- roofline of the receivers kernel (12 B/cell: z in, rcv + gradient out)
-    bytes_rcv = 12.0 * N
-    achieved = bytes_rcv / (t_rcv * 1e-3) / 1e9
+    # ---- roofline of the dominant kernel: the tile contraction (k_tile), timed
+    #      live with HIP events around each launch on the launch stream during
+    #      one extra step; algorithmic bytes per launch = bytes per cell of its
+    #      variant x cells it processed (active tiles)
+    import ctypes
+
+    ctx = _lib.ctx_for()
+    _lib.call("pff_ctx_profile", ctx, 1)
+    step()
+    n_l, t_ms, t_b = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
+    _lib.call("pff_ctx_profile_report", ctx, b"k_tile", ctypes.byref(n_l), ctypes.byref(t_ms), ctypes.byref(t_b))
+    _lib.call("pff_ctx_profile", ctx, 0)
+    launches = max(int(n_l.value), 1)
+    avg_us = 1e3 * t_ms.value / launches
+    bytes_launch = t_b.value / launches
+    achieved = t_b.value / (t_ms.value * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(HERE, "profiles", "k_tile_traffic.json")
+    if os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_receivers",
-                "bytes_per_launch": bytes_rcv}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "k_tile",
+                "launches_per_step": launches, "avg_launch_us": round(avg_us, 2),
+                "bytes_per_launch": round(bytes_launch)}
+    log(f"[bench] roofline {roofline}")
+    bytes_rcv = 12.0 * N
+    rcv_roof = {"kernel": "k_receivers", "us": round(t_rcv * 1e3, 2),
+                "achieved_GBs": round(bytes_rcv / (t_rcv * 1e-3) / 1e9, 1)}
+
+    if args.pmc_calibrate:
+        # an elementwise kernel that reads 1 GiB and writes 1 GiB (four times
+        # the last-level cache, so the read comes from HBM): its FETCH_SIZE and
+        # WRITE_SIZE per dispatch give the counters' scale to bytes
+        a = torch.ones(256 * 2**20, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            torch.add(a, 1.0, out=b)
+        torch.cuda.synchronize()
+        log("[bench] calibration: 3 elementwise passes reading and writing 1 GiB each")
+        del a, b
 
     # ---- CPU baseline (oracle, the reference algorithm restated in C) and
     #      full-size parity of the GPU result against it ----------------------
@@ -173,7 +211,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"depressions {n}x{n} periodic_EW dx=10 m, {pits} Gaussian pits, spectral beta=2.0 "
                                f"relief 1000 m seed 202; receivers + reroute_flow(carve) + accumulate_constant_Q(1e-5)",
-                   "cells": N, "stages": stages, "lakeflow": lake, "parity": parity},
+                   "cells": N, "stages": stages, "lakeflow": lake, "parity": parity, "receivers_kernel": rcv_roof},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -42,9 +42,14 @@ typedef struct {
 } pff_grid;
 
 /* Hydraulic parameters; mirror constants.PREC/MANNING/EDGESW/DT_HYDRO
-   (constants.py:93-103) as set by Flooder (gf_fields.py:62-108). */
+   (constants.py:93-103) as set by Flooder (gf_fields.py:62-108).
+   update: 0 = the reference's depth update (h receives dh twice, zh once,
+   gf_hydrodynamics.py:121-129); 1 = h and zh receive dh once and h is
+   clamped at 0 (no reference counterpart; the reference's update has no
+   steady state once lakes are filled). */
 typedef struct {
   double prec, manning, edge_slope, dt;
+  int32_t update;
 } pff_hydro;
 
 typedef struct pff_ctx pff_ctx;
@@ -58,6 +63,14 @@ int pff_ctx_create(pff_ctx** out, int32_t device, void* hip_stream);
 int pff_ctx_destroy(pff_ctx* ctx);
 int pff_ctx_set_stream(pff_ctx* ctx, void* hip_stream);
 int pff_ctx_synchronize(pff_ctx* ctx);
+/* Opt-in per-kernel timing, no reference counterpart: while enabled, the
+   instrumented launches (the tile contraction, "k_tile") are bracketed by
+   HIP events on the context stream and count the cells they process.
+   Enabling clears the records; report() syncs and sums the launches of one
+   kernel: count, time (ms) and algorithmic bytes. */
+int pff_ctx_profile(pff_ctx* ctx, int32_t enable);
+int pff_ctx_profile_report(pff_ctx* ctx, const char* kernel, int64_t* launches, double* total_ms,
+                           double* total_bytes);
 
 /* ---- flow: receivers -------------------------------------------------- */
 /* receivers.py:15-58  compute_receivers(z, receivers, gradient) */
@@ -140,6 +153,31 @@ int pff_diffuse_Q_constant_prec(pff_ctx* ctx, const pff_grid* g, const pff_hydro
 /* gf_hydrodynamics.py:78-129  graphflood_core_cte_mannings(h, zh, dh, rec, Q) */
 int pff_graphflood_core_cte_mannings(pff_ctx* ctx, const pff_grid* g, const pff_hydro* p, float* h, float* zh,
                                      float* dh, const int32_t* rec, const float* Q);
+
+/* Per-call result of the GraphFlood drivers (host struct). */
+typedef struct {
+  int64_t iterations;      /* iterations run by this call */
+  double residual;         /* max |h(t) - h(t - check_every)| at the last check, m */
+  double first_residual;   /* the same at the first check */
+  int32_t converged;       /* 1 when residual < tol at a check */
+  int32_t checks;          /* number of residual checks */
+  int64_t nan_cells;       /* cells whose h is NaN at the end (h < 0 makes h^(5/3) NaN) */
+} pff_gf_status;
+
+/* gf_fields.py:111-141  Flooder.run_graphflood(N): n_iter iterations of
+   compute_receivers(zh) -> reroute_flow(carve) -> fill_z_add_delta(fill_eps)
+   -> n_diffuse x diffuse_Q_constant_prec -> graphflood_core_cte_mannings.
+   zh (router.z), h and Q are updated in place; rec receives the receivers of
+   the last iteration.  Scratch fields live in the context.  Syncs once per
+   lakeflow trip.  st (host, nullable). */
+int pff_graphflood_run(pff_ctx* ctx, const pff_grid* g, const pff_hydro* p, float* zh, float* h, float* Q,
+                       int32_t* rec, int32_t n_iter, int32_t n_diffuse, double fill_eps, pff_gf_status* st);
+/* The same iteration until max |h(t) - h(t - check_every)| < tol, checked
+   every check_every iterations (steady state), or max_iter iterations.  No
+   reference counterpart: the reference runs a fixed count (gf_fields.py:126). */
+int pff_graphflood_to_steady(pff_ctx* ctx, const pff_grid* g, const pff_hydro* p, float* zh, float* h, float* Q,
+                             int32_t* rec, double tol, int32_t check_every, int64_t max_iter, int32_t n_diffuse,
+                             double fill_eps, pff_gf_status* st);
 
 /* ---- packed keys ------------------------------------------------------- */
 /* f32_i32_struct.py:123-135  pack_full_float_index(arr, ft, it) */

@@ -856,8 +856,12 @@ void ora_graphflood_core_cte_mannings(const ora_grid* g, const ora_hydro* p, flo
     }
     const float Qo = dx * powf(h[i], e53) / man * sqrtf(tS);
     float tdh = (Q[i] - Qo) / dx2 * dt;
-    h[i] += tdh;
-    if (h[i] < 0.0f) tdh += h[i];
+    if (p->update == 0) {
+      h[i] += tdh;
+      if (h[i] < 0.0f) tdh += h[i];
+    } else if (h[i] + tdh < 0.0f) {
+      tdh = -h[i]; /* single update with h >= 0 (not the reference's update) */
+    }
     dh[i] = tdh;
   }
 #pragma omp parallel for schedule(static)
@@ -881,8 +885,12 @@ void ora_graphflood_core_cte_mannings_f64(const ora_grid* g, const ora_hydro* p,
     }
     const double Qo = dx * pow(h[i], 5.0 / 3.0) / p->manning * sqrt(tS);
     double tdh = (Q[i] - Qo) / (dx * dx) * p->dt;
-    h[i] += tdh;
-    if (h[i] < 0.0) tdh += h[i];
+    if (p->update == 0) {
+      h[i] += tdh;
+      if (h[i] < 0.0) tdh += h[i];
+    } else if (h[i] + tdh < 0.0) {
+      tdh = -h[i];
+    }
     dh[i] = tdh;
   }
   for (int64_t i = 0; i < N; ++i) {

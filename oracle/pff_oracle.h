@@ -101,7 +101,7 @@ void ora_fill_z_add_delta_f64(const ora_grid* g, double* zh, double* h, const in
 void ora_topofill(const ora_grid* g, float* z, const int32_t* rec, double eps);
 
 /* GraphFlood kernels */
-typedef struct { double prec, manning, edge_slope, dt; } ora_hydro;
+typedef struct { double prec, manning, edge_slope, dt; int32_t update; } ora_hydro;
 void ora_diffuse_Q_constant_prec(const ora_grid* g, const ora_hydro* p, const float* zh, float* Q);
 void ora_diffuse_Q_constant_prec_f64(const ora_grid* g, const ora_hydro* p, const double* zh, double* Q);
 void ora_graphflood_core_cte_mannings(const ora_grid* g, const ora_hydro* p, float* h, float* zh,

@@ -28,7 +28,7 @@ class OraGrid(ctypes.Structure):
 
 
 class OraHydro(ctypes.Structure):
-    _fields_ = [("prec", c_d), ("manning", c_d), ("edge_slope", c_d), ("dt", c_d)]
+    _fields_ = [("prec", c_d), ("manning", c_d), ("edge_slope", c_d), ("dt", c_d), ("update", c_int)]
 
 
 class OraLakeLog(ctypes.Structure):
@@ -288,8 +288,8 @@ def topofill(g: Grid, z, rec, eps=1e-4):
     return z
 
 
-def hydro(prec=10e-3 / 3600, manning=0.033, edge_slope=1e-2, dt=5e-3):
-    return OraHydro(prec, manning, edge_slope, dt)
+def hydro(prec=10e-3 / 3600, manning=0.033, edge_slope=1e-2, dt=5e-3, update=0):
+    return OraHydro(prec, manning, edge_slope, dt, update)
 
 
 def diffuse(g: Grid, hy, zh, Q, f64=False):

@@ -28,7 +28,7 @@ class PffGrid(ctypes.Structure):
 
 
 class PffHydro(ctypes.Structure):
-    _fields_ = [("prec", c_d), ("manning", c_d), ("edge_slope", c_d), ("dt", c_d)]
+    _fields_ = [("prec", c_d), ("manning", c_d), ("edge_slope", c_d), ("dt", c_d), ("update", c_int)]
 
 
 class PffLakeStatus(ctypes.Structure):
@@ -38,6 +38,17 @@ class PffLakeStatus(ctypes.Structure):
         ("n_iter", c_int),
         ("ndep", c_i64 * 64),
         ("carve_iters", c_int * 64),
+    ]
+
+
+class PffGfStatus(ctypes.Structure):
+    _fields_ = [
+        ("iterations", c_i64),
+        ("residual", c_d),
+        ("first_residual", c_d),
+        ("converged", c_int),
+        ("checks", c_int),
+        ("nan_cells", c_i64),
     ]
 
 
@@ -52,6 +63,9 @@ SIGNATURES = {
     "pff_ctx_destroy": [c_p],
     "pff_ctx_set_stream": [c_p, c_p],
     "pff_ctx_synchronize": [c_p],
+    "pff_ctx_profile": [c_p, c_int],
+    "pff_ctx_profile_report": [c_p, ctypes.c_char_p, ctypes.POINTER(c_i64), ctypes.POINTER(c_d),
+                               ctypes.POINTER(c_d)],
     "pff_compute_receivers": [c_p, G, c_p, c_p, c_p],
     "pff_rcv2donor": [c_p, G, c_p, c_p, c_p],
     "pff_rake_compress_accum": [c_p, G, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_int],
@@ -71,6 +85,9 @@ SIGNATURES = {
     "pff_topofill": [c_p, G, c_p, c_p, c_p, c_p, c_p, c_d],
     "pff_diffuse_Q_constant_prec": [c_p, G, H, c_p, c_p, c_p],
     "pff_graphflood_core_cte_mannings": [c_p, G, H, c_p, c_p, c_p, c_p, c_p],
+    "pff_graphflood_run": [c_p, G, H, c_p, c_p, c_p, c_p, c_int, c_int, c_d, ctypes.POINTER(PffGfStatus)],
+    "pff_graphflood_to_steady": [c_p, G, H, c_p, c_p, c_p, c_p, c_d, c_int, c_i64, c_int, c_d,
+                                 ctypes.POINTER(PffGfStatus)],
     "pff_pack_full_float_index": [c_p, c_i64, c_p, c_p, c_p],
     "pff_unpack_full_float_index": [c_p, c_i64, c_p, c_p, c_p],
     "pff_fill_edges": [c_p, G, c_p],
@@ -151,8 +168,8 @@ def grid(nx: int, ny: int, dx: float, mode: int, boundaries=None) -> PffGrid:
     return PffGrid(int(nx), int(ny), float(dx), int(mode), ptr(boundaries) if boundaries is not None else c_p(0))
 
 
-def hydro(prec: float, manning: float, edge_slope: float, dt: float) -> PffHydro:
-    return PffHydro(float(prec), float(manning), float(edge_slope), float(dt))
+def hydro(prec: float, manning: float, edge_slope: float, dt: float, update: int = 0) -> PffHydro:
+    return PffHydro(float(prec), float(manning), float(edge_slope), float(dt), int(update))
 
 
 def declared_symbols(header: str | None = None) -> list[str]:

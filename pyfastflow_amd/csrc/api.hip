@@ -6,6 +6,7 @@
 #include <cstring>
 #include <string>
 #include <utility>
+#include <algorithm>
 
 #include "pff_internal.hpp"
 
@@ -63,6 +64,32 @@ int scratch(pff_ctx* ctx, int slot, size_t bytes, void** out, bool zero_new) {
   return 0;
 }
 
+unsigned long long* prof_begin(pff_ctx* ctx, const char* name, double bytes_per_cell) {
+  if (!ctx->prof_on || (int)ctx->prof.size() >= pff_ctx::kProfSlots) return nullptr;
+  pff_ctx::ProfRec r;
+  r.name = name;
+  r.bytes_per_cell = bytes_per_cell;
+  r.slot = (int)ctx->prof.size();
+  hipEventCreate(&r.start);
+  hipEventCreate(&r.stop);
+  hipEventRecord(r.start, ctx->stream);
+  ctx->prof.push_back(r);
+  return ctx->prof_cells + r.slot;
+}
+
+void prof_end(pff_ctx* ctx) {
+  if (!ctx->prof_on || ctx->prof.empty()) return;
+  hipEventRecord(ctx->prof.back().stop, ctx->stream);
+}
+
+static void prof_clear(pff_ctx* ctx) {
+  for (auto& r : ctx->prof) {
+    hipEventDestroy(r.start);
+    hipEventDestroy(r.stop);
+  }
+  ctx->prof.clear();
+}
+
 static int begin(pff_ctx* ctx) {
   if (!ctx) return fail(PFF_EINVAL, "context is NULL");
   return check_hip(hipSetDevice(ctx->device), "hipSetDevice");
@@ -113,6 +140,8 @@ int pff_ctx_destroy(pff_ctx* ctx) {
   if (!ctx) return 0;
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
+  prof_clear(ctx);
+  if (ctx->prof_cells) hipFree(ctx->prof_cells);
   for (int s = 0; s < pff_ctx::kSlots; ++s)
     if (ctx->buf[s]) hipFree(ctx->buf[s]);
   if (ctx->dev_words) hipFree(ctx->dev_words);
@@ -124,6 +153,44 @@ int pff_ctx_destroy(pff_ctx* ctx) {
 int pff_ctx_set_stream(pff_ctx* ctx, void* hip_stream) {
   if (!ctx) return fail(PFF_EINVAL, "context is NULL");
   ctx->stream = (hipStream_t)hip_stream;
+  return 0;
+}
+
+int pff_ctx_profile(pff_ctx* ctx, int32_t enable) {
+  PFF_TRY(begin(ctx));
+  PFF_TRY(check_hip(hipStreamSynchronize(ctx->stream), "sync"));
+  prof_clear(ctx);
+  if (enable && !ctx->prof_cells) {
+    if (hipMalloc((void**)&ctx->prof_cells, pff_ctx::kProfSlots * 8) != hipSuccess)
+      return fail(PFF_ENOMEM, "profile counters");
+  }
+  if (enable) PFF_TRY(check_hip(hipMemsetAsync(ctx->prof_cells, 0, pff_ctx::kProfSlots * 8, ctx->stream), "memset"));
+  ctx->prof_on = enable != 0;
+  return 0;
+}
+
+int pff_ctx_profile_report(pff_ctx* ctx, const char* kernel, int64_t* launches, double* total_ms,
+                           double* total_bytes) {
+  PFF_TRY(begin(ctx));
+  if (!kernel || !launches || !total_ms || !total_bytes) return fail(PFF_EINVAL, "profile_report: NULL argument");
+  PFF_TRY(check_hip(hipStreamSynchronize(ctx->stream), "sync"));
+  std::vector<unsigned long long> cells(ctx->prof.size());
+  if (!cells.empty())
+    PFF_TRY(check_hip(hipMemcpy(cells.data(), ctx->prof_cells, cells.size() * 8, hipMemcpyDeviceToHost), "d2h"));
+  int64_t n = 0;
+  double ms = 0, bytes = 0;
+  for (size_t k = 0; k < ctx->prof.size(); ++k) {
+    const auto& r = ctx->prof[k];
+    if (r.name != kernel) continue;
+    float t = 0;
+    PFF_TRY(check_hip(hipEventElapsedTime(&t, r.start, r.stop), "hipEventElapsedTime"));
+    ++n;
+    ms += t;
+    bytes += r.bytes_per_cell * (double)cells[k];
+  }
+  *launches = n;
+  *total_ms = ms;
+  *total_bytes = bytes;
   return 0;
 }
 
@@ -432,6 +499,143 @@ int pff_graphflood_core_cte_mannings(pff_ctx* ctx, const pff_grid* g, const pff_
   if (!p || !h || !zh || !dh || !rec || !Q) return fail(PFF_EINVAL, "graphflood_core_cte_mannings: NULL argument");
   return launch_manning(ctx, d, p, h, zh, dh, rec, Q);
 }
+
+}  // extern "C"
+
+// ---- GraphFlood drivers ---------------------------------------------------------
+namespace {
+
+struct GfFields {
+  int32_t *rec_, *rec__, *bid, *snode;
+  float *zp, *Qt, *dh, *hprev;
+  int64_t *outlet, *saddle;
+  uint8_t *isb, *tag, *tag_;
+};
+
+// one device block holding every field the iteration needs besides zh, h, Q, rec
+int gf_fields(pff_ctx* ctx, const GridD& d, GfFields& f) {
+  const size_t N = (size_t)d.N;
+  const size_t bytes = N * (4 * 4 + 4 * 4 + 8 * 2) + N * 3 + 4096;
+  void* base;
+  PFF_TRY(scratch(ctx, S_GF0, bytes, &base, true));
+  char* p = (char*)base;
+  auto take = [&](size_t n) {
+    char* q = p;
+    p += (n + 255) / 256 * 256;
+    return q;
+  };
+  f.outlet = (int64_t*)take(N * 8);
+  f.saddle = (int64_t*)take(N * 8);
+  f.rec_ = (int32_t*)take(N * 4);
+  f.rec__ = (int32_t*)take(N * 4);
+  f.bid = (int32_t*)take(N * 4);
+  f.snode = (int32_t*)take(N * 4);
+  f.zp = (float*)take(N * 4);
+  f.Qt = (float*)take(N * 4);
+  f.dh = (float*)take(N * 4);
+  f.hprev = (float*)take(N * 4);
+  f.isb = (uint8_t*)take(N);
+  f.tag = (uint8_t*)take(N);
+  f.tag_ = (uint8_t*)take(N);
+  return 0;
+}
+
+// one Flooder iteration (gf_fields.py:126-141)
+int gf_iteration(pff_ctx* ctx, const pff_grid* g, const GridD& d, const pff_hydro* p, float* zh, float* h, float* Q,
+                 int32_t* rec, int32_t n_diffuse, double eps, GfFields& f) {
+  PFF_TRY(launch_receivers(ctx, d, zh, rec, f.zp));  // the gradient output is scratch here
+  PFF_TRY(pff_reroute_flow(ctx, g, f.bid, rec, f.rec_, f.rec__, zh, f.zp, f.isb, f.outlet, f.saddle, f.snode, f.tag,
+                           f.tag_, 1, nullptr, nullptr));
+  PFF_TRY(pff_fill_z_add_delta(ctx, g, zh, h, f.zp, rec, f.rec_, f.rec__, eps));
+  float* a = Q;
+  float* b = f.Qt;
+  for (int k = 0; k < n_diffuse; ++k) {
+    PFF_TRY(launch_diffuse(ctx, d, p, zh, a, b));
+    std::swap(a, b);
+  }
+  if (a != Q) PFF_TRY(check_hip(hipMemcpyAsync(Q, a, (size_t)d.N * 4, hipMemcpyDeviceToDevice, ctx->stream), "copy"));
+  return launch_manning(ctx, d, p, h, zh, f.dh, rec, Q);
+}
+
+int gf_check(pff_ctx* ctx, const GridD& d, const float* h, const float* hprev, double* res, int64_t* nans) {
+  uint32_t* mx = (uint32_t*)(ctx->dev_words + W_GFMAX);
+  unsigned long long* nn = (unsigned long long*)(ctx->dev_words + W_GFNAN);
+  PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_GFMAX, 0, 16, ctx->stream), "memset"));
+  PFF_TRY(launch_maxdiff(ctx, d.N, h, hprev, mx, nn));
+  PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_GFMAX, ctx->dev_words + W_GFMAX, 16, hipMemcpyDeviceToHost,
+                                   ctx->stream), "d2h"));
+  PFF_TRY(check_hip(hipStreamSynchronize(ctx->stream), "sync"));
+  uint32_t bits = (uint32_t)(ctx->host_words[W_GFMAX] & 0xFFFFFFFFu);
+  float m;
+  std::memcpy(&m, &bits, 4);
+  *res = (double)m;
+  *nans = ctx->host_words[W_GFNAN];
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pff_graphflood_run(pff_ctx* ctx, const pff_grid* g, const pff_hydro* p, float* zh, float* h, float* Q,
+                       int32_t* rec, int32_t n_iter, int32_t n_diffuse, double fill_eps, pff_gf_status* st) {
+  PFF_TRY(begin(ctx));
+  GridD d;
+  PFF_TRY(check_grid(g, &d));
+  if (!p || !zh || !h || !Q || !rec) return fail(PFF_EINVAL, "graphflood_run: NULL argument");
+  if (n_iter < 0 || n_diffuse < 0) return fail(PFF_EINVAL, "graphflood_run: negative count");
+  GfFields f;
+  PFF_TRY(gf_fields(ctx, d, f));
+  for (int it = 0; it < n_iter; ++it) PFF_TRY(gf_iteration(ctx, g, d, p, zh, h, Q, rec, n_diffuse, fill_eps, f));
+  if (st) {
+    std::memset(st, 0, sizeof(*st));
+    st->iterations = n_iter;
+  }
+  return 0;
+}
+
+int pff_graphflood_to_steady(pff_ctx* ctx, const pff_grid* g, const pff_hydro* p, float* zh, float* h, float* Q,
+                             int32_t* rec, double tol, int32_t check_every, int64_t max_iter, int32_t n_diffuse,
+                             double fill_eps, pff_gf_status* st) {
+  PFF_TRY(begin(ctx));
+  GridD d;
+  PFF_TRY(check_grid(g, &d));
+  if (!p || !zh || !h || !Q || !rec) return fail(PFF_EINVAL, "graphflood_to_steady: NULL argument");
+  if (check_every < 1 || max_iter < 0 || n_diffuse < 0) return fail(PFF_EINVAL, "graphflood_to_steady: bad counts");
+  GfFields f;
+  PFF_TRY(gf_fields(ctx, d, f));
+  pff_gf_status s;
+  std::memset(&s, 0, sizeof(s));
+  s.residual = -1.0;
+  s.first_residual = -1.0;
+  const size_t n4 = (size_t)d.N * 4;
+  PFF_TRY(check_hip(hipMemcpyAsync(f.hprev, h, n4, hipMemcpyDeviceToDevice, ctx->stream), "copy"));
+  while (s.iterations < max_iter) {
+    const int64_t n = std::min<int64_t>(check_every, max_iter - s.iterations);
+    for (int64_t k = 0; k < n; ++k) PFF_TRY(gf_iteration(ctx, g, d, p, zh, h, Q, rec, n_diffuse, fill_eps, f));
+    s.iterations += n;
+    if (n < check_every) break;  // a partial block cannot be judged
+    double res = 0;
+    int64_t nans = 0;
+    PFF_TRY(gf_check(ctx, d, h, f.hprev, &res, &nans));
+    s.checks++;
+    s.residual = res;
+    s.nan_cells = nans;
+    if (s.first_residual < 0) s.first_residual = res;
+    if (nans > 0) break;
+    if (res < tol) {
+      s.converged = 1;
+      break;
+    }
+    PFF_TRY(check_hip(hipMemcpyAsync(f.hprev, h, n4, hipMemcpyDeviceToDevice, ctx->stream), "copy"));
+  }
+  if (st) *st = s;
+  return 0;
+}
+
+}  // extern "C"
+
+extern "C" {
 
 // ---- packed keys and utilities ------------------------------------------------
 int pff_pack_full_float_index(pff_ctx* ctx, int64_t n, int64_t* arr, const float* ft, const int32_t* it) {

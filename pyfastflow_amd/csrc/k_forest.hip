@@ -344,6 +344,7 @@ struct TileArgs {
   float* Q;
   uint32_t* cnt;
   uint8_t* tag;
+  unsigned long long* prof_cells;  // profiling: cells processed by this launch
 };
 
 template <int W>
@@ -372,6 +373,7 @@ __global__ __launch_bounds__(kTB) void k_tile(TileArgs a) {
   const GridD& g = a.g;
   const int x0 = blockIdx.x * TS, y0 = blockIdx.y * TS;
   const int tw = min(TS, g.nx - x0), th = min(TS, g.ny - y0);
+  if (a.prof_cells && threadIdx.x == 0) atomicAdd(a.prof_cells, (unsigned long long)(tw * th));
   int32_t cell[CPT];
   uint32_t rootmask = 0;
 #pragma unroll
@@ -625,8 +627,12 @@ int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, 
   PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_NEXIT, 0, 8, s), "memset"));
   PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_CYCLE, 0, 8, s), "memset"));
   const dim3 tg = tile_grid(g);
+  // algorithmic bytes per cell: rcv in; E, S, inflow out (+ bid, snode for saddle weights)
+  a.prof_cells = prof_begin(ctx, "k_tile", wmode == W_UNIT ? 16.0 : 24.0);
   if (wmode == W_UNIT) hipLaunchKernelGGL((k_tile<0, 1, W_UNIT>), tg, dim3(kTB), 0, s, a);
   else hipLaunchKernelGGL((k_tile<0, 1, W_SADDLE>), tg, dim3(kTB), 0, s, a);
+  prof_end(ctx);
+  a.prof_cells = nullptr;
   // reduced graph over the exits: P2 (roots = -1), S holds the in-tile sums
   Lists L;
   L.l[0] = b.L0;
@@ -659,8 +665,12 @@ int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, 
     if (cyc) return cycle ? 0 : fail(PFF_ENOTCONV, "tiled_upstream: receivers are not a forest");
   }
   hipLaunchKernelGGL(k_exit_inflow, dim3(list_grid()), dim3(kBT), 0, s, b.exits, a.n_exits, rcv, b.S, b.inflow);
+  // rcv, inflow in; Q (or count) out / rcv, bid, snode, inflow in; tag out
+  a.prof_cells = prof_begin(ctx, "k_tile", wmode == W_UNIT ? 12.0 : 17.0);
   if (wmode == W_UNIT) hipLaunchKernelGGL((k_tile<0, 3, W_UNIT>), tg, dim3(kTB), 0, s, a);
   else hipLaunchKernelGGL((k_tile<0, 3, W_SADDLE>), tg, dim3(kTB), 0, s, a);
+  prof_end(ctx);
+  a.prof_cells = nullptr;
   return check_hip(hipGetLastError(), "tiled_upstream");
 }
 
@@ -682,7 +692,10 @@ int tiled_labels(pff_ctx* ctx, const GridD& g, const int32_t* rec, const uint32_
   PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_CYCLE, 0, 8, s), "memset"));
   PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_NPIT, 0, 8, s), "memset"));
   const dim3 tg = tile_grid(g);
+  a.prof_cells = prof_begin(ctx, "k_tile", 8.0);  // rcv in, E out
   PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL((k_tile<M, 1, W_ROOTS>), tg, dim3(kTB), 0, s, a));
+  prof_end(ctx);
+  a.prof_cells = nullptr;
   Lists L;
   L.l[0] = b.L0;
   L.l[1] = b.L1;

@@ -166,6 +166,7 @@ int launch_diffuse(pff_ctx* ctx, const GridD& g, const pff_hydro* p, const float
 // ---------------------------------------------------------------------------
 struct ManningC {
   float dx, edgesw, manning, dt, dx2, e53;
+  int update;
 };
 
 __global__ void k_manning1(int32_t N, ManningC c, float* __restrict__ h, const float* __restrict__ zh,
@@ -181,10 +182,16 @@ __global__ void k_manning1(int32_t N, ManningC c, float* __restrict__ h, const f
   const float hi = h[i];
   const float Qo = c.dx * powf(hi, c.e53) / c.manning * sqrtf(tS);
   float tdh = (Q[i] - Qo) / c.dx2 * c.dt;
-  const float hn = hi + tdh;
-  if (hn < 0.0f) tdh += hn;
-  h[i] = hn;
-  dh[i] = tdh;
+  if (c.update == 0) {
+    const float hn = hi + tdh;
+    if (hn < 0.0f) tdh += hn;
+    h[i] = hn;
+    dh[i] = tdh;
+  } else {
+    // single update, depth clamped at zero: the second kernel adds dh to h and zh
+    if (hi + tdh < 0.0f) tdh = -hi;
+    dh[i] = tdh;
+  }
 }
 
 __global__ void k_manning2(int32_t N, float* __restrict__ h, float* __restrict__ zh, const float* __restrict__ dh) {
@@ -204,10 +211,43 @@ int launch_manning(pff_ctx* ctx, const GridD& g, const pff_hydro* p, float* h, f
   c.dt = (float)p->dt;
   c.dx2 = (float)((double)g.dx * (double)g.dx);
   c.e53 = (float)(5.0 / 3.0);
+  c.update = p->update;
   const dim3 gr(grid_for(g.N)), bl(kBlock);
   hipLaunchKernelGGL(k_manning1, gr, bl, 0, ctx->stream, g.N, c, h, zh, dh, rec, Q);
   hipLaunchKernelGGL(k_manning2, gr, bl, 0, ctx->stream, g.N, h, zh, dh);
   return check_hip(hipGetLastError(), "manning");
+}
+
+// max |a - b| over N cells (both finite) and the number of NaN cells of a;
+// out[0] holds the max as float bits (non-negative floats order as integers),
+// out[1] the NaN count
+__global__ __launch_bounds__(256) void k_maxdiff(int32_t N, const float* __restrict__ a, const float* __restrict__ b,
+                                                 uint32_t* out_max, unsigned long long* out_nan) {
+  float m = 0.0f;
+  unsigned nan = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+    const float x = a[i];
+    if (x != x) {
+      ++nan;
+      continue;
+    }
+    const float d = fabsf(x - b[i]);
+    m = d > m ? d : m;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    m = fmaxf(m, __shfl_down(m, o));
+    nan += __shfl_down(nan, o);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(out_max, __float_as_uint(m));
+    if (nan) atomicAdd(out_nan, (unsigned long long)nan);
+  }
+}
+
+int launch_maxdiff(pff_ctx* ctx, int32_t N, const float* a, const float* b, uint32_t* out_max,
+                   unsigned long long* out_nan) {
+  hipLaunchKernelGGL(k_maxdiff, dim3(1024), dim3(256), 0, ctx->stream, N, a, b, out_max, out_nan);
+  return check_hip(hipGetLastError(), "maxdiff");
 }
 
 }  // namespace pff

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/pff.h"
 #include "pff_common.hpp"
@@ -22,6 +23,18 @@ struct pff_ctx {
   int64_t* dev_words = nullptr;
   // reduced-graph rounds the last tiled labelling needed (a start estimate)
   int last_label_rounds = 8;
+  // opt-in per-kernel timing (pff_ctx_profile): HIP events around each
+  // instrumented launch on the launch stream, plus the cells it processed
+  struct ProfRec {
+    std::string name;
+    double bytes_per_cell;
+    hipEvent_t start, stop;
+    int slot;
+  };
+  bool prof_on = false;
+  std::vector<ProfRec> prof;
+  unsigned long long* prof_cells = nullptr;  // device, kProfSlots counters
+  static constexpr int kProfSlots = 4096;
 };
 
 namespace pff {
@@ -30,6 +43,10 @@ void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);
 int check_hip(hipError_t e, const char* what);
 int check_grid(const pff_grid* g, GridD* out);
+// profiling: returns the device counter for the launch's processed cells
+// (null when profiling is off); call prof_end after the launch
+unsigned long long* prof_begin(pff_ctx* ctx, const char* name, double bytes_per_cell);
+void prof_end(pff_ctx* ctx);
 // device scratch slot of at least `bytes`
 int scratch(pff_ctx* ctx, int slot, size_t bytes, void** out, bool zero_new = false);
 
@@ -43,11 +60,12 @@ enum Slot : int {
   S_MISC,
   S_TAG,                                   // carve tag words, kept all-zero between calls
   S_B1,                                    // byte N-array (tiled carve tags)
-  S_TE, S_TP2, S_TBID, S_TEXIT, S_TS, S_TIN, S_TPIT, S_TACT  // tiled contraction buffers
+  S_TE, S_TP2, S_TBID, S_TEXIT, S_TS, S_TIN, S_TPIT, S_TACT,  // tiled contraction buffers
+  S_GF0  // GraphFlood driver fields (one block, see graphflood_fields)
 };
 
 // device word indices (dev_words / host_words)
-enum Word : int { W_COUNT0 = 0, W_COUNT1, W_COUNT2, W_FLAG, W_FLAG2, W_CYCLE, W_NEXIT, W_NDEP, W_NPIT, W_LIVE, W_EMPTY, W_NWORDS = 16 };
+enum Word : int { W_COUNT0 = 0, W_COUNT1, W_COUNT2, W_FLAG, W_FLAG2, W_CYCLE, W_NEXIT, W_NDEP, W_NPIT, W_LIVE, W_EMPTY, W_GFMAX, W_GFNAN, W_NWORDS = 16 };
 
 constexpr int kBlock = 256;
 inline unsigned grid_for(int64_t n, int block = kBlock, int64_t cap = 1 << 20) {
@@ -114,6 +132,8 @@ int launch_manning(pff_ctx* ctx, const GridD& g, const pff_hydro* p, float* h, f
                    const int32_t* rec, const float* Q);
 
 int launch_edge_maps(pff_ctx* ctx, const GridD& g, uint8_t* edges, uint8_t* out);
+int launch_maxdiff(pff_ctx* ctx, int32_t N, const float* a, const float* b, uint32_t* out_max,
+                   unsigned long long* out_nan);
 int launch_pack(pff_ctx* ctx, int64_t n, int64_t* arr, const float* ft, const int32_t* it);
 int launch_unpack(pff_ctx* ctx, int64_t n, const int64_t* arr, float* ft, int32_t* it);
 int launch_swap(pff_ctx* ctx, int64_t n, uint32_t* a, uint32_t* b);

@@ -3,6 +3,8 @@ GraphFlood iteration on a FlowRouter."""
 
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 import torch
 
@@ -18,9 +20,16 @@ class Flooder:
 
     precipitation_rates in m/s, manning's n, edge_slope for cells draining out
     of the domain; dt is constants.DT_HYDRO (5e-3 s) unless given.
+
+    update selects the depth update of graphflood_core_cte_mannings:
+    "reference" (default) adds dh to h twice and to zh once, exactly as
+    gf_hydrodynamics.py:121-129; "single" adds dh once to both and clamps h
+    at 0.  With "reference", filled lakes lose depth every iteration until h
+    is negative and h^(5/3) is NaN, so runs to steady state use "single".
     """
 
-    def __init__(self, router, precipitation_rates=10e-3 / 3600, manning=0.033, edge_slope=1e-2, dt=None):
+    def __init__(self, router, precipitation_rates=10e-3 / 3600, manning=0.033, edge_slope=1e-2, dt=None,
+                 update="reference"):
         self.nx, self.ny, self.dx = router.nx, router.ny, router.dx
         self.rshp = router.rshp
         self.router = router
@@ -31,6 +40,9 @@ class Flooder:
         self.manning = float(manning)
         self.edge_slope = float(edge_slope)
         self.dt = float(cte.DT_HYDRO if dt is None else dt)
+        if update not in ("reference", "single"):
+            raise ValueError("update must be 'reference' or 'single'")
+        self.update = update
         # globals for the module-level kernels (gf_fields.py:106-108)
         cte.PREC = self.precipitation_rates
         cte.MANNING = self.manning
@@ -38,10 +50,12 @@ class Flooder:
         self.iterations = 0
 
     def _hydro(self):
-        return _lib.hydro(self.precipitation_rates, self.manning, self.edge_slope, self.dt)
+        return _lib.hydro(self.precipitation_rates, self.manning, self.edge_slope, self.dt,
+                          0 if self.update == "reference" else 1)
 
-    def step(self, n_diffuse=5):
-        """One GraphFlood iteration (gf_fields.py:126-141)."""
+    def step_modules(self, n_diffuse=5):
+        """One GraphFlood iteration through the module-level calls, as the
+        reference sequences them (gf_fields.py:126-141)."""
         r = self.router
         ctx, g, hy = r._ctx(), r._grid(), self._hydro()
         r.compute_receivers()
@@ -54,27 +68,31 @@ class Flooder:
                   _lib.ptr(r.receivers), _lib.ptr(r.Q))
         self.iterations += 1
 
-    def run_graphflood(self, N=10):
-        """N GraphFlood iterations (gf_fields.py:111-141)."""
-        for _ in range(int(N)):
-            self.step()
+    def step(self, n_diffuse=5):
+        """One GraphFlood iteration (gf_fields.py:126-141)."""
+        self.run_graphflood(1, n_diffuse=n_diffuse)
 
-    def run_to_steady(self, tol=1e-6, check_every=100, max_iter=1_000_000):
+    def run_graphflood(self, N=10, n_diffuse=5):
+        """N GraphFlood iterations (gf_fields.py:111-141), looped inside the
+        HIP library."""
+        r = self.router
+        st = _lib.PffGfStatus()
+        _lib.call("pff_graphflood_run", r._ctx(), r._grid(), self._hydro(), _lib.ptr(r.z), _lib.ptr(self.h),
+                  _lib.ptr(r.Q), _lib.ptr(r.receivers), int(N), int(n_diffuse), 1e-3, ctypes.byref(st))
+        self.iterations += int(N)
+        return st
+
+    def run_to_steady(self, tol=1e-6, check_every=100, max_iter=1_000_000, n_diffuse=5):
         """Iterate until max|h(t) - h(t - check_every)| < tol, checked every
-        `check_every` iterations.  Returns (iterations, last max|dh|,
-        converged)."""
-        prev = self.h.data.clone()
-        it = 0
-        res = float("inf")
-        while it < max_iter:
-            for _ in range(check_every):
-                self.step()
-            it += check_every
-            res = float((self.h.data - prev).abs().max().item())
-            if res < tol:
-                return it, res, True
-            prev.copy_(self.h.data)
-        return it, res, False
+        `check_every` iterations on the GPU, or max_iter iterations.  Returns
+        the PffGfStatus (iterations, residual, converged, NaN cells)."""
+        r = self.router
+        st = _lib.PffGfStatus()
+        _lib.call("pff_graphflood_to_steady", r._ctx(), r._grid(), self._hydro(), _lib.ptr(r.z), _lib.ptr(self.h),
+                  _lib.ptr(r.Q), _lib.ptr(r.receivers), float(tol), int(check_every), int(max_iter), int(n_diffuse),
+                  1e-3, ctypes.byref(st))
+        self.iterations += int(st.iterations)
+        return st
 
     def set_h(self, val):
         """Set h (gf_fields.py:143-150); zh is not touched, as in the reference."""

@@ -77,7 +77,8 @@ def test_diffuse_vs_f64(pf, ora, mode):
     _close(r.Q.to_numpy(), ref, "diffuse")
 
 
-def test_manning_vs_f64(pf, ora):
+@pytest.mark.parametrize("update", [0, 1])
+def test_manning_vs_f64(pf, ora, update):
     ny, nx = 64, 64
     r, z = _setup(pf, ny, nx, "normal", 34, 3)
     r.compute_receivers()
@@ -90,15 +91,26 @@ def test_manning_vs_f64(pf, ora):
     h.from_numpy(h0)
     r.Q.from_numpy(Q0)
     pf.constants.MANNING, pf.constants.EDGESW, pf.constants.DT_HYDRO = 0.033, 1e-2, 5e-3
-    pf.graphflood.graphflood_core_cte_mannings(h, r.z, dh, r.receivers, r.Q)
+    h0[:100] = 1e-4  # shallow cells that a strong outflow empties
+    h.from_numpy(h0)
+    if update == 0:
+        pf.graphflood.graphflood_core_cte_mannings(h, r.z, dh, r.receivers, r.Q)
+    else:
+        from pyfastflow_amd import _lib
+
+        hy_g = _lib.hydro(1e-5, 0.033, 1e-2, 5e-3, 1)
+        _lib.call("pff_graphflood_core_cte_mannings", _lib.ctx_for(), r._grid(), hy_g, _lib.ptr(h), _lib.ptr(r.z),
+                  _lib.ptr(dh), _lib.ptr(r.receivers), _lib.ptr(r.Q))
+        assert (h.to_numpy() >= 0).all()
     g = ora.Grid(nx, ny, 10.0, 0)
-    hy = ora.hydro(manning=0.033, edge_slope=1e-2, dt=5e-3)
+    hy = ora.hydro(manning=0.033, edge_slope=1e-2, dt=5e-3, update=update)
     h_ref, zh_ref = ora.manning(g, hy, h0, z, rec, Q0, f64=True)
     _close(h.to_numpy(), h_ref, "h")
     _close(r.z.to_numpy(), zh_ref, "zh")
 
 
-def test_graphflood_iterations_vs_oracle(pf, ora):
+@pytest.mark.parametrize("update", ["reference", "single"])
+def test_graphflood_iterations_vs_oracle(pf, ora, update):
     """Five Flooder iterations from a cold start against the f32 oracle run
     from the same state each iteration: receivers and the filled surface are
     bit-exact, h and Q within tolerance (the diffusion sums in a different
@@ -111,9 +123,9 @@ def test_graphflood_iterations_vs_oracle(pf, ora):
     differently by construction."""
     ny, nx = 64, 80
     r, z = _setup(pf, ny, nx, "normal", 35, 4)
-    fl = pf.graphflood.Flooder(r, precipitation_rates=1e-5, manning=0.033, edge_slope=1e-2)
+    fl = pf.graphflood.Flooder(r, precipitation_rates=1e-5, manning=0.033, edge_slope=1e-2, update=update)
     g = ora.Grid(nx, ny, 10.0, 0)
-    hy = ora.hydro(prec=1e-5, manning=0.033, edge_slope=1e-2, dt=5e-3)
+    hy = ora.hydro(prec=1e-5, manning=0.033, edge_slope=1e-2, dt=5e-3, update=0 if update == "reference" else 1)
     zh = z.ravel().copy()
     h = np.zeros(nx * ny, np.float32)
     Q = np.zeros(nx * ny, np.float32)
