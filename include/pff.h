@@ -179,6 +179,45 @@ int pff_graphflood_to_steady(pff_ctx* ctx, const pff_grid* g, const pff_hydro* p
                              int32_t* rec, double tol, int32_t check_every, int64_t max_iter, int32_t n_diffuse,
                              double fill_eps, pff_gf_status* st);
 
+/* ---- ragged batch of independent grids ---------------------------------- */
+/* One grid of a batch: 'normal' boundaries, its own nx, ny; device arrays. */
+typedef struct {
+  int32_t nx, ny;
+  const float* z;   /* in, nx*ny elevations */
+  int32_t* rcv;     /* out, nx*ny receivers (tile-local indices) */
+  float* Q;         /* out, nx*ny discharge */
+} pff_tile;
+
+/* Per-call result of pff_batch_route (host struct). */
+typedef struct {
+  int64_t cells;          /* sum of the tiles' cells */
+  int64_t canvas_cells;   /* cells of the packed canvas, padding included */
+  int32_t canvas_nx, canvas_ny;
+  int64_t ndep0;          /* initial depressions, all tiles */
+  int32_t n_loop;         /* lakeflow trips run over the canvas */
+  int32_t per_tile;       /* 1 when the batch was run tile by tile (see below) */
+  int32_t converged;      /* 1 when every accumulation converged */
+  int64_t capped_tiles;   /* tiles whose lakeflow stopped at their trip cap with depressions left */
+} pff_batch_status;
+
+/* No reference counterpart: the reference routes one grid per process
+   (environment.py:28-29).  For each tile, the result of FlowRouter(nx, ny,
+   dx, 'normal') -> compute_receivers() -> reroute_flow(carve) ->
+   accumulate_constant_Q(value, area) (receivers.py:15-58,
+   lakeflow.py:466-544, flowfields.py:254-292), for all tiles in one launch
+   sequence: the tiles are packed into one canvas separated by padding cells,
+   routed together, and unpacked.  lakeflow: 0 no reroute_flow, 1 carve
+   (the default of flowfields.py:228), 2 jump (whose result the reference
+   leaves in receivers_, so rcv keeps the unrerouted receivers, Q8).
+   Results equal per-tile runs bit for bit: each tile keeps its own lakeflow
+   trip cap (a tile that reaches it stops there, as its own loop would) and
+   the cell 42 that the "no outlet" sentinel names.  When the receivers stop
+   being a forest, or lakeflow is 2, the batch runs tile by tile instead
+   (st->per_tile = 1).  All tiles share dx.  Syncs (once per lakeflow trip,
+   and at the end).  st (host, nullable). */
+int pff_batch_route(pff_ctx* ctx, int32_t ntiles, const pff_tile* tiles, float dx, int32_t lakeflow, double value,
+                    int32_t area, pff_batch_status* st);
+
 /* ---- packed keys ------------------------------------------------------- */
 /* f32_i32_struct.py:123-135  pack_full_float_index(arr, ft, it) */
 int pff_pack_full_float_index(pff_ctx* ctx, int64_t n, int64_t* arr, const float* ft, const int32_t* it);

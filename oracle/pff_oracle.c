@@ -456,6 +456,8 @@ void ora_basin_identification(const ora_grid* g, const int32_t* rec, int32_t npa
 }
 
 /* saddlesort: lakeflow.py:105-281 (five passes). */
+static int64_t g_sentinel_hits = 0;
+
 void ora_saddlesort(const ora_grid* g, const int32_t* bid, const float* z, uint8_t* is_border, float* zp,
                     int64_t* basin_saddle, int32_t* saddlenode, int64_t* outlet) {
   const int64_t N = NN(g);
@@ -554,11 +556,20 @@ void ora_saddlesort(const ora_grid* g, const int32_t* bid, const float* z, uint8
     ora_unpack_key(outlet[b1], &tf, &r2);
     const int32_t b2 = (r2 >= 0 && r2 < N) ? bid[r2] : -1;
     if (b2 == b && b1 < b) {
+      if (outlet[b1] == INVALID_KEY) g_sentinel_hits++;
       outlet[b] = INVALID_KEY;
       basin_saddle[b] = INVALID_KEY;
       saddlenode[b] = -1;
     }
   }
+}
+
+/* number of pass-5 removals decided by reading an already removed outlet
+   (the sentinel, which decodes to node 42) since the last reset */
+int64_t ora_sentinel_hits(int32_t reset) {
+  const int64_t h = g_sentinel_hits;
+  if (reset) g_sentinel_hits = 0;
+  return h;
 }
 
 /* reroute_jump: lakeflow.py:284-304 */

@@ -75,6 +75,8 @@ void ora_basin_identification(const ora_grid* g, const int32_t* rec, int32_t npa
 void ora_saddlesort(const ora_grid* g, const int32_t* bid, const float* z, uint8_t* is_border,
                     float* z_prime, int64_t* basin_saddle, int32_t* saddlenode, int64_t* outlet);
 void ora_reroute_jump(const ora_grid* g, int32_t* rec, const int64_t* outlet);
+/* pass-5 removals that read the sentinel (node 42) since the last reset */
+int64_t ora_sentinel_hits(int32_t reset);
 /* returns the number of tag iterations run */
 int  ora_reroute_carve(const ora_grid* g, int32_t* rec, const int32_t* rec_in, const int32_t* saddlenode,
                        const int64_t* outlet, uint8_t* tag_out);

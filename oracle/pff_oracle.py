@@ -64,6 +64,8 @@ def lib():
         _lib.ora_depression_counter.restype = c_i64
         _lib.ora_receivers_div_ambiguity.restype = c_i64
         _lib.ora_num_threads.restype = c_int
+        _lib.ora_sentinel_hits.restype = c_i64
+        _lib.ora_sentinel_hits.argtypes = [c_int]
         for name in ("ora_accumulate_constant_Q",):
             getattr(_lib, name).argtypes = [c_p, c_p, c_d, c_int, c_p]
         _lib.ora_fill_z_add_delta.argtypes = [c_p, c_p, c_p, c_p, c_d]
@@ -208,6 +210,13 @@ def basin_identification(g: Grid, rec, npass=None):
     bid = np.empty(g.N, np.int32)
     lib().ora_basin_identification(g.ref, _p(rec), c_int(npass), _p(bid))
     return bid
+
+
+def sentinel_hits(reset: bool = True) -> int:
+    """Pass-5 removals (lakeflow.py:255-281) decided by reading an outlet that
+    an earlier basin of the same pass had removed: the removed key is the
+    sentinel pack(1e8, 42) and decodes to node 42."""
+    return int(lib().ora_sentinel_hits(1 if reset else 0))
 
 
 def saddlesort(g: Grid, bid, z, is_border=None):

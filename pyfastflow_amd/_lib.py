@@ -52,6 +52,24 @@ class PffGfStatus(ctypes.Structure):
     ]
 
 
+class PffTile(ctypes.Structure):
+    _fields_ = [("nx", c_int), ("ny", c_int), ("z", c_p), ("rcv", c_p), ("Q", c_p)]
+
+
+class PffBatchStatus(ctypes.Structure):
+    _fields_ = [
+        ("cells", c_i64),
+        ("canvas_cells", c_i64),
+        ("canvas_nx", c_int),
+        ("canvas_ny", c_int),
+        ("ndep0", c_i64),
+        ("n_loop", c_int),
+        ("per_tile", c_int),
+        ("converged", c_int),
+        ("capped_tiles", c_i64),
+    ]
+
+
 G = ctypes.POINTER(PffGrid)
 H = ctypes.POINTER(PffHydro)
 
@@ -81,6 +99,8 @@ SIGNATURES = {
     "pff_reroute_carve": [c_p, G, c_p, c_p, c_p, c_p, c_p, c_p, c_p],
     "pff_reroute_flow": [c_p, G, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_int, c_p,
                          ctypes.POINTER(PffLakeStatus)],
+    "pff_batch_route": [c_p, c_int, ctypes.POINTER(PffTile), c_f, c_int, c_d, c_int,
+                        ctypes.POINTER(PffBatchStatus)],
     "pff_fill_z_add_delta": [c_p, G, c_p, c_p, c_p, c_p, c_p, c_p, c_d],
     "pff_topofill": [c_p, G, c_p, c_p, c_p, c_p, c_p, c_d],
     "pff_diffuse_Q_constant_prec": [c_p, G, H, c_p, c_p, c_p],

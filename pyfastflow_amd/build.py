@@ -19,7 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpff_hip.so")
 OBJDIR = os.path.join(HERE, "_build")
 
-SOURCES = ["api.hip", "k_flow.hip", "k_forest.hip", "k_lake.hip", "k_gf.hip"]
+SOURCES = ["api.hip", "k_flow.hip", "k_forest.hip", "k_lake.hip", "k_gf.hip", "batch.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -51,7 +51,7 @@ def _compile(src: str) -> str:
 
 def build_library(verbose: bool = False) -> str:
     os.makedirs(OBJDIR, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=min(4, len(SOURCES))) as ex:
+    with ThreadPoolExecutor(max_workers=min(6, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
     cmd = [_hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]

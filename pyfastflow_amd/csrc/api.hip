@@ -46,14 +46,14 @@ int scratch(pff_ctx* ctx, int slot, size_t bytes, void** out, bool zero_new) {
   if (ctx->cap[slot] < bytes) {
     if (ctx->buf[slot]) {
       // the old buffer may still be read by queued kernels
-      hipStreamSynchronize(ctx->stream);
-      hipFree(ctx->buf[slot]);
+      (void)hipStreamSynchronize(ctx->stream);
+      (void)hipFree(ctx->buf[slot]);
       ctx->buf[slot] = nullptr;
       ctx->cap[slot] = 0;
     }
     void* p = nullptr;
     if (hipMalloc(&p, bytes) != hipSuccess || !p) {
-      hipGetLastError();
+      (void)hipGetLastError();
       return fail(PFF_ENOMEM, "device scratch allocation of " + std::to_string(bytes) + " bytes failed");
     }
     ctx->buf[slot] = p;
@@ -70,22 +70,22 @@ unsigned long long* prof_begin(pff_ctx* ctx, const char* name, double bytes_per_
   r.name = name;
   r.bytes_per_cell = bytes_per_cell;
   r.slot = (int)ctx->prof.size();
-  hipEventCreate(&r.start);
-  hipEventCreate(&r.stop);
-  hipEventRecord(r.start, ctx->stream);
+  (void)hipEventCreate(&r.start);
+  (void)hipEventCreate(&r.stop);
+  (void)hipEventRecord(r.start, ctx->stream);
   ctx->prof.push_back(r);
   return ctx->prof_cells + r.slot;
 }
 
 void prof_end(pff_ctx* ctx) {
   if (!ctx->prof_on || ctx->prof.empty()) return;
-  hipEventRecord(ctx->prof.back().stop, ctx->stream);
+  (void)hipEventRecord(ctx->prof.back().stop, ctx->stream);
 }
 
 static void prof_clear(pff_ctx* ctx) {
   for (auto& r : ctx->prof) {
-    hipEventDestroy(r.start);
-    hipEventDestroy(r.stop);
+    (void)hipEventDestroy(r.start);
+    (void)hipEventDestroy(r.stop);
   }
   ctx->prof.clear();
 }
@@ -104,6 +104,138 @@ static int ceil_log2(int64_t n) {
   int r = 0;
   while ((1ll << r) < n) ++r;
   return r;
+}
+
+int accumulate_core(pff_ctx* ctx, const GridD& d, const int32_t* rcv, float q0, float* Q, int32_t* converged) {
+  int32_t cycle = 0;
+  PFF_TRY(tiled_upstream(ctx, d, rcv, 1, nullptr, nullptr, q0, Q, nullptr, nullptr, nullptr, &cycle));
+  if (cycle) {
+    // not a forest: exact compacted pointer jumping, reported as unconverged
+    void* cnt;
+    PFF_TRY(scratch(ctx, S_I5, (size_t)d.N * 4, &cnt));
+    int32_t cyc2 = 0;
+    PFF_TRY(upstream_count(ctx, d, rcv, (uint32_t*)cnt, &cyc2));
+    PFF_TRY(launch_scale_counts(ctx, d, (const uint32_t*)cnt, q0, Q));
+  }
+  if (converged) *converged = cycle ? 0 : 1;
+  return 0;
+}
+
+// lakeflow.py:466-544 reroute_flow, the trip loop.  rec_ holds the working
+// receivers (rec_ <- rec, :491); with carve, rec ends as the carved receivers.
+int reroute_core(pff_ctx* ctx, const GridD& d, int32_t* bid, int32_t* rec, int32_t* rec_, int32_t* rec__,
+                 const float* z, float* z_prime, uint8_t* is_border, int64_t* outlet, int64_t* basin_saddle,
+                 int32_t* basin_saddlenode, int32_t carve, int32_t* bid_first, pff_lake_status* st, BatchState* bs,
+                 bool* fallback) {
+  hipStream_t s = ctx->stream;
+  const size_t n4 = (size_t)d.N * 4;
+  if (st) std::memset(st, 0, sizeof(*st));
+  if (fallback) *fallback = false;
+  // a canvas runs the tiled path only; anything else is rerun tile by tile
+  if (bs && !carve) {
+    *fallback = true;
+    return 0;
+  }
+  // rec_ <- rec (lakeflow.py:491)
+  PFF_TRY(check_hip(hipMemcpyAsync(rec_, rec, n4, hipMemcpyDeviceToDevice, s), "copy"));
+  const int npass = npass_for(d.N);
+  // The tiled contraction is used while the receivers form a forest; if a
+  // cycle appears (possible only through rerouting, e.g. a basin-graph cycle
+  // longer than 2) every later step runs the exact synchronous semantics.
+  bool exact = !carve;  // jump mode can leave non-forest receivers: keep exact semantics
+  int32_t *levels = nullptr, *fin = nullptr;
+  auto ensure_levels = [&]() -> int {
+    if (levels) return 0;
+    void *lv, *fv;
+    PFF_TRY(scratch(ctx, S_LEVELS, (size_t)(npass + 1) * n4, &lv));
+    PFF_TRY(scratch(ctx, S_I4, n4, &fv));
+    levels = (int32_t*)lv;
+    fin = (int32_t*)fv;
+    return 0;
+  };
+  // tile activity flags: trip t labels, sorts and carves only the tiles that
+  // held a closed basin after trip t-1 (all tiles in the first trip)
+  uint32_t *flag_a = nullptr, *flag_b = nullptr;
+  PFF_TRY(tile_flags(ctx, d, &flag_a, &flag_b));
+  const uint32_t* tact = nullptr;
+  int label_rounds = 0;
+  int nouter = 1;
+  for (int it = 0; it < nouter; ++it) {
+    int64_t ndb = 0;
+    if (!exact) {
+      int32_t cyc = 0;
+      PFF_TRY(tiled_labels(ctx, d, rec_, tact, &ndb, &cyc, &label_rounds));
+      if (cyc) exact = true;
+    }
+    if (exact && bs) {
+      *fallback = true;
+      return 0;
+    }
+    if (exact) PFF_TRY(depression_count(ctx, d, rec_, &ndb));
+    bool frozen_only = false;
+    if (bs) PFF_TRY(batch_after_labels(ctx, bs, d, it, ndb, &frozen_only));
+    if (it == 0) {
+      // lakeflow.py:493 (rec_ == rec here)
+      if (st) st->ndep0 = ndb;
+      if (ndb == 0) return 0;  // :496-497, bid untouched
+      nouter = ceil_log2(ndb) + 1;  // :500
+    }
+    if (st) {
+      if (st->n_loop < 64) st->ndep[st->n_loop] = ndb;  // :501
+      st->n_loop++;
+    }
+    // basin labels on rec_ (:507-511)
+    if (!exact) {
+      PFF_TRY(tiled_labels_write(ctx, d, bid, tact, flag_a));
+      tact = flag_a;
+      std::swap(flag_a, flag_b);
+    } else if (carve) {
+      PFF_TRY(ensure_levels());
+      PFF_TRY(labels_with_levels(ctx, d, rec_, bid, rec__, levels, fin));
+    } else {
+      PFF_TRY(basin_labels(ctx, d, rec_, bid, rec__));
+    }
+    if (it == 0 && bid_first)
+      PFF_TRY(check_hip(hipMemcpyAsync(bid_first, bid, n4, hipMemcpyDeviceToDevice, s), "copy"));
+    if (ndb == 0 || frozen_only) break;  // :512-514 (per tile on a canvas)
+    if (bs) PFF_TRY(batch_freeze(ctx, bs, d, bid));
+    // saddles and outlets (:520)
+    if (!exact) {
+      int32_t* pits = nullptr;
+      int64_t* npits = nullptr;
+      PFF_TRY(tiled_pits(ctx, d, &pits, &npits));
+      if (bs) PFF_TRY(batch_mark42(ctx, bs, d, bid, 1));
+      PFF_TRY(saddlesort_active(ctx, d, bid, is_border, z_prime, basin_saddle, basin_saddlenode, outlet, z, tact, pits,
+                                npits));
+      if (bs) PFF_TRY(batch_mark42(ctx, bs, d, bid, 0));
+      if (st) st->n_iter++;
+      // carve (:525-526): rec equals rec_ here; rec is edited, then copied to rec_
+      int32_t cyc = 0;
+      PFF_TRY(carve_tiled(ctx, d, rec, rec_, bid, basin_saddlenode, outlet, rec_, tact, pits, npits, label_rounds,
+                          &cyc));
+      if (!cyc) continue;
+      // not reachable (the labels proved rec_ a forest); fall back regardless
+      if (bs) {
+        *fallback = true;
+        return 0;
+      }
+      exact = true;
+      PFF_TRY(ensure_levels());
+      PFF_TRY(labels_with_levels(ctx, d, rec_, bid, rec__, levels, fin));
+      PFF_TRY(launch_saddlesort(ctx, d, bid, is_border, z_prime, basin_saddle, basin_saddlenode, outlet, z));
+    } else {
+      PFF_TRY(launch_saddlesort(ctx, d, bid, is_border, z_prime, basin_saddle, basin_saddlenode, outlet, z));
+      if (st) st->n_iter++;
+    }
+    if (carve) {
+      int32_t iters = 0;
+      PFF_TRY(carve_from_levels(ctx, d, rec, rec_, levels, fin, rec__, basin_saddlenode, outlet, rec_, nullptr, npass,
+                                &iters));
+    } else {
+      PFF_TRY(launch_reroute_jump(ctx, d, rec_, outlet));  // :529
+    }
+  }
+  return launch_basin_id_init(ctx, d, bid);  // :537
 }
 
 }  // namespace pff
@@ -127,25 +259,25 @@ int pff_ctx_create(pff_ctx** out, int32_t device, void* hip_stream) {
     return fail(PFF_ENOMEM, "pinned host words");
   }
   if (hipMalloc((void**)&c->dev_words, W_NWORDS * sizeof(int64_t)) != hipSuccess) {
-    hipHostFree(c->host_words);
+    (void)hipHostFree(c->host_words);
     delete c;
     return fail(PFF_ENOMEM, "device words");
   }
-  hipMemset(c->dev_words, 0, W_NWORDS * sizeof(int64_t));
+  (void)hipMemset(c->dev_words, 0, W_NWORDS * sizeof(int64_t));
   *out = c;
   return 0;
 }
 
 int pff_ctx_destroy(pff_ctx* ctx) {
   if (!ctx) return 0;
-  hipSetDevice(ctx->device);
-  hipStreamSynchronize(ctx->stream);
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
   prof_clear(ctx);
-  if (ctx->prof_cells) hipFree(ctx->prof_cells);
+  if (ctx->prof_cells) (void)hipFree(ctx->prof_cells);
   for (int s = 0; s < pff_ctx::kSlots; ++s)
-    if (ctx->buf[s]) hipFree(ctx->buf[s]);
-  if (ctx->dev_words) hipFree(ctx->dev_words);
-  if (ctx->host_words) hipHostFree(ctx->host_words);
+    if (ctx->buf[s]) (void)hipFree(ctx->buf[s]);
+  if (ctx->dev_words) (void)hipFree(ctx->dev_words);
+  if (ctx->host_words) (void)hipHostFree(ctx->host_words);
   delete ctx;
   return 0;
 }
@@ -260,18 +392,7 @@ int pff_accumulate_constant_Q(pff_ctx* ctx, const pff_grid* g, const int32_t* rc
   // Q.fill(value * (DX*DX if area else 1)) evaluates in double, stored as f32
   // (flowfields.py:279)
   const float q0 = (float)(value * (area ? (double)g->dx * (double)g->dx : 1.0));
-  int32_t cycle = 0;
-  PFF_TRY(tiled_upstream(ctx, d, rcv, 1, nullptr, nullptr, q0, Q, nullptr, nullptr, nullptr, &cycle));
-  if (cycle) {
-    // not a forest: exact compacted pointer jumping, reported as unconverged
-    void* cnt;
-    PFF_TRY(scratch(ctx, S_I5, (size_t)d.N * 4, &cnt));
-    int32_t cyc2 = 0;
-    PFF_TRY(upstream_count(ctx, d, rcv, (uint32_t*)cnt, &cyc2));
-    PFF_TRY(launch_scale_counts(ctx, d, (const uint32_t*)cnt, q0, Q));
-  }
-  if (converged) *converged = cycle ? 0 : 1;
-  return 0;
+  return accumulate_core(ctx, d, rcv, q0, Q, converged);
 }
 
 // ---- lakeflow -------------------------------------------------------------
@@ -355,96 +476,26 @@ int pff_reroute_flow(pff_ctx* ctx, const pff_grid* g, int32_t* bid, int32_t* rec
     return fail(PFF_EINVAL, "reroute_flow: NULL array");
   (void)tag;
   (void)tag_;
-  hipStream_t s = ctx->stream;
-  const size_t n4 = (size_t)d.N * 4;
-  if (st) std::memset(st, 0, sizeof(*st));
-  // rec_ <- rec (lakeflow.py:491)
-  PFF_TRY(check_hip(hipMemcpyAsync(rec_, rec, n4, hipMemcpyDeviceToDevice, s), "copy"));
-  const int npass = npass_for(d.N);
-  // The tiled contraction is used while the receivers form a forest; if a
-  // cycle appears (possible only through rerouting, e.g. a basin-graph cycle
-  // longer than 2) every later step runs the exact synchronous semantics.
-  bool exact = !carve;  // jump mode can leave non-forest receivers: keep exact semantics
-  int32_t *levels = nullptr, *fin = nullptr;
-  auto ensure_levels = [&]() -> int {
-    if (levels) return 0;
-    void *lv, *fv;
-    PFF_TRY(scratch(ctx, S_LEVELS, (size_t)(npass + 1) * n4, &lv));
-    PFF_TRY(scratch(ctx, S_I4, n4, &fv));
-    levels = (int32_t*)lv;
-    fin = (int32_t*)fv;
-    return 0;
-  };
-  // tile activity flags: trip t labels, sorts and carves only the tiles that
-  // held a closed basin after trip t-1 (all tiles in the first trip)
-  uint32_t *flag_a = nullptr, *flag_b = nullptr;
-  PFF_TRY(tile_flags(ctx, d, &flag_a, &flag_b));
-  const uint32_t* tact = nullptr;
-  int label_rounds = 0;
-  int nouter = 1;
-  for (int it = 0; it < nouter; ++it) {
-    int64_t ndb = 0;
-    if (!exact) {
-      int32_t cyc = 0;
-      PFF_TRY(tiled_labels(ctx, d, rec_, tact, &ndb, &cyc, &label_rounds));
-      if (cyc) exact = true;
-    }
-    if (exact) PFF_TRY(depression_count(ctx, d, rec_, &ndb));
-    if (it == 0) {
-      // lakeflow.py:493 (rec_ == rec here)
-      if (st) st->ndep0 = ndb;
-      if (ndb == 0) return 0;  // :496-497, bid untouched
-      nouter = ceil_log2(ndb) + 1;  // :500
-    }
-    if (st) {
-      if (st->n_loop < 64) st->ndep[st->n_loop] = ndb;  // :501
-      st->n_loop++;
-    }
-    // basin labels on rec_ (:507-511)
-    if (!exact) {
-      PFF_TRY(tiled_labels_write(ctx, d, bid, tact, flag_a));
-      tact = flag_a;
-      std::swap(flag_a, flag_b);
-    } else if (carve) {
-      PFF_TRY(ensure_levels());
-      PFF_TRY(labels_with_levels(ctx, d, rec_, bid, rec__, levels, fin));
-    } else {
-      PFF_TRY(basin_labels(ctx, d, rec_, bid, rec__));
-    }
-    if (it == 0 && bid_first)
-      PFF_TRY(check_hip(hipMemcpyAsync(bid_first, bid, n4, hipMemcpyDeviceToDevice, s), "copy"));
-    if (ndb == 0) break;  // :512-514
-    // saddles and outlets (:520)
-    if (!exact) {
-      int32_t* pits = nullptr;
-      int64_t* npits = nullptr;
-      PFF_TRY(tiled_pits(ctx, d, &pits, &npits));
-      PFF_TRY(saddlesort_active(ctx, d, bid, is_border, z_prime, basin_saddle, basin_saddlenode, outlet, z, tact, pits,
-                                npits));
-      if (st) st->n_iter++;
-      // carve (:525-526): rec equals rec_ here; rec is edited, then copied to rec_
-      int32_t cyc = 0;
-      PFF_TRY(carve_tiled(ctx, d, rec, rec_, bid, basin_saddlenode, outlet, rec_, tact, pits, npits, label_rounds,
-                          &cyc));
-      if (!cyc) continue;
-      // not reachable (the labels proved rec_ a forest); fall back regardless
-      exact = true;
-      PFF_TRY(ensure_levels());
-      PFF_TRY(labels_with_levels(ctx, d, rec_, bid, rec__, levels, fin));
-      PFF_TRY(launch_saddlesort(ctx, d, bid, is_border, z_prime, basin_saddle, basin_saddlenode, outlet, z));
-    } else {
-      PFF_TRY(launch_saddlesort(ctx, d, bid, is_border, z_prime, basin_saddle, basin_saddlenode, outlet, z));
-      if (st) st->n_iter++;
-    }
-    if (carve) {
-      int32_t iters = 0;
-      PFF_TRY(carve_from_levels(ctx, d, rec, rec_, levels, fin, rec__, basin_saddlenode, outlet, rec_, nullptr, npass,
-                                &iters));
-    } else {
-      PFF_TRY(launch_reroute_jump(ctx, d, rec_, outlet));  // :529
-    }
+  return reroute_core(ctx, d, bid, rec, rec_, rec__, z, z_prime, is_border, outlet, basin_saddle, basin_saddlenode,
+                      carve, bid_first, st);
+}
+
+// ---- ragged batch -----------------------------------------------------------
+int pff_batch_route(pff_ctx* ctx, int32_t ntiles, const pff_tile* tiles, float dx, int32_t lakeflow, double value,
+                    int32_t area, pff_batch_status* st) {
+  PFF_TRY(begin(ctx));
+  if (ntiles < 0) return fail(PFF_EINVAL, "batch_route: ntiles < 0");
+  if (ntiles > 0 && !tiles) return fail(PFF_EINVAL, "batch_route: tiles is NULL");
+  if (!(dx > 0.0f)) return fail(PFF_EINVAL, "batch_route: dx must be positive");
+  if (lakeflow < 0 || lakeflow > 2) return fail(PFF_EINVAL, "batch_route: lakeflow must be 0, 1 or 2");
+  for (int32_t t = 0; t < ntiles; ++t) {
+    const pff_tile& T = tiles[t];
+    if (T.nx < 2 || T.ny < 2) return fail(PFF_EINVAL, "batch_route: tile " + std::to_string(t) + " needs nx, ny >= 2");
+    if ((int64_t)T.nx * T.ny >= (int64_t)0x7FFFFFFF)
+      return fail(PFF_EINVAL, "batch_route: tile " + std::to_string(t) + " has 2^31 cells or more");
+    if (!T.z || !T.rcv || !T.Q) return fail(PFF_EINVAL, "batch_route: tile " + std::to_string(t) + " has a NULL array");
   }
-  return launch_basin_id_init(ctx, d, bid);  // :537
+  return batch_route(ctx, ntiles, tiles, dx, lakeflow, value, area, st);
 }
 
 // ---- fill -------------------------------------------------------------------

@@ -195,8 +195,25 @@ __global__ void k_outlet(GridD g, const int32_t* __restrict__ bid, const float* 
 // basins may invalidate in the same loop; in serial ascending order that
 // read only matters for the basin holding cell 42 (the sentinel's index):
 // it is also removed when it drains into a lower-id basin that was removed.
+//
+// holds42(b): b is the basin of the cell the sentinel decodes to, i.e. cell
+// 42 of the grid, or of b's own tile on a tile canvas (f42 marks those).
+__device__ __forceinline__ bool holds42(int32_t N, const int32_t* __restrict__ bid, const uint8_t* __restrict__ f42,
+                                        int b) {
+  return f42 ? f42[b] != 0 : (N > 42 && bid[42] == b);
+}
+
+// basin of the node an outlet key names; an invalid key decodes to cell 42
+// (of the grid, or of the asking basin b's tile on a canvas)
+__device__ __forceinline__ int outlet_basin(int32_t N, const int32_t* __restrict__ bid,
+                                            const uint8_t* __restrict__ f42, int64_t key, int b) {
+  if (f42 && key == INVALID_KEY) return f42[b] ? b : -1;
+  const int r = key_index(key);
+  return (r >= 0 && r < N) ? bid[r] : -1;
+}
+
 __global__ void k_cycle_flag(int32_t N, const int32_t* __restrict__ bid, const int64_t* __restrict__ outlet,
-                             uint8_t* __restrict__ inv) {
+                             uint8_t* __restrict__ inv, const uint8_t* __restrict__ f42) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= N) return;
   uint8_t f = 0;
@@ -204,8 +221,7 @@ __global__ void k_cycle_flag(int32_t N, const int32_t* __restrict__ bid, const i
     const int r1 = key_index(outlet[b]);
     const int b1 = (r1 >= 0 && r1 < N) ? bid[r1] : -1;
     if (b1 > 0 && b1 < N) {
-      const int r2 = key_index(outlet[b1]);
-      const int b2 = (r2 >= 0 && r2 < N) ? bid[r2] : -1;
+      const int b2 = outlet_basin(N, bid, f42, outlet[b1], b);
       if (b2 == b && b1 < b) f = 1;
     }
   }
@@ -213,11 +229,11 @@ __global__ void k_cycle_flag(int32_t N, const int32_t* __restrict__ bid, const i
 }
 
 __global__ void k_cycle_apply(int32_t N, const int32_t* __restrict__ bid, int64_t* outlet, int64_t* saddle,
-                              int32_t* snode, const uint8_t* __restrict__ inv) {
+                              int32_t* snode, const uint8_t* __restrict__ inv, const uint8_t* __restrict__ f42) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b < 1 || b >= N) return;
   bool kill = inv[b] != 0;
-  if (!kill && N > 42 && bid[42] == b && outlet[b] != INVALID_KEY) {
+  if (!kill && holds42(N, bid, f42, b) && outlet[b] != INVALID_KEY) {
     const int r1 = key_index(outlet[b]);
     const int b1 = (r1 >= 0 && r1 < N) ? bid[r1] : -1;
     if (b1 > 0 && b1 < b && inv[b1]) kill = true;
@@ -240,8 +256,8 @@ int launch_saddlesort(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t*
   PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_saddle_min<M>, gr, bl, 0, s, g, bid, isb, zp, saddle));
   PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_saddle_node<M>, gr, bl, 0, s, g, bid, isb, zp, saddle, snode));
   PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_outlet<M>, gr, bl, 0, s, g, bid, z, saddle, snode, outlet));
-  hipLaunchKernelGGL(k_cycle_flag, gr, bl, 0, s, g.N, bid, outlet, (uint8_t*)inv);
-  hipLaunchKernelGGL(k_cycle_apply, gr, bl, 0, s, g.N, bid, outlet, saddle, snode, (const uint8_t*)inv);
+  hipLaunchKernelGGL(k_cycle_flag, gr, bl, 0, s, g.N, bid, outlet, (uint8_t*)inv, g.F42);
+  hipLaunchKernelGGL(k_cycle_apply, gr, bl, 0, s, g.N, bid, outlet, saddle, snode, (const uint8_t*)inv, g.F42);
   return check_hip(hipGetLastError(), "saddlesort");
 }
 
@@ -555,7 +571,8 @@ __global__ __launch_bounds__(kBT) void k_outlet_list(GridD g, const int32_t* __r
 // pass 5 (:255-281) over the pits, in the two phases of k_cycle_flag/apply
 __global__ __launch_bounds__(kBT) void k_cycle_flag_list(int32_t N, const int32_t* __restrict__ pits, const int64_t* np,
                                                          const int32_t* __restrict__ bid,
-                                                         const int64_t* __restrict__ outlet, uint8_t* __restrict__ inv) {
+                                                         const int64_t* __restrict__ outlet, uint8_t* __restrict__ inv,
+                                                         const uint8_t* __restrict__ f42) {
   const int64_t n = *np;
   for (int64_t w = (int64_t)blockIdx.x * kBT + threadIdx.x; w < n; w += (int64_t)gridDim.x * kBT) {
     const int b = pits[w];
@@ -565,8 +582,7 @@ __global__ __launch_bounds__(kBT) void k_cycle_flag_list(int32_t N, const int32_
       const int r1 = key_index(outlet[b]);
       const int b1 = (r1 >= 0 && r1 < N) ? bid[r1] : -1;
       if (b1 > 0 && b1 < N) {
-        const int r2 = key_index(outlet[b1]);
-        const int b2 = (r2 >= 0 && r2 < N) ? bid[r2] : -1;
+        const int b2 = outlet_basin(N, bid, f42, outlet[b1], b);
         if (b2 == b && b1 < b) f = 1;
       }
     }
@@ -577,13 +593,14 @@ __global__ __launch_bounds__(kBT) void k_cycle_flag_list(int32_t N, const int32_
 __global__ __launch_bounds__(kBT) void k_cycle_apply_list(int32_t N, const int32_t* __restrict__ pits,
                                                           const int64_t* np, const int32_t* __restrict__ bid,
                                                           int64_t* outlet, int64_t* saddle, int32_t* snode,
-                                                          const uint8_t* __restrict__ inv) {
+                                                          const uint8_t* __restrict__ inv,
+                                                          const uint8_t* __restrict__ f42) {
   const int64_t n = *np;
   for (int64_t w = (int64_t)blockIdx.x * kBT + threadIdx.x; w < n; w += (int64_t)gridDim.x * kBT) {
     const int b = pits[w];
     if (b >= N) continue;
     bool kill = inv[b] != 0;
-    if (!kill && N > 42 && bid[42] == b && outlet[b] != INVALID_KEY) {
+    if (!kill && holds42(N, bid, f42, b) && outlet[b] != INVALID_KEY) {
       const int r1 = key_index(outlet[b]);
       const int b1 = (r1 >= 0 && r1 < N) ? bid[r1] : -1;
       if (b1 > 0 && b1 < b && inv[b1]) kill = true;
@@ -611,9 +628,10 @@ int saddlesort_active(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t*
                                                (const uint8_t*)isb, (const float*)zp, (const int64_t*)saddle, snode));
   PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_outlet_list<M>, lg, bl, 0, s, g, pits, npits, bid, z,
                                                (const int64_t*)saddle, (const int32_t*)snode, outlet));
-  hipLaunchKernelGGL(k_cycle_flag_list, lg, bl, 0, s, g.N, pits, npits, bid, (const int64_t*)outlet, (uint8_t*)inv);
+  hipLaunchKernelGGL(k_cycle_flag_list, lg, bl, 0, s, g.N, pits, npits, bid, (const int64_t*)outlet, (uint8_t*)inv,
+                     g.F42);
   hipLaunchKernelGGL(k_cycle_apply_list, lg, bl, 0, s, g.N, pits, npits, bid, outlet, saddle, snode,
-                     (const uint8_t*)inv);
+                     (const uint8_t*)inv, g.F42);
   return check_hip(hipGetLastError(), "saddlesort_active");
 }
 

@@ -13,8 +13,11 @@ namespace pff {
 struct GridD {
   int32_t nx, ny, N;
   float dx;
-  int32_t mode;         // 0 normal, 1 periodic_EW, 2 periodic_NS, 3 custom
-  const uint8_t* B;     // custom boundary codes (device), mode 3 only
+  int32_t mode;         // 0 normal, 1 periodic_EW, 2 periodic_NS, 3 custom, 4 tile canvas
+  const uint8_t* B;     // per-cell codes (device): custom codes in mode 3, canvas codes in mode 4
+  // mode 4 only: f42[b] != 0 marks the basin b that holds cell 42 of its own
+  // tile (the index the "no outlet" sentinel decodes to); nullptr elsewhere
+  const uint8_t* F42 = nullptr;
 };
 
 // Edge class of which_edge (neighbourer_flat.py:75-110) from (row, col); valid
@@ -72,6 +75,21 @@ __device__ __forceinline__ int neighbour(const GridD& g, int i, int row, int col
       case 2: return col < nx - 1 ? i + 1 : -1;
       default: return row < ny - 1 ? i + nx : i - nx * (ny - 1);
     }
+  } else if constexpr (MODE == 4) {
+    // tile canvas: several 'normal' grids packed into one array, separated by
+    // padding.  Codes: 0 padding, 1 tile interior, 3 tile edge.  A link is
+    // valid when both ends are tile cells, which inside a tile is exactly the
+    // normal-mode rule since every tile is framed by padding or the canvas
+    // border.
+    if (g.B[i] == 0) return -1;
+    int node;
+    switch (k) {
+      case 0: node = row > 0 ? i - nx : -1; break;
+      case 1: node = col > 0 ? i - 1 : -1; break;
+      case 2: node = col < nx - 1 ? i + 1 : -1; break;
+      default: node = row < ny - 1 ? i + nx : -1; break;
+    }
+    return (node < 0 || g.B[node] == 0) ? -1 : node;
   } else {  // custom per-node codes 0/1/3/7/9 (:360-462)
     const uint8_t tb = g.B[i];
     if (tb == 0) return -1;
@@ -108,6 +126,7 @@ __device__ __forceinline__ bool can_leave(const GridD& g, int i, int row, int co
   if constexpr (MODE == 0) return row == 0 || row == g.ny - 1 || col == 0 || col == g.nx - 1;
   else if constexpr (MODE == 1) return row == 0 || row == g.ny - 1;
   else if constexpr (MODE == 2) return col == 0 || col == g.nx - 1;
+  else if constexpr (MODE == 4) return g.B[i] != 1;  // tile edges, and padding (never a pit)
   else return g.B[i] == 3;
 }
 
@@ -159,6 +178,7 @@ __device__ __forceinline__ void rc_of(const GridD& g, int i, int& row, int& col)
       case 0: { constexpr int M = 0; __VA_ARGS__; } break; \
       case 1: { constexpr int M = 1; __VA_ARGS__; } break; \
       case 2: { constexpr int M = 2; __VA_ARGS__; } break; \
+      case 4: { constexpr int M = 4; __VA_ARGS__; } break; \
       default: { constexpr int M = 3; __VA_ARGS__; } break; \
     }                                           \
   } while (0)

@@ -61,7 +61,9 @@ enum Slot : int {
   S_TAG,                                   // carve tag words, kept all-zero between calls
   S_B1,                                    // byte N-array (tiled carve tags)
   S_TE, S_TP2, S_TBID, S_TEXIT, S_TS, S_TIN, S_TPIT, S_TACT,  // tiled contraction buffers
-  S_GF0  // GraphFlood driver fields (one block, see graphflood_fields)
+  S_GF0,    // GraphFlood driver fields (one block, see graphflood_fields)
+  S_BATCH,  // tile-canvas arrays of pff_batch_route (one block)
+  S_BATCHT  // tile-canvas tables (per tile)
 };
 
 // device word indices (dev_words / host_words)
@@ -130,6 +132,31 @@ int launch_apply_fill(pff_ctx* ctx, const GridD& g, float* zh, float* h, const f
 int launch_diffuse(pff_ctx* ctx, const GridD& g, const pff_hydro* p, const float* zh, const float* Q, float* temp);
 int launch_manning(pff_ctx* ctx, const GridD& g, const pff_hydro* p, float* h, float* zh, float* dh,
                    const int32_t* rec, const float* Q);
+
+// ---- drivers shared by the entry points (api.hip) ----------------------------
+struct BatchState;
+// reroute_flow on an internal grid.  With a batch state (tile canvas), the
+// trip loop applies each tile's own outer cap; *fallback is set (and nothing
+// more is done) when the receivers stop being a forest, which the canvas
+// does not handle.
+int reroute_core(pff_ctx* ctx, const GridD& d, int32_t* bid, int32_t* rec, int32_t* rec_, int32_t* rec__,
+                 const float* z, float* z_prime, uint8_t* is_border, int64_t* outlet, int64_t* basin_saddle,
+                 int32_t* basin_saddlenode, int32_t carve, int32_t* bid_first, pff_lake_status* st,
+                 BatchState* bs = nullptr, bool* fallback = nullptr);
+int accumulate_core(pff_ctx* ctx, const GridD& d, const int32_t* rcv, float q0, float* Q, int32_t* converged);
+
+// ---- tile canvas (batch.hip) -------------------------------------------------
+// per-trip hook after the labels: counts each tile's depressions; on trip 0
+// sets the tiles' outer caps; a tile with depressions left that has used its
+// cap is frozen for the rest of the call; *stop when only frozen tiles are left
+int batch_after_labels(pff_ctx* ctx, BatchState* bs, const GridD& d, int trip, int64_t ndep, bool* stop);
+// bid <- 0 on the frozen tiles, so that no basin of theirs gets a saddle,
+// an outlet or a carve in this trip
+int batch_freeze(pff_ctx* ctx, BatchState* bs, const GridD& d, int32_t* bid);
+// f42[bid[cell 42 of each tile]] = value
+int batch_mark42(pff_ctx* ctx, BatchState* bs, const GridD& d, const int32_t* bid, uint8_t value);
+int batch_route(pff_ctx* ctx, int32_t ntiles, const pff_tile* tiles, float dx, int32_t lakeflow, double value,
+                int32_t area, pff_batch_status* st);
 
 int launch_edge_maps(pff_ctx* ctx, const GridD& g, uint8_t* edges, uint8_t* out);
 int launch_maxdiff(pff_ctx* ctx, int32_t N, const float* a, const float* b, uint32_t* out_max,

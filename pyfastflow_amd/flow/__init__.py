@@ -11,11 +11,12 @@ from .util_taichi import *  # noqa: F401,F403
 from .flowfields import *  # noqa: F401,F403
 from .fill_topo import *  # noqa: F401,F403
 from .environment import *  # noqa: F401,F403
+from .batch import *  # noqa: F401,F403
 from . import neighbourer_flat, receivers, downstream_propag, lakeflow, f32_i32_struct, util_taichi  # noqa: F401
-from . import flowfields, fill_topo, environment  # noqa: F401
+from . import flowfields, fill_topo, environment, batch  # noqa: F401
 from .. import constants  # noqa: F401
 
 __all__ = [
     "neighbourer_flat", "receivers", "downstream_propag", "lakeflow", "f32_i32_struct", "util_taichi",
-    "flowfields", "fill_topo", "environment",
+    "flowfields", "fill_topo", "environment", "batch",
 ]

@@ -7,7 +7,7 @@ import torch
 from .. import _lib
 from ._core import check, ctx_of, grid_from_constants, n_cells
 
-__all__ = ["rcv2donor", "rake_compress_accum", "fuse"]
+__all__ = ["rcv2donor", "rake_compress_accum", "fuse", "upstream_count"]
 
 
 def rcv2donor(rcv, dnr, ndnr):
@@ -38,3 +38,14 @@ def fuse(A, src, B, iteration: int):
     ts = check("src", src, n, torch.int32)
     tb = check("B", B, n, torch.float32)
     _lib.call("pff_fuse", ctx_of(ta), grid_from_constants(), _lib.ptr(ta), _lib.ptr(ts), _lib.ptr(tb), int(iteration))
+
+
+def upstream_count(rcv, count):
+    """count[i] = number of cells whose receiver chain passes through i,
+    itself included (uint32 stored in an int32 field).  This is the integer
+    core of FlowRouter.accumulate_constant_Q (flowfields.py:254-292): the
+    converged discharge is count * Q0."""
+    n = n_cells()
+    tr = check("rcv", rcv, n, torch.int32)
+    tc = check("count", count, n, torch.int32)
+    _lib.call("pff_upstream_count", ctx_of(tr), grid_from_constants(), _lib.ptr(tr), _lib.ptr(tc))

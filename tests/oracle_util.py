@@ -108,3 +108,24 @@ def spill_elevation(z, ny, nx, mode, start):
                 best[j] = nh
                 heapq.heappush(heap, (nh, j))
     return np.inf
+
+
+# Small DEMs on which the cycle pass of saddlesort (lakeflow.py:255-281)
+# removes an outlet by reading the sentinel pack(1e8, 42) that an earlier
+# removal left (so node 42's basin decides), and every cell drains out in the
+# end; found with oracle.sentinel_hits over seeds 0..3999 of sentinel_dem.
+SENTINEL_SEEDS = [68, 132, 149, 447]
+# DEMs of the same family on which reroute_flow stops at its outer cap
+# ceil(log2 Ndep)+1 (lakeflow.py:500) with a depression left: a closed basin
+# whose only contact with other basins is through edge cells, which the
+# saddle pass skips (lakeflow.py:132-157), never gets a saddle.  157 also
+# reads the sentinel.
+CAPPED_SEEDS = [113, 157, 382, 1533]
+
+
+def sentinel_dem(seed, dx=10.0):
+    from pyfastflow_amd import synth
+
+    rng = np.random.default_rng(seed)
+    ny, nx = int(rng.integers(6, 40)), int(rng.integers(6, 40))
+    return synth.dem(ny, nx, 2.0, 50.0, seed, n_pits=int(rng.integers(1, 12)), dx=dx)

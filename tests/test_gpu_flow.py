@@ -206,3 +206,23 @@ def test_edge_maps(pf, ora, mode):
     g = _ograd(ora, nx, ny, 1.0, mode)
     np.testing.assert_array_equal(e.cpu().numpy(), [ora.which_edge(g, i) for i in range(g.N)])
     np.testing.assert_array_equal(o.cpu().numpy(), [int(ora.can_leave(g, i)) for i in range(g.N)])
+
+
+@pytest.mark.parametrize("seed", [68, 132, 149, 447, 113, 157, 382, 1533])
+def test_reroute_sentinel_and_capped_dems(pf, ora, seed):
+    """DEMs on which the cycle pass reads the sentinel (node 42), or the trip
+    loop stops at its cap with a depression left: receivers bit-exact."""
+    from oracle_util import sentinel_dem
+
+    print(f"case reroute sentinel/capped seed {seed}", flush=True)
+    z = sentinel_dem(seed)
+    ny, nx = z.shape
+    r = _router(pf, nx, ny, 10.0, "normal", z)
+    r.compute_receivers()
+    r.reroute_flow(carve=True)
+    g = _ograd(ora, nx, ny, 10.0, 0)
+    rcv, _ = ora.compute_receivers(g, z)
+    lk = ora.reroute_flow(g, z, rcv, carve=True)
+    np.testing.assert_array_equal(r.receivers.to_numpy(), lk["rec"])
+    st = r.last_lake_status
+    assert [int(st.ndep[i]) for i in range(st.n_loop)] == lk["ndep"]

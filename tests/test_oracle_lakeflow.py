@@ -135,3 +135,56 @@ def test_jump_mode_leaves_receivers_unchanged(ora):
     np.testing.assert_array_equal(r["rec"], rcv)
     pit = 4 * nx + 4
     assert r["rec_"][pit] != pit
+
+
+def test_sentinel_rule_fires_on_known_dems(ora):
+    """The cycle pass reads outlets that the same pass removed earlier (serial
+    order, lakeflow.py:255-281); a removed outlet is the sentinel and decodes
+    to node 42.  These DEMs exercise that read, so GPU parity on them pins
+    the rule (and, in a batch, that node 42 is the tile's own cell 42)."""
+    from oracle_util import SENTINEL_SEEDS, sentinel_dem
+
+    for s in SENTINEL_SEEDS:
+        z = sentinel_dem(s)
+        ny, nx = z.shape
+        g = ora.Grid(nx, ny, 10.0, 0)
+        rcv, _ = ora.compute_receivers(g, z)
+        ora.sentinel_hits(True)
+        lk = ora.reroute_flow(g, z, rcv, carve=True)
+        assert ora.sentinel_hits(True) >= 1, s
+        # the rerouted receivers drain every cell out of the grid
+        roots = roots_by_walking(lk["rec"])
+        assert outlets(ny, nx, 0)[roots].all()
+
+
+def test_capped_dems_keep_a_depression(ora):
+    """A closed basin whose every edge cell drains into it, touching the only
+    open cells through edge cells, has no border cell (can_leave cells are
+    skipped, lakeflow.py:132-157) and so no saddle: the trip loop runs its
+    cap ceil(log2 Ndep0)+1 (:500) and ends with that depression."""
+    from oracle_util import CAPPED_SEEDS, sentinel_dem
+
+    for s in CAPPED_SEEDS:
+        z = sentinel_dem(s)
+        ny, nx = z.shape
+        g = ora.Grid(nx, ny, 10.0, 0)
+        rcv, _ = ora.compute_receivers(g, z)
+        lk = ora.reroute_flow(g, z, rcv, carve=True)
+        cap = int(np.ceil(np.log2(lk["ndep0"]))) + 1
+        assert lk["n_loop"] == cap, s
+        assert ora.depression_counter(g, lk["rec"]) >= 1
+        roots = roots_by_walking(lk["rec"])
+        left = np.unique(roots[~outlets(ny, nx, 0)[roots]])
+        assert left.size >= 1
+        # that basin has no interior cell next to a cell of another basin
+        bid = ora.basin_identification(g, lk["rec"], 64)
+        nb = neighbour_table(ny, nx, 0)
+        out = outlets(ny, nx, 0)
+        for p in left:
+            cells = np.flatnonzero(bid == p + 1)
+            for c in cells:
+                if out[c]:
+                    continue
+                for k in range(4):
+                    j = nb[k, c]
+                    assert j < 0 or bid[j] == p + 1
