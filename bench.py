@@ -38,6 +38,10 @@ def parse():
     p.add_argument("--size", type=int, default=4096)
     p.add_argument("--pits", type=int, default=2000)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and full-size parity check")
+    p.add_argument("--no-batch", action="store_true",
+                   help="skip the ragged-batch line (256 tiles, sides 128-1024, routed in one launch sequence)")
+    p.add_argument("--no-graphflood", action="store_true",
+                   help="skip the GraphFlood line (2048^2, iterations of the reference schedule)")
     p.add_argument("--pmc-calibrate", action="store_true",
                    help="after the steps, run a 1 GiB read + 1 GiB write elementwise pass 3 times "
                         "(scale for rocprofv3 FETCH/WRITE_SIZE)")
@@ -160,6 +164,57 @@ def main():
         log("[bench] calibration: 3 elementwise passes reading and writing 1 GiB each")
         del a, b
 
+    # ---- ragged batch: 256 'normal' tiles, sides log-uniform in [128, 1024],
+    #      routed together (receivers + reroute_flow + accumulate per tile)
+    batch = None
+    if not args.no_batch:
+        tiles = [t["z"] for t in synth.ragged_batch(256)]
+        zt = [torch.from_numpy(z).cuda() for z in tiles]
+        bcells = sum(z.size for z in tiles)
+        res = pf.flow.route_batch(zt, dx=10.0, value=1e-5)
+        torch.cuda.synchronize()
+        eb = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        reps = 3
+        eb[0].record(s)
+        for _ in range(reps):
+            res = pf.flow.route_batch(zt, dx=10.0, value=1e-5, out=res)
+        eb[1].record(s)
+        torch.cuda.synchronize()
+        bms = eb[0].elapsed_time(eb[1]) / reps
+        batch = {"workload": "256 tiles, sides round(128*8^u), u~U[0,1), seed 404, normal, dx=10 m",
+                 "cells": int(bcells), "ms": round(bms, 3), "Mcells_per_s": round(bcells / (bms * 1e-3) / 1e6, 1),
+                 "status": res.status}
+        log(f"[bench] batch {batch}")
+        del zt, res
+
+    # ---- GraphFlood: 2048^2 open-edge DEM, cold start, run_graphflood
+    #      iterations (receivers -> reroute_flow -> fill -> 5 diffusions ->
+    #      Manning) with the reference's depth update
+    gflood = None
+    if not args.no_graphflood:
+        cg = synth.graphflood_plain()
+        rg = pf.flow.FlowRouter(cg["nx"], cg["ny"], cg["dx"], boundary_mode="normal")
+        rg.set_z(cg["z"])
+        fl = pf.graphflood.Flooder(rg, precipitation_rates=cg["precipitation"], manning=cg["manning"],
+                                   edge_slope=1e-2)
+        fl.run_graphflood(5)
+        torch.cuda.synchronize()
+        n_it = 200
+        eg = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        eg[0].record(s)
+        fl.run_graphflood(n_it)
+        eg[1].record(s)
+        torch.cuda.synchronize()
+        gms = eg[0].elapsed_time(eg[1]) / n_it
+        h = fl.get_h()
+        gflood = {"workload": "2048x2048 normal dx=10 m, beta=2.2, relief 300 m, 500 pits, seed 303; P=1e-5 m/s, "
+                              "n=0.033, edge_slope=1e-2, dt=5e-3 s, fill eps=1e-3, 5 diffusions; reference update",
+                  "iterations_timed": n_it, "ms_per_iteration": round(gms, 4),
+                  "Mcells_per_s": round(cg["nx"] * cg["ny"] / (gms * 1e-3) / 1e6, 1),
+                  "h_mean_m": float(np.nanmean(h)), "h_nan_cells": int(np.isnan(h).sum())}
+        log(f"[bench] graphflood {gflood}")
+        del rg, fl
+
     # ---- CPU baseline (oracle, the reference algorithm restated in C) and
     #      full-size parity of the GPU result against it ----------------------
     cpu = None
@@ -211,7 +266,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"depressions {n}x{n} periodic_EW dx=10 m, {pits} Gaussian pits, spectral beta=2.0 "
                                f"relief 1000 m seed 202; receivers + reroute_flow(carve) + accumulate_constant_Q(1e-5)",
-                   "cells": N, "stages": stages, "lakeflow": lake, "parity": parity, "receivers_kernel": rcv_roof},
+                   "cells": N, "stages": stages, "lakeflow": lake, "parity": parity, "receivers_kernel": rcv_roof,
+                   "ragged_batch": batch, "graphflood": gflood},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

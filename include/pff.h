@@ -218,6 +218,41 @@ typedef struct {
 int pff_batch_route(pff_ctx* ctx, int32_t ntiles, const pff_tile* tiles, float dx, int32_t lakeflow, double value,
                     int32_t area, pff_batch_status* st);
 
+/* ---- GraphFlood warm-start surrogate ------------------------------------
+   No reference counterpart (the reference has no learned component).  A
+   small U-Net on BF16 activations (x[n][y][x][c], c contiguous, stored as
+   uint16_t bit patterns) with FP32 accumulation on the matrix cores.  3x3
+   convolutions: stride 1, zero padding 1, channels multiples of 32.  Master
+   weights are f32 w[cout][cin][3][3]; pff_conv_pack_weights makes the bf16
+   forward (wf[9][cout][cin]) and backward-data (wt[9][cin][cout], taps
+   mirrored) copies.  Gradients accumulate (+=) into dw, db. */
+int pff_conv_pack_weights(pff_ctx* ctx, int32_t cin, int32_t cout, const float* w, uint16_t* wf, uint16_t* wt);
+int pff_conv3x3_fwd(pff_ctx* ctx, int32_t n, int32_t h, int32_t w, int32_t cin, int32_t cout, const uint16_t* x,
+                    const uint16_t* wf, const float* bias, int32_t relu, uint16_t* y);
+int pff_conv3x3_bwd_data(pff_ctx* ctx, int32_t n, int32_t h, int32_t w, int32_t cin, int32_t cout,
+                         const uint16_t* dy, const uint16_t* wt, uint16_t* dx);
+int pff_conv3x3_bwd_weight(pff_ctx* ctx, int32_t n, int32_t h, int32_t w, int32_t cin, int32_t cout,
+                           const uint16_t* x, const uint16_t* dy, float* dw, float* db);
+/* dy[i] = 0 where y[i] <= 0 (ReLU backward, in place) */
+int pff_relu_bwd(pff_ctx* ctx, int64_t n, const uint16_t* y, uint16_t* dy);
+/* 2x2 average pooling of an (n, h, w, c) tensor (h, w even) and its backward */
+int pff_avgpool2(pff_ctx* ctx, int32_t n, int32_t h, int32_t w, int32_t c, const uint16_t* x, uint16_t* y);
+int pff_avgpool2_bwd(pff_ctx* ctx, int32_t n, int32_t h, int32_t w, int32_t c, const uint16_t* dy, uint16_t* dx);
+/* nearest 2x upsampling of an (n, h, w, c) tensor and its backward */
+int pff_upsample2(pff_ctx* ctx, int32_t n, int32_t h, int32_t w, int32_t c, const uint16_t* x, uint16_t* y);
+int pff_upsample2_bwd(pff_ctx* ctx, int32_t n, int32_t h, int32_t w, int32_t c, const uint16_t* dy, uint16_t* dx);
+/* channel concatenation y = [a, b] over npix pixels, and its split */
+int pff_concat(pff_ctx* ctx, int64_t npix, int32_t ca, int32_t cb, const uint16_t* a, const uint16_t* b,
+               uint16_t* y);
+int pff_concat_bwd(pff_ctx* ctx, int64_t npix, int32_t ca, int32_t cb, const uint16_t* dy, uint16_t* da,
+                   uint16_t* db);
+/* 1x1 head to one f32 channel, y = b + x.w (c <= 256), and its backward (dx nullable) */
+int pff_head1x1(pff_ctx* ctx, int64_t npix, int32_t c, const uint16_t* x, const float* w, const float* b, float* y);
+int pff_head1x1_bwd(pff_ctx* ctx, int64_t npix, int32_t c, const uint16_t* x, const float* w, const float* dy,
+                    uint16_t* dx, float* dw, float* db);
+/* nplanes f32 planes [nplanes][npix] -> bf16 pixels of c channels (zero padded) */
+int pff_pack_input(pff_ctx* ctx, int64_t npix, int32_t nplanes, int32_t c, const float* planes, uint16_t* x);
+
 /* ---- packed keys ------------------------------------------------------- */
 /* f32_i32_struct.py:123-135  pack_full_float_index(arr, ft, it) */
 int pff_pack_full_float_index(pff_ctx* ctx, int64_t n, int64_t* arr, const float* ft, const int32_t* it);

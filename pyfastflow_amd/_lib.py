@@ -101,6 +101,20 @@ SIGNATURES = {
                          ctypes.POINTER(PffLakeStatus)],
     "pff_batch_route": [c_p, c_int, ctypes.POINTER(PffTile), c_f, c_int, c_d, c_int,
                         ctypes.POINTER(PffBatchStatus)],
+    "pff_conv_pack_weights": [c_p, c_int, c_int, c_p, c_p, c_p],
+    "pff_conv3x3_fwd": [c_p, c_int, c_int, c_int, c_int, c_int, c_p, c_p, c_p, c_int, c_p],
+    "pff_conv3x3_bwd_data": [c_p, c_int, c_int, c_int, c_int, c_int, c_p, c_p, c_p],
+    "pff_conv3x3_bwd_weight": [c_p, c_int, c_int, c_int, c_int, c_int, c_p, c_p, c_p, c_p],
+    "pff_relu_bwd": [c_p, c_i64, c_p, c_p],
+    "pff_avgpool2": [c_p, c_int, c_int, c_int, c_int, c_p, c_p],
+    "pff_avgpool2_bwd": [c_p, c_int, c_int, c_int, c_int, c_p, c_p],
+    "pff_upsample2": [c_p, c_int, c_int, c_int, c_int, c_p, c_p],
+    "pff_upsample2_bwd": [c_p, c_int, c_int, c_int, c_int, c_p, c_p],
+    "pff_concat": [c_p, c_i64, c_int, c_int, c_p, c_p, c_p],
+    "pff_concat_bwd": [c_p, c_i64, c_int, c_int, c_p, c_p, c_p],
+    "pff_head1x1": [c_p, c_i64, c_int, c_p, c_p, c_p, c_p],
+    "pff_head1x1_bwd": [c_p, c_i64, c_int, c_p, c_p, c_p, c_p, c_p, c_p],
+    "pff_pack_input": [c_p, c_i64, c_int, c_int, c_p, c_p],
     "pff_fill_z_add_delta": [c_p, G, c_p, c_p, c_p, c_p, c_p, c_p, c_d],
     "pff_topofill": [c_p, G, c_p, c_p, c_p, c_p, c_p, c_d],
     "pff_diffuse_Q_constant_prec": [c_p, G, H, c_p, c_p, c_p],

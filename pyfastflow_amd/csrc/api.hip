@@ -160,13 +160,27 @@ int reroute_core(pff_ctx* ctx, const GridD& d, int32_t* bid, int32_t* rec, int32
   const uint32_t* tact = nullptr;
   int label_rounds = 0;
   int nouter = 1;
+  int64_t nprev = 0;
+  bool relabelled = false;  // this trip's labels came from the previous trip's outlets
   for (int it = 0; it < nouter; ++it) {
     int64_t ndb = 0;
-    if (!exact) {
+    relabelled = false;
+    if (!exact && it > 0) {
+      // labels from the previous labels and outlets (no pass over rec_)
+      int32_t cyc = 0;
+      PFF_TRY(relabel_from_outlets(ctx, d, bid, outlet, nprev, tact, flag_a, &ndb, &cyc));
+      if (cyc) {
+        exact = true;
+      } else {
+        relabelled = true;
+        label_rounds = 0;  // the carve finds its own round count
+      }
+    } else if (!exact) {
       int32_t cyc = 0;
       PFF_TRY(tiled_labels(ctx, d, rec_, tact, &ndb, &cyc, &label_rounds));
       if (cyc) exact = true;
     }
+    nprev = ndb;
     if (exact && bs) {
       *fallback = true;
       return 0;
@@ -185,7 +199,10 @@ int reroute_core(pff_ctx* ctx, const GridD& d, int32_t* bid, int32_t* rec, int32
       st->n_loop++;
     }
     // basin labels on rec_ (:507-511)
-    if (!exact) {
+    if (relabelled) {
+      tact = flag_a;
+      std::swap(flag_a, flag_b);
+    } else if (!exact) {
       PFF_TRY(tiled_labels_write(ctx, d, bid, tact, flag_a));
       tact = flag_a;
       std::swap(flag_a, flag_b);

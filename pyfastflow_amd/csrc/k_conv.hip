@@ -1,0 +1,568 @@
+// k_conv.hip -- kernels of the GraphFlood warm-start surrogate: 3x3 and 1x1
+// convolutions on BF16 NHWC activations with FP32 accumulation on the CDNA4
+// matrix cores (v_mfma_f32_16x16x32_bf16), and the data-movement ops of a
+// U-Net (2x2 average pooling, nearest 2x upsampling, channel concatenation,
+// ReLU backward).  No reference counterpart: the reference has no surrogate.
+//
+// Layouts
+//   activations  x[n][y][x][c] bf16, c contiguous; 3x3 convs need Cin % 32 == 0
+//                and Cout % 32 == 0
+//   weights      master copy f32 w[co][ci][ky][kx] (torch's layout), packed per
+//                call into bf16 wf[tap][co][ci] (forward) and wt[tap][ci][co]
+//                (backward-data, taps mirrored), tap = 3*ky + kx
+//
+// Convolution as an implicit GEMM: rows = output pixels, columns = output
+// channels, k = (tap, input channel).  A block computes 64 consecutive pixels
+// of one image row for 16*NW output channels (NW waves of 16 channels each).
+// The three input rows it needs (66 pixels with the halo, 32 channels at a
+// time) are staged in LDS; every MFMA operand is one 16-byte read: A from LDS
+// (8 channels of one pixel), B from the packed weights (8 input channels of one
+// output channel).  Within one MFMA the hardware pairs A and B elements by the
+// same (lane group, element) position, so the order of the 32 k values inside
+// an operand does not matter as long as A and B use the same one.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <string>
+
+#include "pff_internal.hpp"
+
+namespace pff {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int kPix = 64;        // output pixels per block (one row segment)
+constexpr int kKC = 32;         // input channels per staged chunk (one MFMA k)
+constexpr int kRow = kKC + 8;   // LDS row pitch in bf16: 80 bytes, conflict-free 16-byte reads
+
+__device__ __forceinline__ float bf2f(uint16_t v) { return __uint_as_float((uint32_t)v << 16); }
+
+// f32 -> bf16, round to nearest even (NaN kept quiet)
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  const uint32_t u = __float_as_uint(f);
+  if ((u & 0x7F800000u) == 0x7F800000u && (u & 0x007FFFFFu)) return (uint16_t)((u >> 16) | 0x40);
+  const uint32_t r = u + 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(r >> 16);
+}
+
+// forward 3x3 convolution, stride 1, zero padding 1; optional ReLU
+template <int NW>
+__global__ __launch_bounds__(NW * 64) void k_conv3x3_fwd(int H, int W, int Cin, int Cout,
+                                                          const uint16_t* __restrict__ x,
+                                                          const uint16_t* __restrict__ wf,
+                                                          const float* __restrict__ bias, int relu,
+                                                          uint16_t* __restrict__ y) {
+  __shared__ __attribute__((aligned(16))) uint16_t tile[3][kPix + 2][kRow];
+  const int x0 = blockIdx.x * kPix;
+  const int row = blockIdx.y;  // image * H + yy
+  const int img = row / H, yy = row - img * H;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int co = blockIdx.z * (16 * NW) + 16 * wave + (lane & 15);
+  const int kq = 8 * (lane >> 4);  // this lane's 8 k values within the chunk
+  f32x4 acc[4];
+#pragma unroll
+  for (int g = 0; g < 4; ++g) acc[g] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int ci0 = 0; ci0 < Cin; ci0 += kKC) {
+    // stage rows yy-1..yy+1, pixels x0-1..x0+64, channels ci0..ci0+31 (zero outside)
+    for (int it = threadIdx.x; it < 3 * (kPix + 2) * 4; it += NW * 64) {
+      const int r = it / ((kPix + 2) * 4);
+      const int rem = it - r * (kPix + 2) * 4;
+      const int px = rem >> 2, q = rem & 3;
+      const int gy = yy + r - 1, gx = x0 + px - 1;
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (gy >= 0 && gy < H && gx >= 0 && gx < W)
+        v = *reinterpret_cast<const uint4*>(x + (((size_t)img * H + gy) * W + gx) * Cin + ci0 + 8 * q);
+      *reinterpret_cast<uint4*>(&tile[r][px][8 * q]) = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+      const int ky = tap / 3, kx = tap - 3 * ky;
+      const bf16x8 b = *reinterpret_cast<const bf16x8*>(wf + ((size_t)tap * Cout + co) * Cin + ci0 + kq);
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&tile[ky][16 * g + (lane & 15) + kx][kq]);
+        acc[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[g], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+  // D layout: lane holds rows 4*(lane/16)+r (pixels) of column lane%16 (channel)
+  const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int px = x0 + 16 * g + 4 * (lane >> 4) + r;
+      if (px >= W) continue;
+      float v = acc[g][r] + bv;
+      if (relu) v = fmaxf(v, 0.f);
+      y[(((size_t)img * H + yy) * W + px) * Cout + co] = f2bf(v);
+    }
+  }
+}
+
+// weight gradient dw[co][ci][ky][kx] += sum over pixels of dy[p][co] *
+// x[p shifted by the tap][ci].  A block takes 64 output channels x 32 input
+// channels and a run of image rows; per 64-pixel segment it stages dy
+// transposed (channel-major) and the three shifted input rows transposed, one
+// copy per horizontal shift so every operand read is aligned.
+constexpr int kWgRows = 8;  // image rows per block
+template <int NW>
+__global__ __launch_bounds__(NW * 64) void k_conv3x3_wgrad(int n_img, int H, int W, int Cin, int Cout,
+                                                            const uint16_t* __restrict__ x,
+                                                            const uint16_t* __restrict__ dy, float* __restrict__ dw) {
+  constexpr int NT = NW * 64;
+  __shared__ __attribute__((aligned(16))) uint16_t dyt[16 * NW][kPix + 8];
+  __shared__ __attribute__((aligned(16))) uint16_t xt[3][3][32][kPix + 8];  // [kx][ky][ci][px]
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int co0 = blockIdx.y * (16 * NW), ci0 = blockIdx.z * 32;
+  const int segs = (W + kPix - 1) / kPix;
+  const int64_t nrows = (int64_t)n_img * H;
+  const int64_t r0 = (int64_t)blockIdx.x * kWgRows;
+  const int kq = 8 * (lane >> 4);
+  f32x4 acc[9][2];
+#pragma unroll
+  for (int t = 0; t < 9; ++t)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[t][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int64_t rr = r0; rr < r0 + kWgRows && rr < nrows; ++rr) {
+    const int img = (int)(rr / H), yy = (int)(rr - (int64_t)img * H);
+    for (int sg = 0; sg < segs; ++sg) {
+      const int x0 = sg * kPix;
+      // dy[row][x0..x0+63][co0..co0+16*NW-1] -> dyt[co][px] (zero past the row end)
+      for (int it = threadIdx.x; it < kPix * 2 * NW; it += NT) {
+        const int px = it / (2 * NW), q = it - px * (2 * NW);
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (x0 + px < W)
+          v = *reinterpret_cast<const uint4*>(dy + (((size_t)img * H + yy) * W + x0 + px) * Cout + co0 + 8 * q);
+        const uint16_t* e = reinterpret_cast<const uint16_t*>(&v);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) dyt[8 * q + j][px] = e[j];
+      }
+      // x[row+ky-1][x0+px+kx-1][ci0..ci0+31] -> xt[kx][ky][ci][px]
+      for (int it = threadIdx.x; it < 3 * (kPix + 2) * 4; it += NT) {
+        const int r = it / ((kPix + 2) * 4);
+        const int rem = it - r * (kPix + 2) * 4;
+        const int hp = rem >> 2, q = rem & 3;  // hp: halo pixel 0..65 = x0-1..x0+64
+        const int gy = yy + r - 1, gx = x0 + hp - 1;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (gy >= 0 && gy < H && gx >= 0 && gx < W)
+          v = *reinterpret_cast<const uint4*>(x + (((size_t)img * H + gy) * W + gx) * Cin + ci0 + 8 * q);
+        const uint16_t* e = reinterpret_cast<const uint16_t*>(&v);
+#pragma unroll
+        for (int kx = 0; kx < 3; ++kx) {
+          const int px = hp - kx;  // output pixel whose tap kx reads halo pixel hp
+          if (px < 0 || px >= kPix) continue;
+#pragma unroll
+          for (int j = 0; j < 8; ++j) xt[kx][r][8 * q + j][px] = e[j];
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int ks = 0; ks < kPix / 32; ++ks) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&dyt[16 * wave + (lane & 15)][32 * ks + kq]);
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+          const int ky = tap / 3, kx = tap - 3 * ky;
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            const bf16x8 b = *reinterpret_cast<const bf16x8*>(&xt[kx][ky][16 * j + (lane & 15)][32 * ks + kq]);
+            acc[tap][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[tap][j], 0, 0, 0);
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // rows = output channels 4*(lane/16)+r of this wave's 16, columns = input channels
+#pragma unroll
+  for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int co = co0 + 16 * wave + 4 * (lane >> 4) + r;
+        const int ci = ci0 + 16 * j + (lane & 15);
+        atomicAdd(&dw[((size_t)co * Cin + ci) * 9 + tap], acc[tap][j][r]);
+      }
+}
+
+// bias gradient db[c] += sum over pixels of dy[p][c]
+__global__ __launch_bounds__(256) void k_bias_grad(int64_t npix, int C, const uint16_t* __restrict__ dy,
+                                                  float* __restrict__ db) {
+  extern __shared__ float part[];
+  for (int c = threadIdx.x; c < C; c += 256) part[c] = 0.f;
+  __syncthreads();
+  const int64_t per = (npix + gridDim.x - 1) / gridDim.x;
+  const int64_t p0 = (int64_t)blockIdx.x * per;
+  const int64_t p1 = p0 + per < npix ? p0 + per : npix;
+  for (int64_t e = p0 * C + threadIdx.x; e < p1 * C; e += 256) atomicAdd(&part[e % C], bf2f(dy[e]));
+  __syncthreads();
+  for (int c = threadIdx.x; c < C; c += 256) atomicAdd(&db[c], part[c]);
+}
+
+// wf[tap][co][ci] and wt[tap'][ci][co] (tap' = 8 - tap) from w[co][ci][ky][kx]
+__global__ void k_pack_weights(int Cin, int Cout, const float* __restrict__ w, uint16_t* __restrict__ wf,
+                               uint16_t* __restrict__ wt) {
+  const int64_t n = (int64_t)Cout * Cin * 9;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int tap = (int)(e % 9);
+    const int64_t cc = e / 9;
+    const int ci = (int)(cc % Cin), co = (int)(cc / Cin);
+    const uint16_t v = f2bf(w[e]);
+    if (wf) wf[((int64_t)tap * Cout + co) * Cin + ci] = v;
+    if (wt) wt[((int64_t)(8 - tap) * Cin + ci) * Cout + co] = v;
+  }
+}
+
+__global__ void k_relu_bwd(int64_t n, const uint16_t* __restrict__ y, uint16_t* __restrict__ dy) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+    if (!(bf2f(y[e]) > 0.f)) dy[e] = 0;
+}
+
+// 2x2 average pooling (H, W even) and its backward
+__global__ void k_avgpool2(int n_img, int H, int W, int C, const uint16_t* __restrict__ x, uint16_t* __restrict__ y) {
+  const int Ho = H / 2, Wo = W / 2;
+  const int64_t n = (int64_t)n_img * Ho * Wo * C;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e % C);
+    int64_t p = e / C;
+    const int xo = (int)(p % Wo);
+    p /= Wo;
+    const int yo = (int)(p % Ho);
+    const int img = (int)(p / Ho);
+    const size_t b = (((size_t)img * H + 2 * yo) * W + 2 * xo) * C + c;
+    const float s = bf2f(x[b]) + bf2f(x[b + C]) + bf2f(x[b + (size_t)W * C]) + bf2f(x[b + (size_t)W * C + C]);
+    y[e] = f2bf(0.25f * s);
+  }
+}
+
+__global__ void k_avgpool2_bwd(int n_img, int H, int W, int C, const uint16_t* __restrict__ dy,
+                               uint16_t* __restrict__ dx) {
+  const int Ho = H / 2, Wo = W / 2;
+  const int64_t n = (int64_t)n_img * H * W * C;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e % C);
+    int64_t p = e / C;
+    const int xx = (int)(p % W);
+    p /= W;
+    const int yy = (int)(p % H);
+    const int img = (int)(p / H);
+    dx[e] = f2bf(0.25f * bf2f(dy[(((size_t)img * Ho + yy / 2) * Wo + xx / 2) * C + c]));
+  }
+}
+
+// nearest 2x upsampling and its backward (sum of the 4 children)
+__global__ void k_upsample2(int n_img, int H, int W, int C, const uint16_t* __restrict__ x, uint16_t* __restrict__ y) {
+  const int Ho = 2 * H, Wo = 2 * W;
+  const int64_t n = (int64_t)n_img * Ho * Wo * C;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e % C);
+    int64_t p = e / C;
+    const int xo = (int)(p % Wo);
+    p /= Wo;
+    const int yo = (int)(p % Ho);
+    const int img = (int)(p / Ho);
+    y[e] = x[(((size_t)img * H + yo / 2) * W + xo / 2) * C + c];
+  }
+}
+
+__global__ void k_upsample2_bwd(int n_img, int H, int W, int C, const uint16_t* __restrict__ dy,
+                                uint16_t* __restrict__ dx) {
+  const int Wo = 2 * W;
+  const int64_t n = (int64_t)n_img * H * W * C;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e % C);
+    int64_t p = e / C;
+    const int xx = (int)(p % W);
+    p /= W;
+    const int yy = (int)(p % H);
+    const int img = (int)(p / H);
+    const size_t b = (((size_t)img * 2 * H + 2 * yy) * Wo + 2 * xx) * C + c;
+    const float s = bf2f(dy[b]) + bf2f(dy[b + C]) + bf2f(dy[b + (size_t)Wo * C]) + bf2f(dy[b + (size_t)Wo * C + C]);
+    dx[e] = f2bf(s);
+  }
+}
+
+// y[p] = [a[p], b[p]] along channels, and the split backward
+__global__ void k_concat(int64_t npix, int Ca, int Cb, const uint16_t* __restrict__ a, const uint16_t* __restrict__ b,
+                         uint16_t* __restrict__ y, int backward) {
+  const int C = Ca + Cb;
+  const int64_t n = npix * C;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e % C);
+    const int64_t p = e / C;
+    uint16_t* ap = const_cast<uint16_t*>(a);
+    uint16_t* bp = const_cast<uint16_t*>(b);
+    if (!backward) {
+      y[e] = c < Ca ? a[p * Ca + c] : b[p * Cb + (c - Ca)];
+    } else if (c < Ca) {
+      ap[p * Ca + c] = y[e];
+    } else {
+      bp[p * Cb + (c - Ca)] = y[e];
+    }
+  }
+}
+
+// 1x1 head to one f32 channel: y[p] = b + sum_c x[p][c] w[c]; backward gives
+// dx (bf16), and accumulates dw, db
+__global__ __launch_bounds__(256) void k_head(int64_t npix, int C, const uint16_t* __restrict__ x,
+                                             const float* __restrict__ w, const float* __restrict__ b,
+                                             float* __restrict__ y) {
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += (int64_t)gridDim.x * blockDim.x) {
+    float s = b[0];
+    for (int c = 0; c < C; ++c) s += bf2f(x[p * C + c]) * w[c];
+    y[p] = s;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_head_bwd(int64_t npix, int C, const uint16_t* __restrict__ x,
+                                                 const float* __restrict__ w, const float* __restrict__ dy,
+                                                 uint16_t* __restrict__ dx, float* __restrict__ dw,
+                                                 float* __restrict__ db) {
+  __shared__ float pw[256];
+  __shared__ float pb;
+  for (int c = threadIdx.x; c < C; c += 256) pw[c] = 0.f;
+  if (threadIdx.x == 0) pb = 0.f;
+  __syncthreads();
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += (int64_t)gridDim.x * blockDim.x) {
+    const float g = dy[p];
+    atomicAdd(&pb, g);
+    for (int c = 0; c < C; ++c) {
+      if (dx) dx[p * C + c] = f2bf(g * w[c]);
+      atomicAdd(&pw[c], g * bf2f(x[p * C + c]));
+    }
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < C; c += 256) atomicAdd(&dw[c], pw[c]);
+  if (threadIdx.x == 0) atomicAdd(db, pb);
+}
+
+// f32 planes -> bf16 NHWC with C channels (planes first, zeros after)
+__global__ void k_pack_input(int64_t npix, int nplanes, int C, const float* __restrict__ planes,
+                             uint16_t* __restrict__ x) {
+  const int64_t n = npix * C;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e % C);
+    const int64_t p = e / C;
+    x[e] = c < nplanes ? f2bf(planes[(int64_t)c * npix + p]) : (uint16_t)0;
+  }
+}
+
+unsigned ew_grid(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  if (b > 65536) b = 65536;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+
+}  // namespace
+
+int launch_conv3x3_fwd(pff_ctx* ctx, int n_img, int H, int W, int Cin, int Cout, const uint16_t* x,
+                       const uint16_t* wf, const float* bias, int relu, uint16_t* y) {
+  const dim3 grid((W + kPix - 1) / kPix, n_img * H, Cout / 64 > 0 && Cout % 64 == 0 ? Cout / 64 : Cout / 32);
+  if (Cout % 64 == 0)
+    hipLaunchKernelGGL(k_conv3x3_fwd<4>, grid, dim3(256), 0, ctx->stream, H, W, Cin, Cout, x, wf, bias, relu, y);
+  else
+    hipLaunchKernelGGL(k_conv3x3_fwd<2>, grid, dim3(128), 0, ctx->stream, H, W, Cin, Cout, x, wf, bias, relu, y);
+  return check_hip(hipGetLastError(), "conv3x3_fwd");
+}
+
+int launch_conv3x3_wgrad(pff_ctx* ctx, int n_img, int H, int W, int Cin, int Cout, const uint16_t* x,
+                         const uint16_t* dy, float* dw, float* db) {
+  const int64_t nrows = (int64_t)n_img * H;
+  const unsigned gx = (unsigned)((nrows + kWgRows - 1) / kWgRows);
+  if (Cout % 64 == 0)
+    hipLaunchKernelGGL(k_conv3x3_wgrad<4>, dim3(gx, Cout / 64, Cin / 32), dim3(256), 0, ctx->stream, n_img, H, W, Cin,
+                       Cout, x, dy, dw);
+  else
+    hipLaunchKernelGGL(k_conv3x3_wgrad<2>, dim3(gx, Cout / 32, Cin / 32), dim3(128), 0, ctx->stream, n_img, H, W, Cin,
+                       Cout, x, dy, dw);
+  PFF_TRY(check_hip(hipGetLastError(), "conv3x3_wgrad"));
+  if (db) {
+    const int64_t npix = nrows * W;
+    hipLaunchKernelGGL(k_bias_grad, dim3(512), dim3(256), (size_t)Cout * 4, ctx->stream, npix, Cout, dy, db);
+    PFF_TRY(check_hip(hipGetLastError(), "bias_grad"));
+  }
+  return 0;
+}
+
+int launch_pack_weights(pff_ctx* ctx, int Cin, int Cout, const float* w, uint16_t* wf, uint16_t* wt) {
+  hipLaunchKernelGGL(k_pack_weights, dim3(ew_grid((int64_t)Cin * Cout * 9)), dim3(256), 0, ctx->stream, Cin, Cout, w,
+                     wf, wt);
+  return check_hip(hipGetLastError(), "pack_weights");
+}
+
+int launch_relu_bwd(pff_ctx* ctx, int64_t n, const uint16_t* y, uint16_t* dy) {
+  hipLaunchKernelGGL(k_relu_bwd, dim3(ew_grid(n)), dim3(256), 0, ctx->stream, n, y, dy);
+  return check_hip(hipGetLastError(), "relu_bwd");
+}
+
+int launch_avgpool2(pff_ctx* ctx, int n_img, int H, int W, int C, const uint16_t* x, uint16_t* y, int backward) {
+  const int64_t n = backward ? (int64_t)n_img * H * W * C : (int64_t)n_img * (H / 2) * (W / 2) * C;
+  if (backward)
+    hipLaunchKernelGGL(k_avgpool2_bwd, dim3(ew_grid(n)), dim3(256), 0, ctx->stream, n_img, H, W, C, x, y);
+  else
+    hipLaunchKernelGGL(k_avgpool2, dim3(ew_grid(n)), dim3(256), 0, ctx->stream, n_img, H, W, C, x, y);
+  return check_hip(hipGetLastError(), "avgpool2");
+}
+
+int launch_upsample2(pff_ctx* ctx, int n_img, int H, int W, int C, const uint16_t* x, uint16_t* y, int backward) {
+  const int64_t n = backward ? (int64_t)n_img * H * W * C : (int64_t)n_img * 4 * H * W * C;
+  if (backward)
+    hipLaunchKernelGGL(k_upsample2_bwd, dim3(ew_grid(n)), dim3(256), 0, ctx->stream, n_img, H, W, C, x, y);
+  else
+    hipLaunchKernelGGL(k_upsample2, dim3(ew_grid(n)), dim3(256), 0, ctx->stream, n_img, H, W, C, x, y);
+  return check_hip(hipGetLastError(), "upsample2");
+}
+
+int launch_concat(pff_ctx* ctx, int64_t npix, int Ca, int Cb, const uint16_t* a, const uint16_t* b, uint16_t* y,
+                  int backward) {
+  hipLaunchKernelGGL(k_concat, dim3(ew_grid(npix * (Ca + Cb))), dim3(256), 0, ctx->stream, npix, Ca, Cb, a, b, y,
+                     backward);
+  return check_hip(hipGetLastError(), "concat");
+}
+
+int launch_head(pff_ctx* ctx, int64_t npix, int C, const uint16_t* x, const float* w, const float* b, float* y) {
+  hipLaunchKernelGGL(k_head, dim3(ew_grid(npix)), dim3(256), 0, ctx->stream, npix, C, x, w, b, y);
+  return check_hip(hipGetLastError(), "head");
+}
+
+int launch_head_bwd(pff_ctx* ctx, int64_t npix, int C, const uint16_t* x, const float* w, const float* dy,
+                    uint16_t* dx, float* dw, float* db) {
+  const int64_t nb = (npix + 255) / 256;
+  hipLaunchKernelGGL(k_head_bwd, dim3((unsigned)(nb < 2048 ? (nb < 1 ? 1 : nb) : 2048)), dim3(256), 0, ctx->stream,
+                     npix, C, x, w, dy, dx, dw, db);
+  return check_hip(hipGetLastError(), "head_bwd");
+}
+
+int launch_pack_input(pff_ctx* ctx, int64_t npix, int nplanes, int C, const float* planes, uint16_t* x) {
+  hipLaunchKernelGGL(k_pack_input, dim3(ew_grid(npix * C)), dim3(256), 0, ctx->stream, npix, nplanes, C, planes, x);
+  return check_hip(hipGetLastError(), "pack_input");
+}
+
+}  // namespace pff
+
+// ---- C ABI ---------------------------------------------------------------------
+using namespace pff;
+
+namespace {
+int sg_begin(pff_ctx* ctx) {
+  if (!ctx) return fail(PFF_EINVAL, "context is NULL");
+  return check_hip(hipSetDevice(ctx->device), "hipSetDevice");
+}
+int sg_conv_shape(int32_t n, int32_t h, int32_t w, int32_t cin, int32_t cout, const char* what) {
+  if (n < 1 || h < 1 || w < 1) return fail(PFF_EINVAL, std::string(what) + ": empty image");
+  if (cin < 32 || cin % 32 || cout < 32 || cout % 32)
+    return fail(PFF_EINVAL, std::string(what) + ": channels must be positive multiples of 32");
+  if ((int64_t)n * h >= (int64_t)1 << 31 || (int64_t)n * h * w * std::max(cin, cout) >= (int64_t)1 << 40)
+    return fail(PFF_EINVAL, std::string(what) + ": too large");
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int pff_conv_pack_weights(pff_ctx* ctx, int32_t cin, int32_t cout, const float* w, uint16_t* wf, uint16_t* wt) {
+  PFF_TRY(sg_begin(ctx));
+  if (cin < 1 || cout < 1 || !w || (!wf && !wt)) return fail(PFF_EINVAL, "conv_pack_weights: bad argument");
+  return launch_pack_weights(ctx, cin, cout, w, wf, wt);
+}
+
+int pff_conv3x3_fwd(pff_ctx* ctx, int32_t n, int32_t h, int32_t w, int32_t cin, int32_t cout, const uint16_t* x,
+                    const uint16_t* wf, const float* bias, int32_t relu, uint16_t* y) {
+  PFF_TRY(sg_begin(ctx));
+  PFF_TRY(sg_conv_shape(n, h, w, cin, cout, "conv3x3_fwd"));
+  if (!x || !wf || !y) return fail(PFF_EINVAL, "conv3x3_fwd: NULL array");
+  return launch_conv3x3_fwd(ctx, n, h, w, cin, cout, x, wf, bias, relu, y);
+}
+
+int pff_conv3x3_bwd_data(pff_ctx* ctx, int32_t n, int32_t h, int32_t w, int32_t cin, int32_t cout,
+                         const uint16_t* dy, const uint16_t* wt, uint16_t* dx) {
+  PFF_TRY(sg_begin(ctx));
+  PFF_TRY(sg_conv_shape(n, h, w, cin, cout, "conv3x3_bwd_data"));
+  if (!dy || !wt || !dx) return fail(PFF_EINVAL, "conv3x3_bwd_data: NULL array");
+  // the transposed convolution is a forward one over dy with mirrored,
+  // transposed weights: Cout of the layer becomes the input channels
+  return launch_conv3x3_fwd(ctx, n, h, w, cout, cin, dy, wt, nullptr, 0, dx);
+}
+
+int pff_conv3x3_bwd_weight(pff_ctx* ctx, int32_t n, int32_t h, int32_t w, int32_t cin, int32_t cout,
+                           const uint16_t* x, const uint16_t* dy, float* dw, float* db) {
+  PFF_TRY(sg_begin(ctx));
+  PFF_TRY(sg_conv_shape(n, h, w, cin, cout, "conv3x3_bwd_weight"));
+  if (!x || !dy || !dw) return fail(PFF_EINVAL, "conv3x3_bwd_weight: NULL array");
+  return launch_conv3x3_wgrad(ctx, n, h, w, cin, cout, x, dy, dw, db);
+}
+
+int pff_relu_bwd(pff_ctx* ctx, int64_t n, const uint16_t* y, uint16_t* dy) {
+  PFF_TRY(sg_begin(ctx));
+  if (n < 0 || (n && (!y || !dy))) return fail(PFF_EINVAL, "relu_bwd: bad argument");
+  return n ? launch_relu_bwd(ctx, n, y, dy) : 0;
+}
+
+int pff_avgpool2(pff_ctx* ctx, int32_t n, int32_t h, int32_t w, int32_t c, const uint16_t* x, uint16_t* y) {
+  PFF_TRY(sg_begin(ctx));
+  if (n < 1 || h < 2 || w < 2 || h % 2 || w % 2 || c < 1 || !x || !y)
+    return fail(PFF_EINVAL, "avgpool2: needs even h, w and non-NULL arrays");
+  return launch_avgpool2(ctx, n, h, w, c, x, y, 0);
+}
+
+int pff_avgpool2_bwd(pff_ctx* ctx, int32_t n, int32_t h, int32_t w, int32_t c, const uint16_t* dy, uint16_t* dx) {
+  PFF_TRY(sg_begin(ctx));
+  if (n < 1 || h < 2 || w < 2 || h % 2 || w % 2 || c < 1 || !dy || !dx)
+    return fail(PFF_EINVAL, "avgpool2_bwd: needs even h, w and non-NULL arrays");
+  return launch_avgpool2(ctx, n, h, w, c, dy, dx, 1);
+}
+
+int pff_upsample2(pff_ctx* ctx, int32_t n, int32_t h, int32_t w, int32_t c, const uint16_t* x, uint16_t* y) {
+  PFF_TRY(sg_begin(ctx));
+  if (n < 1 || h < 1 || w < 1 || c < 1 || !x || !y) return fail(PFF_EINVAL, "upsample2: bad argument");
+  return launch_upsample2(ctx, n, h, w, c, x, y, 0);
+}
+
+int pff_upsample2_bwd(pff_ctx* ctx, int32_t n, int32_t h, int32_t w, int32_t c, const uint16_t* dy, uint16_t* dx) {
+  PFF_TRY(sg_begin(ctx));
+  if (n < 1 || h < 1 || w < 1 || c < 1 || !dy || !dx) return fail(PFF_EINVAL, "upsample2_bwd: bad argument");
+  return launch_upsample2(ctx, n, h, w, c, dy, dx, 1);
+}
+
+int pff_concat(pff_ctx* ctx, int64_t npix, int32_t ca, int32_t cb, const uint16_t* a, const uint16_t* b,
+               uint16_t* y) {
+  PFF_TRY(sg_begin(ctx));
+  if (npix < 1 || ca < 1 || cb < 1 || !a || !b || !y) return fail(PFF_EINVAL, "concat: bad argument");
+  return launch_concat(ctx, npix, ca, cb, a, b, y, 0);
+}
+
+int pff_concat_bwd(pff_ctx* ctx, int64_t npix, int32_t ca, int32_t cb, const uint16_t* dy, uint16_t* da,
+                   uint16_t* db) {
+  PFF_TRY(sg_begin(ctx));
+  if (npix < 1 || ca < 1 || cb < 1 || !dy || !da || !db) return fail(PFF_EINVAL, "concat_bwd: bad argument");
+  return launch_concat(ctx, npix, ca, cb, da, db, const_cast<uint16_t*>(dy), 1);
+}
+
+int pff_head1x1(pff_ctx* ctx, int64_t npix, int32_t c, const uint16_t* x, const float* w, const float* b, float* y) {
+  PFF_TRY(sg_begin(ctx));
+  if (npix < 1 || c < 1 || c > 256 || !x || !w || !b || !y) return fail(PFF_EINVAL, "head1x1: bad argument");
+  return launch_head(ctx, npix, c, x, w, b, y);
+}
+
+int pff_head1x1_bwd(pff_ctx* ctx, int64_t npix, int32_t c, const uint16_t* x, const float* w, const float* dy,
+                    uint16_t* dx, float* dw, float* db) {
+  PFF_TRY(sg_begin(ctx));
+  if (npix < 1 || c < 1 || c > 256 || !x || !w || !dy || !dw || !db)
+    return fail(PFF_EINVAL, "head1x1_bwd: bad argument");
+  return launch_head_bwd(ctx, npix, c, x, w, dy, dx, dw, db);
+}
+
+int pff_pack_input(pff_ctx* ctx, int64_t npix, int32_t nplanes, int32_t c, const float* planes, uint16_t* x) {
+  PFF_TRY(sg_begin(ctx));
+  if (npix < 1 || nplanes < 1 || c < nplanes || !planes || !x) return fail(PFF_EINVAL, "pack_input: bad argument");
+  return launch_pack_input(ctx, npix, nplanes, c, planes, x);
+}
+
+}  // extern "C"

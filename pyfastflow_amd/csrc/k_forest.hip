@@ -74,9 +74,13 @@ __global__ __launch_bounds__(kBT) void k_count_init_all(int32_t N, const int32_t
 __global__ __launch_bounds__(kBT) void k_count_gather2(const int32_t* __restrict__ list, const int64_t* count,
                                                        const uint32_t* __restrict__ S, const int32_t* __restrict__ P,
                                                        int32_t* __restrict__ tgt, uint32_t* __restrict__ val,
-                                                       int32_t* __restrict__ nxt, int64_t* next_count) {
+                                                       int32_t* __restrict__ nxt, int64_t* next_count,
+                                                       int64_t* first_empty, int32_t round) {
   const int64_t n = *count;
-  if (blockIdx.x == 0 && threadIdx.x == 0) *next_count = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *next_count = 0;
+    if (n == 0 && first_empty) atomicMin((long long*)first_empty, (long long)round);
+  }
   for (int64_t w = (int64_t)blockIdx.x * kBT + threadIdx.x; w < n; w += (int64_t)gridDim.x * kBT) {
     const int m = list[w];
     const int pm = P[m];
@@ -104,6 +108,40 @@ __global__ __launch_bounds__(kBT) void k_count_scatter2(const int32_t* __restric
       if (w < n) {
         const int m = list[w];
         atomicAdd(&S[tgt[w]], val[w]);
+        const int q = nxt[w];
+        P[m] = q;
+        v[j] = m;
+        take[j] = q >= 0;
+      }
+    }
+    block_append<kItems>(v, take, out, next_count);
+  }
+}
+
+// phase 2 for flags ("is anything upstream marked"): OR into the 2^k-th
+// successor.  OR is idempotent, so a plain store of 1 into a zero target is
+// enough and no atomics are needed; the doubling schedule of the sums
+// delivers every marked descendant to every ancestor at least once.
+__global__ __launch_bounds__(kBT) void k_flag_scatter2(const int32_t* __restrict__ list, const int64_t* count,
+                                                       uint32_t* S, int32_t* P, const int32_t* __restrict__ tgt,
+                                                       const uint32_t* __restrict__ val,
+                                                       const int32_t* __restrict__ nxt, int32_t* __restrict__ out,
+                                                       int64_t* next_count) {
+  const int64_t n = *count;
+  for (int64_t base = (int64_t)blockIdx.x * kChunk; base < n; base += (int64_t)gridDim.x * kChunk) {
+    int32_t v[kItems];
+    bool take[kItems];
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+      const int64_t w = base + j * kBT + threadIdx.x;
+      take[j] = false;
+      v[j] = 0;
+      if (w < n) {
+        const int m = list[w];
+        if (val[w] != 0u) {
+          const int t = tgt[w];
+          if (S[t] == 0u) S[t] = 1u;
+        }
         const int q = nxt[w];
         P[m] = q;
         v[j] = m;
@@ -203,13 +241,17 @@ struct Lists {
 static unsigned list_grid() { return 1024; }
 
 static int count_rounds(pff_ctx* ctx, Lists& L, uint32_t* S, int32_t* P, int32_t* tgt, uint32_t* val, int32_t* nxt,
-                        int rounds) {
-  for (int k = 0; k < rounds; ++k) {
+                        int rounds, bool flags = false, int first_round = 0, int64_t* first_empty = nullptr) {
+  for (int k = first_round; k < first_round + rounds; ++k) {
     const int a = L.cur, b = 1 - L.cur;
     hipLaunchKernelGGL(k_count_gather2, dim3(list_grid()), dim3(kBT), 0, ctx->stream, L.l[a], L.c[a], S, P, tgt, val,
-                       nxt, L.c[b]);
-    hipLaunchKernelGGL(k_count_scatter2, dim3(list_grid()), dim3(kBT), 0, ctx->stream, L.l[a], L.c[a], S, P, tgt, val,
-                       nxt, L.l[b], L.c[b]);
+                       nxt, L.c[b], first_empty, k);
+    if (flags)
+      hipLaunchKernelGGL(k_flag_scatter2, dim3(list_grid()), dim3(kBT), 0, ctx->stream, L.l[a], L.c[a], S, P, tgt,
+                         val, nxt, L.l[b], L.c[b]);
+    else
+      hipLaunchKernelGGL(k_count_scatter2, dim3(list_grid()), dim3(kBT), 0, ctx->stream, L.l[a], L.c[a], S, P, tgt,
+                         val, nxt, L.l[b], L.c[b]);
     L.cur = b;
   }
   return check_hip(hipGetLastError(), "count rounds");
@@ -641,25 +683,35 @@ int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, 
   L.c[1] = ctx->dev_words + W_COUNT1;
   PFF_TRY(check_hip(hipMemcpyAsync(L.c[0], a.n_exits, 8, hipMemcpyDeviceToDevice, s), "copy"));
   hipLaunchKernelGGL(k_exit_link, dim3(list_grid()), dim3(kBT), 0, s, a, b.exits, a.n_exits, 0, b.L0);
+  // saddle tags only ask whether anything upstream is marked: OR rounds
+  const bool flags = wmode == W_SADDLE;
   if (rounds > 0 && !cycle) {
     // the caller knows the depth bound (the labels of the same graph)
-    PFF_TRY(count_rounds(ctx, L, b.S, b.P2, b.t0, (uint32_t*)b.t1, b.t2, rounds));
+    PFF_TRY(count_rounds(ctx, L, b.S, b.P2, b.t0, (uint32_t*)b.t1, b.t2, rounds, flags));
   } else {
+    // start from the round count the previous call of this kind needed,
+    // then confirm on the host that the list has emptied
     const int cap = ceil_log2i(g.N) + 1;
+    int& last = flags ? ctx->last_tag_rounds : ctx->last_count_rounds;
+    int64_t* fe = ctx->dev_words + W_EMPTY2;
+    PFF_TRY(launch_fill_i64(ctx, 1, fe, (int64_t)1 << 40));
     int done_rounds = 0;
     int64_t live = 1;
-    int step = rounds > 0 ? rounds : 8;
+    int step = rounds > 0 ? rounds : std::max(2, std::min(last + 1, cap));
     while (done_rounds < cap) {
       const int r = std::min(step, cap - done_rounds);
-      PFF_TRY(count_rounds(ctx, L, b.S, b.P2, b.t0, (uint32_t*)b.t1, b.t2, r));
+      PFF_TRY(count_rounds(ctx, L, b.S, b.P2, b.t0, (uint32_t*)b.t1, b.t2, r, flags, done_rounds, fe));
       done_rounds += r;
-      step = 4;
+      step = 3;
       PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_LIVE, L.c[L.cur], 8, hipMemcpyDeviceToHost, s), "d2h"));
       PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_CYCLE, a.cycle, 8, hipMemcpyDeviceToHost, s), "d2h"));
+      PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_EMPTY2, fe, 8, hipMemcpyDeviceToHost, s), "d2h"));
       PFF_TRY(check_hip(hipStreamSynchronize(s), "sync"));
       live = ctx->host_words[W_LIVE];
       if (live == 0 || ctx->host_words[W_CYCLE] != 0) break;
     }
+    const int64_t f = ctx->host_words[W_EMPTY2];
+    last = (live == 0 && f < done_rounds) ? std::max<int>((int)f, 1) : done_rounds;
     const int cyc = (live != 0 || ctx->host_words[W_CYCLE] != 0) ? 1 : 0;
     if (cycle) *cycle = cyc;
     if (cyc) return cycle ? 0 : fail(PFF_ENOTCONV, "tiled_upstream: receivers are not a forest");
@@ -752,6 +804,161 @@ int tiled_pits(pff_ctx* ctx, const GridD& g, int32_t** pits, int64_t** npits) {
   PFF_TRY(tile_bufs(ctx, g, b));
   *pits = b.pits;
   *npits = ctx->dev_words + W_NPIT;
+  return 0;
+}
+
+// ---- basin-level relabel between lakeflow trips ------------------------------
+// After a trip, the receivers change only on the carved paths: every cell of a
+// carved basin b now drains through its saddle node to its outlet cell, which
+// lies in basin bid[outlet(b)] (0 if it drains out), and every other cell
+// keeps its path.  The next trip's labels init(root) (lakeflow.py:507-511)
+// are therefore the old labels mapped through the basin-level chain
+// b -> bid[outlet(b)] -> ... to its end: an uncarved basin (its pit stays a
+// root) or 0.  The pits of the next trip are the uncarved basins.
+
+// M[b] = bid[outlet node of b], or b when b has no outlet (then b is kept in
+// the next pits list); M[0] = 0
+__global__ __launch_bounds__(kBT) void k_basin_next(const int32_t* __restrict__ pits, const int64_t* np,
+                                                    const int64_t* __restrict__ outlet,
+                                                    const int32_t* __restrict__ bid, int32_t N, int32_t* __restrict__ M0,
+                                                    int32_t* __restrict__ M1, int32_t* __restrict__ next_pits,
+                                                    int64_t* n_next) {
+  const int64_t n = *np;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    M0[0] = 0;
+    M1[0] = 0;
+  }
+  // the loop bound is uniform per block: block_append needs every thread
+  for (int64_t base = (int64_t)blockIdx.x * kBT; base < n; base += (int64_t)gridDim.x * kBT) {
+    const int64_t w = base + threadIdx.x;
+    int32_t v[1] = {0};
+    bool take[1] = {false};
+    if (w < n) {
+      const int b = pits[w];
+      if (b > 0 && b < N) {
+        const int64_t o = outlet[b];
+        int m = b;
+        if (o != INVALID_KEY) {
+          const int r = key_index(o);
+          m = (r >= 0 && r < N) ? bid[r] : 0;
+        } else {
+          v[0] = b;
+          take[0] = true;
+        }
+        M0[b] = m;
+        M1[b] = m;
+      }
+    }
+    block_append<1, kBT>(v, take, next_pits, n_next);
+  }
+}
+
+// one synchronous pointer-jumping round Mout[b] = Min[Min[b]] over the pits;
+// *chg = the latest round (1-based) in which a label changed
+__global__ __launch_bounds__(kBT) void k_basin_jump(const int32_t* __restrict__ pits, const int64_t* np, int32_t N,
+                                                    const int32_t* __restrict__ Min, int32_t* __restrict__ Mout,
+                                                    int round, int64_t* chg) {
+  const int64_t n = *np;
+  for (int64_t base = (int64_t)blockIdx.x * kBT; base < n; base += (int64_t)gridDim.x * kBT) {
+    const int64_t w = base + threadIdx.x;
+    bool changed = false;
+    if (w < n) {
+      const int b = pits[w];
+      if (b > 0 && b < N) {
+        const int m = Min[b];
+        const int mm = (m >= 0 && m < N) ? Min[m] : 0;
+        Mout[b] = mm;
+        changed = mm != m;
+      }
+    }
+    // rounds run in order, so a plain store of this round's number is the
+    // latest change; one store per block that saw a change
+    if (__syncthreads_or(changed) && threadIdx.x == 0) *chg = round;
+  }
+}
+
+// bid <- M[bid] on the tiles of tact_in; flags the tiles with a closed basin
+__global__ __launch_bounds__(kBT) void k_relabel_tiles(GridD g, int tiles_x, const uint32_t* __restrict__ tact_in,
+                                                       const int32_t* __restrict__ M, int32_t* __restrict__ bid,
+                                                       uint32_t* __restrict__ tact_out) {
+  int bx, by;
+  xcd_block(bx, by);
+  const int t = (by >> 4) * tiles_x + bx;
+  if (tact_in && tact_in[t] == 0u) return;  // uniform per block
+  const int col = bx * 64 + threadIdx.x;
+  const int row = by * 4 + threadIdx.y;
+  int any = 0;
+  if (col < g.nx && row < g.ny) {
+    const int c = row * g.nx + col;
+    const int b = bid[c];
+    if (b != 0) {
+      const int nb = (b > 0 && b < g.N) ? M[b] : 0;
+      if (nb != b) bid[c] = nb;
+      any = nb != 0;
+    }
+  }
+  any = __syncthreads_or(any);
+  if (any && threadIdx.x == 0 && threadIdx.y == 0) atomicOr(&tact_out[t], 1u);
+}
+
+int relabel_from_outlets(pff_ctx* ctx, const GridD& g, int32_t* bid, const int64_t* outlet, int64_t n_prev,
+                         const uint32_t* tact_in, uint32_t* tact_out, int64_t* ndep, int32_t* cycle) {
+  TileBufs b;
+  PFF_TRY(tile_bufs(ctx, g, b));
+  const size_t n4 = (size_t)g.N * 4;
+  void *m0, *m1, *np2;
+  PFF_TRY(scratch(ctx, S_M0, n4, &m0));
+  PFF_TRY(scratch(ctx, S_M1, n4, &m1));
+  PFF_TRY(scratch(ctx, S_TPIT2, n4, &np2));
+  int32_t* M[2] = {(int32_t*)m0, (int32_t*)m1};
+  int32_t* next_pits = (int32_t*)np2;
+  hipStream_t s = ctx->stream;
+  int64_t* npit = ctx->dev_words + W_NPIT;
+  int64_t* n_next = ctx->dev_words + W_NPIT2;
+  int64_t* chg = ctx->dev_words + W_MCHG;
+  *cycle = 0;
+  PFF_TRY(check_hip(hipMemsetAsync(n_next, 0, 8, s), "memset"));
+  PFF_TRY(check_hip(hipMemsetAsync(chg, 0, 8, s), "memset"));
+  const unsigned lg = grid_for(std::max<int64_t>(n_prev, 1), kBT, 4096);
+  hipLaunchKernelGGL(k_basin_next, dim3(lg), dim3(kBT), 0, s, (const int32_t*)b.pits, (const int64_t*)npit, outlet,
+                     (const int32_t*)bid, g.N, M[0], M[1], next_pits, n_next);
+  // a chain of basins is at most n_prev long: ceil(log2 n_prev) + 1 rounds
+  // reach every end; a label still changing then means a cycle
+  const int cap = ceil_log2i(std::max<int64_t>(n_prev, 2)) + 1;
+  int cur = 0, done = 0;
+  int step = std::max(2, std::min(cap + 1, ctx->last_jump_rounds));
+  while (true) {
+    for (int k = 0; k < step; ++k) {
+      hipLaunchKernelGGL(k_basin_jump, dim3(lg), dim3(kBT), 0, s, (const int32_t*)b.pits, (const int64_t*)npit, g.N,
+                         (const int32_t*)M[cur], M[cur ^ 1], done + k + 1, chg);
+      cur ^= 1;
+    }
+    done += step;
+    PFF_TRY(check_hip(hipGetLastError(), "basin_jump"));
+    PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_MCHG, chg, 8, hipMemcpyDeviceToHost, s), "d2h"));
+    PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_NPIT2, n_next, 8, hipMemcpyDeviceToHost, s), "d2h"));
+    PFF_TRY(check_hip(hipStreamSynchronize(s), "sync"));
+    // converged when the last round changed nothing
+    if (ctx->host_words[W_MCHG] < done) {
+      ctx->last_jump_rounds = std::max<int>((int)ctx->host_words[W_MCHG] + 1, 1);
+      break;
+    }
+    if (done >= cap + 1) {
+      *cycle = 1;
+      return 0;
+    }
+    step = std::min(4, cap + 1 - done);
+  }
+  const int64_t n2 = ctx->host_words[W_NPIT2];
+  PFF_TRY(check_hip(hipMemsetAsync(tact_out, 0, (size_t)tiles_count(g) * 4, s), "memset"));
+  hipLaunchKernelGGL(k_relabel_tiles, dim3((g.nx + 63) / 64, (g.ny + 3) / 4), dim3(64, 4), 0, s, g,
+                     (g.nx + TS - 1) / TS, tact_in, (const int32_t*)M[cur], bid, tact_out);
+  PFF_TRY(check_hip(hipGetLastError(), "relabel_tiles"));
+  // the uncarved basins become the pits list
+  if (n2 > 0)
+    PFF_TRY(check_hip(hipMemcpyAsync(b.pits, next_pits, (size_t)n2 * 4, hipMemcpyDeviceToDevice, s), "copy"));
+  PFF_TRY(check_hip(hipMemcpyAsync(npit, n_next, 8, hipMemcpyDeviceToDevice, s), "copy"));
+  *ndep = n2;
   return 0;
 }
 

@@ -21,8 +21,12 @@ struct pff_ctx {
   int64_t* host_words = nullptr;
   // device words: counters and flags
   int64_t* dev_words = nullptr;
-  // reduced-graph rounds the last tiled labelling needed (a start estimate)
+  // reduced-graph rounds the last tiled labelling / upstream count / saddle
+  // tagging and basin-chain jump needed (start estimates for the next call)
   int last_label_rounds = 8;
+  int last_count_rounds = 8;
+  int last_tag_rounds = 8;
+  int last_jump_rounds = 4;
   // opt-in per-kernel timing (pff_ctx_profile): HIP events around each
   // instrumented launch on the launch stream, plus the cells it processed
   struct ProfRec {
@@ -63,11 +67,12 @@ enum Slot : int {
   S_TE, S_TP2, S_TBID, S_TEXIT, S_TS, S_TIN, S_TPIT, S_TACT,  // tiled contraction buffers
   S_GF0,    // GraphFlood driver fields (one block, see graphflood_fields)
   S_BATCH,  // tile-canvas arrays of pff_batch_route (one block)
-  S_BATCHT  // tile-canvas tables (per tile)
+  S_BATCHT, // tile-canvas tables (per tile)
+  S_M0, S_M1, S_TPIT2  // basin-level relabel: label maps (by basin id) and the next pits list
 };
 
 // device word indices (dev_words / host_words)
-enum Word : int { W_COUNT0 = 0, W_COUNT1, W_COUNT2, W_FLAG, W_FLAG2, W_CYCLE, W_NEXIT, W_NDEP, W_NPIT, W_LIVE, W_EMPTY, W_GFMAX, W_GFNAN, W_NWORDS = 16 };
+enum Word : int { W_COUNT0 = 0, W_COUNT1, W_COUNT2, W_FLAG, W_FLAG2, W_CYCLE, W_NEXIT, W_NDEP, W_NPIT, W_LIVE, W_EMPTY, W_GFMAX, W_GFNAN, W_NPIT2, W_MCHG, W_EMPTY2, W_NWORDS = 16 };
 
 constexpr int kBlock = 256;
 inline unsigned grid_for(int64_t n, int block = kBlock, int64_t cap = 1 << 20) {
@@ -107,6 +112,16 @@ int tiled_labels(pff_ctx* ctx, const GridD& g, const int32_t* rec, const uint32_
                  int32_t* cycle, int* rounds_used = nullptr);
 int tiled_labels_write(pff_ctx* ctx, const GridD& g, int32_t* bid, const uint32_t* tact_in, uint32_t* tact_out);
 int tiled_pits(pff_ctx* ctx, const GridD& g, int32_t** pits, int64_t** npits);
+// Labels of the next lakeflow trip from this trip's labels and outlets,
+// without a pass over the receivers: a carved basin's cells all drain through
+// its saddle to its outlet, so their new label is the label at the end of the
+// basin-level chain b -> bid[outlet(b)] -> ...; uncarved basins keep theirs.
+// Rewrites bid on the tiles of tact_in, flags in tact_out the tiles that still
+// hold a closed basin, and replaces the pits list by the uncarved basins.
+// *ndep (host): the new depression count; *cycle (host): the chain has a
+// cycle (the labels are then not written).  n_prev: pits in the current list.
+int relabel_from_outlets(pff_ctx* ctx, const GridD& g, int32_t* bid, const int64_t* outlet, int64_t n_prev,
+                         const uint32_t* tact_in, uint32_t* tact_out, int64_t* ndep, int32_t* cycle);
 int tile_flags(pff_ctx* ctx, const GridD& g, uint32_t** flags_a, uint32_t** flags_b);
 int tiles_count(const GridD& g);
 int carve_tiled(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t* rec_in, const int32_t* bid,
@@ -157,6 +172,22 @@ int batch_freeze(pff_ctx* ctx, BatchState* bs, const GridD& d, int32_t* bid);
 int batch_mark42(pff_ctx* ctx, BatchState* bs, const GridD& d, const int32_t* bid, uint8_t value);
 int batch_route(pff_ctx* ctx, int32_t ntiles, const pff_tile* tiles, float dx, int32_t lakeflow, double value,
                 int32_t area, pff_batch_status* st);
+
+// surrogate (k_conv.hip)
+int launch_conv3x3_fwd(pff_ctx* ctx, int n_img, int H, int W, int Cin, int Cout, const uint16_t* x,
+                       const uint16_t* wf, const float* bias, int relu, uint16_t* y);
+int launch_conv3x3_wgrad(pff_ctx* ctx, int n_img, int H, int W, int Cin, int Cout, const uint16_t* x,
+                         const uint16_t* dy, float* dw, float* db);
+int launch_pack_weights(pff_ctx* ctx, int Cin, int Cout, const float* w, uint16_t* wf, uint16_t* wt);
+int launch_relu_bwd(pff_ctx* ctx, int64_t n, const uint16_t* y, uint16_t* dy);
+int launch_avgpool2(pff_ctx* ctx, int n_img, int H, int W, int C, const uint16_t* x, uint16_t* y, int backward);
+int launch_upsample2(pff_ctx* ctx, int n_img, int H, int W, int C, const uint16_t* x, uint16_t* y, int backward);
+int launch_concat(pff_ctx* ctx, int64_t npix, int Ca, int Cb, const uint16_t* a, const uint16_t* b, uint16_t* y,
+                  int backward);
+int launch_head(pff_ctx* ctx, int64_t npix, int C, const uint16_t* x, const float* w, const float* b, float* y);
+int launch_head_bwd(pff_ctx* ctx, int64_t npix, int C, const uint16_t* x, const float* w, const float* dy,
+                    uint16_t* dx, float* dw, float* db);
+int launch_pack_input(pff_ctx* ctx, int64_t npix, int nplanes, int C, const float* planes, uint16_t* x);
 
 int launch_edge_maps(pff_ctx* ctx, const GridD& g, uint8_t* edges, uint8_t* out);
 int launch_maxdiff(pff_ctx* ctx, int32_t N, const float* a, const float* b, uint32_t* out_max,
