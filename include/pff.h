@@ -137,6 +137,17 @@ int pff_reroute_flow(pff_ctx* ctx, const pff_grid* g, int32_t* bid, int32_t* rec
                      int32_t* basin_saddlenode, uint8_t* tag, uint8_t* tag_, int32_t carve, int32_t* bid_first,
                      pff_lake_status* st);
 
+/* Arms a log of the lake outlets for the next pff_reroute_flow call on this
+   context (then disarms; that call syncs).  After each trip's saddle and
+   cycle passes (lakeflow.py:520, the state after :255-281), every basin b of
+   the trip appends (b, outlet[b], basin_saddlenode[b]) to the device arrays
+   basin/outlet/saddlenode (capacity entries; entries past it are counted,
+   not written).  trip_counts (host, max_trips) receives the cumulative
+   number of entries at the end of each trip, -1 for trips not run.  No
+   reference counterpart: the reference keeps only the last trip's arrays. */
+int pff_ctx_lake_log(pff_ctx* ctx, int32_t* basin, int64_t* outlet, int32_t* saddlenode, int64_t capacity,
+                     int64_t* trip_counts, int32_t max_trips);
+
 /* ---- flow: fill -------------------------------------------------------- */
 /* fill_topo.py:99-125  fill_z_add_delta(zh, h, z_, receivers, receivers_, receivers__, epsilon) */
 int pff_fill_z_add_delta(pff_ctx* ctx, const pff_grid* g, float* zh, float* h, float* z_, const int32_t* rcv,
@@ -162,6 +173,8 @@ typedef struct {
   int32_t converged;       /* 1 when residual < tol at a check */
   int32_t checks;          /* number of residual checks */
   int64_t nan_cells;       /* cells whose h is NaN at the end (h < 0 makes h^(5/3) NaN) */
+  double above_frac;       /* fraction of cells with |h(t) - h(t - check_every)| > tol (or NaN), last check */
+  double first_above_frac; /* the same at the first check */
 } pff_gf_status;
 
 /* gf_fields.py:111-141  Flooder.run_graphflood(N): n_iter iterations of
@@ -172,12 +185,14 @@ typedef struct {
    lakeflow trip.  st (host, nullable). */
 int pff_graphflood_run(pff_ctx* ctx, const pff_grid* g, const pff_hydro* p, float* zh, float* h, float* Q,
                        int32_t* rec, int32_t n_iter, int32_t n_diffuse, double fill_eps, pff_gf_status* st);
-/* The same iteration until max |h(t) - h(t - check_every)| < tol, checked
-   every check_every iterations (steady state), or max_iter iterations.  No
-   reference counterpart: the reference runs a fixed count (gf_fields.py:126). */
+/* The same iteration until it is steady, checked every check_every
+   iterations on d = |h(t) - h(t - check_every)|: with frac = 0, when
+   max d < tol; with 0 < frac < 1, when at most frac*N cells have d > tol.
+   Stops at max_iter iterations or when h has NaN cells.  No reference
+   counterpart: the reference runs a fixed count (gf_fields.py:126). */
 int pff_graphflood_to_steady(pff_ctx* ctx, const pff_grid* g, const pff_hydro* p, float* zh, float* h, float* Q,
-                             int32_t* rec, double tol, int32_t check_every, int64_t max_iter, int32_t n_diffuse,
-                             double fill_eps, pff_gf_status* st);
+                             int32_t* rec, double tol, double frac, int32_t check_every, int64_t max_iter,
+                             int32_t n_diffuse, double fill_eps, pff_gf_status* st);
 
 /* ---- ragged batch of independent grids ---------------------------------- */
 /* One grid of a batch: 'normal' boundaries, its own nx, ny; device arrays. */

@@ -49,6 +49,8 @@ class PffGfStatus(ctypes.Structure):
         ("converged", c_int),
         ("checks", c_int),
         ("nan_cells", c_i64),
+        ("above_frac", c_d),
+        ("first_above_frac", c_d),
     ]
 
 
@@ -99,6 +101,7 @@ SIGNATURES = {
     "pff_reroute_carve": [c_p, G, c_p, c_p, c_p, c_p, c_p, c_p, c_p],
     "pff_reroute_flow": [c_p, G, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_int, c_p,
                          ctypes.POINTER(PffLakeStatus)],
+    "pff_ctx_lake_log": [c_p, c_p, c_p, c_p, c_i64, ctypes.POINTER(c_i64), c_int],
     "pff_batch_route": [c_p, c_int, ctypes.POINTER(PffTile), c_f, c_int, c_d, c_int,
                         ctypes.POINTER(PffBatchStatus)],
     "pff_conv_pack_weights": [c_p, c_int, c_int, c_p, c_p, c_p],
@@ -120,7 +123,7 @@ SIGNATURES = {
     "pff_diffuse_Q_constant_prec": [c_p, G, H, c_p, c_p, c_p],
     "pff_graphflood_core_cte_mannings": [c_p, G, H, c_p, c_p, c_p, c_p, c_p],
     "pff_graphflood_run": [c_p, G, H, c_p, c_p, c_p, c_p, c_int, c_int, c_d, ctypes.POINTER(PffGfStatus)],
-    "pff_graphflood_to_steady": [c_p, G, H, c_p, c_p, c_p, c_p, c_d, c_int, c_i64, c_int, c_d,
+    "pff_graphflood_to_steady": [c_p, G, H, c_p, c_p, c_p, c_p, c_d, c_d, c_int, c_i64, c_int, c_d,
                                  ctypes.POINTER(PffGfStatus)],
     "pff_pack_full_float_index": [c_p, c_i64, c_p, c_p, c_p],
     "pff_unpack_full_float_index": [c_p, c_i64, c_p, c_p, c_p],

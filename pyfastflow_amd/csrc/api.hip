@@ -64,6 +64,22 @@ int scratch(pff_ctx* ctx, int slot, size_t bytes, void** out, bool zero_new) {
   return 0;
 }
 
+int wait_stream(pff_ctx* ctx) {
+  if (!ctx->sync_ev) PFF_TRY(check_hip(hipEventCreateWithFlags(&ctx->sync_ev, hipEventDisableTiming), "event"));
+  PFF_TRY(check_hip(hipEventRecord(ctx->sync_ev, ctx->stream), "event record"));
+  for (;;) {
+    const hipError_t e = hipEventQuery(ctx->sync_ev);
+    if (e == hipSuccess) return 0;
+    if (e != hipErrorNotReady) return check_hip(e, "event query");
+  }
+}
+
+int read_words(pff_ctx* ctx) {
+  PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words, ctx->dev_words, W_NWORDS * sizeof(int64_t),
+                                   hipMemcpyDeviceToHost, ctx->stream), "d2h"));
+  return wait_stream(ctx);
+}
+
 unsigned long long* prof_begin(pff_ctx* ctx, const char* name, double bytes_per_cell) {
   if (!ctx->prof_on || (int)ctx->prof.size() >= pff_ctx::kProfSlots) return nullptr;
   pff_ctx::ProfRec r;
@@ -119,6 +135,15 @@ int accumulate_core(pff_ctx* ctx, const GridD& d, const int32_t* rcv, float q0, 
   }
   if (converged) *converged = cycle ? 0 : 1;
   return 0;
+}
+
+// armed lake-outlet log: entries of this trip, cumulative count per trip
+static int log_trip(pff_ctx* ctx, const GridD& d, const int32_t* bid, const int64_t* outlet, const int32_t* snode,
+                    int trip) {
+  auto& L = ctx->lake_log;
+  if (!L.armed || trip >= L.max_trips) return 0;
+  PFF_TRY(launch_log_outlets(ctx, d, bid, outlet, snode, L.dev_count));
+  return check_hip(hipMemcpyAsync(L.trip_counts + trip, L.dev_count, 8, hipMemcpyDeviceToHost, ctx->stream), "d2h");
 }
 
 // lakeflow.py:466-544 reroute_flow, the trip loop.  rec_ holds the working
@@ -225,6 +250,7 @@ int reroute_core(pff_ctx* ctx, const GridD& d, int32_t* bid, int32_t* rec, int32
       PFF_TRY(saddlesort_active(ctx, d, bid, is_border, z_prime, basin_saddle, basin_saddlenode, outlet, z, tact, pits,
                                 npits));
       if (bs) PFF_TRY(batch_mark42(ctx, bs, d, bid, 0));
+      PFF_TRY(log_trip(ctx, d, bid, outlet, basin_saddlenode, st ? st->n_iter : it));
       if (st) st->n_iter++;
       // carve (:525-526): rec equals rec_ here; rec is edited, then copied to rec_
       int32_t cyc = 0;
@@ -242,6 +268,7 @@ int reroute_core(pff_ctx* ctx, const GridD& d, int32_t* bid, int32_t* rec, int32
       PFF_TRY(launch_saddlesort(ctx, d, bid, is_border, z_prime, basin_saddle, basin_saddlenode, outlet, z));
     } else {
       PFF_TRY(launch_saddlesort(ctx, d, bid, is_border, z_prime, basin_saddle, basin_saddlenode, outlet, z));
+      PFF_TRY(log_trip(ctx, d, bid, outlet, basin_saddlenode, st ? st->n_iter : it));
       if (st) st->n_iter++;
     }
     if (carve) {
@@ -294,6 +321,8 @@ int pff_ctx_destroy(pff_ctx* ctx) {
   for (int s = 0; s < pff_ctx::kSlots; ++s)
     if (ctx->buf[s]) (void)hipFree(ctx->buf[s]);
   if (ctx->dev_words) (void)hipFree(ctx->dev_words);
+  if (ctx->lake_log.dev_count) (void)hipFree(ctx->lake_log.dev_count);
+  if (ctx->sync_ev) (void)hipEventDestroy(ctx->sync_ev);
   if (ctx->host_words) (void)hipHostFree(ctx->host_words);
   delete ctx;
   return 0;
@@ -493,8 +522,34 @@ int pff_reroute_flow(pff_ctx* ctx, const pff_grid* g, int32_t* bid, int32_t* rec
     return fail(PFF_EINVAL, "reroute_flow: NULL array");
   (void)tag;
   (void)tag_;
-  return reroute_core(ctx, d, bid, rec, rec_, rec__, z, z_prime, is_border, outlet, basin_saddle, basin_saddlenode,
-                      carve, bid_first, st);
+  const int rc = reroute_core(ctx, d, bid, rec, rec_, rec__, z, z_prime, is_border, outlet, basin_saddle,
+                              basin_saddlenode, carve, bid_first, st);
+  if (ctx->lake_log.armed) {
+    ctx->lake_log.armed = false;
+    PFF_TRY(check_hip(hipStreamSynchronize(ctx->stream), "sync"));
+  }
+  return rc;
+}
+
+int pff_ctx_lake_log(pff_ctx* ctx, int32_t* basin, int64_t* outlet, int32_t* saddlenode, int64_t capacity,
+                     int64_t* trip_counts, int32_t max_trips) {
+  PFF_TRY(begin(ctx));
+  auto& L = ctx->lake_log;
+  if (!basin || !outlet || !saddlenode || capacity < 0 || !trip_counts || max_trips < 0)
+    return fail(PFF_EINVAL, "ctx_lake_log: bad argument");
+  if (!L.dev_count) {
+    if (hipMalloc((void**)&L.dev_count, 8) != hipSuccess) return fail(PFF_ENOMEM, "lake log counter");
+  }
+  PFF_TRY(check_hip(hipMemsetAsync(L.dev_count, 0, 8, ctx->stream), "memset"));
+  for (int32_t t = 0; t < max_trips; ++t) trip_counts[t] = -1;
+  L.basin = basin;
+  L.outlet = outlet;
+  L.snode = saddlenode;
+  L.capacity = capacity;
+  L.trip_counts = trip_counts;
+  L.max_trips = max_trips;
+  L.armed = true;
+  return 0;
 }
 
 // ---- ragged batch -----------------------------------------------------------
@@ -625,19 +680,21 @@ int gf_iteration(pff_ctx* ctx, const pff_grid* g, const GridD& d, const pff_hydr
   return launch_manning(ctx, d, p, h, zh, f.dh, rec, Q);
 }
 
-int gf_check(pff_ctx* ctx, const GridD& d, const float* h, const float* hprev, double* res, int64_t* nans) {
+int gf_check(pff_ctx* ctx, const GridD& d, const float* h, const float* hprev, double tol, double* res,
+             int64_t* nans, int64_t* above) {
   uint32_t* mx = (uint32_t*)(ctx->dev_words + W_GFMAX);
   unsigned long long* nn = (unsigned long long*)(ctx->dev_words + W_GFNAN);
+  unsigned long long* ab = (unsigned long long*)(ctx->dev_words + W_GFABOVE);
   PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_GFMAX, 0, 16, ctx->stream), "memset"));
-  PFF_TRY(launch_maxdiff(ctx, d.N, h, hprev, mx, nn));
-  PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_GFMAX, ctx->dev_words + W_GFMAX, 16, hipMemcpyDeviceToHost,
-                                   ctx->stream), "d2h"));
-  PFF_TRY(check_hip(hipStreamSynchronize(ctx->stream), "sync"));
+  PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_GFABOVE, 0, 8, ctx->stream), "memset"));
+  PFF_TRY(launch_maxdiff(ctx, d.N, h, hprev, mx, nn, (float)tol, ab));
+  PFF_TRY(read_words(ctx));
   uint32_t bits = (uint32_t)(ctx->host_words[W_GFMAX] & 0xFFFFFFFFu);
   float m;
   std::memcpy(&m, &bits, 4);
   *res = (double)m;
   *nans = ctx->host_words[W_GFNAN];
+  *above = ctx->host_words[W_GFABOVE];
   return 0;
 }
 
@@ -663,19 +720,22 @@ int pff_graphflood_run(pff_ctx* ctx, const pff_grid* g, const pff_hydro* p, floa
 }
 
 int pff_graphflood_to_steady(pff_ctx* ctx, const pff_grid* g, const pff_hydro* p, float* zh, float* h, float* Q,
-                             int32_t* rec, double tol, int32_t check_every, int64_t max_iter, int32_t n_diffuse,
-                             double fill_eps, pff_gf_status* st) {
+                             int32_t* rec, double tol, double frac, int32_t check_every, int64_t max_iter,
+                             int32_t n_diffuse, double fill_eps, pff_gf_status* st) {
   PFF_TRY(begin(ctx));
   GridD d;
   PFF_TRY(check_grid(g, &d));
   if (!p || !zh || !h || !Q || !rec) return fail(PFF_EINVAL, "graphflood_to_steady: NULL argument");
   if (check_every < 1 || max_iter < 0 || n_diffuse < 0) return fail(PFF_EINVAL, "graphflood_to_steady: bad counts");
+  if (!(frac >= 0.0 && frac < 1.0)) return fail(PFF_EINVAL, "graphflood_to_steady: frac must be in [0, 1)");
   GfFields f;
   PFF_TRY(gf_fields(ctx, d, f));
   pff_gf_status s;
   std::memset(&s, 0, sizeof(s));
   s.residual = -1.0;
   s.first_residual = -1.0;
+  s.above_frac = -1.0;
+  s.first_above_frac = -1.0;
   const size_t n4 = (size_t)d.N * 4;
   PFF_TRY(check_hip(hipMemcpyAsync(f.hprev, h, n4, hipMemcpyDeviceToDevice, ctx->stream), "copy"));
   while (s.iterations < max_iter) {
@@ -684,14 +744,20 @@ int pff_graphflood_to_steady(pff_ctx* ctx, const pff_grid* g, const pff_hydro* p
     s.iterations += n;
     if (n < check_every) break;  // a partial block cannot be judged
     double res = 0;
-    int64_t nans = 0;
-    PFF_TRY(gf_check(ctx, d, h, f.hprev, &res, &nans));
+    int64_t nans = 0, above = 0;
+    PFF_TRY(gf_check(ctx, d, h, f.hprev, tol, &res, &nans, &above));
     s.checks++;
     s.residual = res;
     s.nan_cells = nans;
-    if (s.first_residual < 0) s.first_residual = res;
+    s.above_frac = (double)above / (double)d.N;
+    if (s.first_residual < 0) {
+      s.first_residual = res;
+      s.first_above_frac = s.above_frac;
+    }
     if (nans > 0) break;
-    if (res < tol) {
+    // frac = 0: every cell within tol (the max-norm test); otherwise at most
+    // frac of the cells may still move by more than tol
+    if (frac == 0.0 ? res < tol : (double)above <= frac * (double)d.N) {
       s.converged = 1;
       break;
     }

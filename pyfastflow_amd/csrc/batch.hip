@@ -161,7 +161,7 @@ int batch_after_labels(pff_ctx* ctx, BatchState* bs, const GridD& d, int trip, i
                        (const int64_t*)npits, bs->tmap, bs->cnt_d);
     PFF_TRY(check_hip(hipGetLastError(), "batch_count"));
     PFF_TRY(check_hip(hipMemcpyAsync(bs->cnt_h.data(), bs->cnt_d, (size_t)nt * 4, hipMemcpyDeviceToHost, s), "d2h"));
-    PFF_TRY(check_hip(hipStreamSynchronize(s), "sync"));
+    PFF_TRY(wait_stream(ctx));
   }
   if (trip == 0) {
     // each tile's own cap, lakeflow.py:500 (a tile without depressions

@@ -386,6 +386,9 @@ struct TileArgs {
   float* Q;
   uint32_t* cnt;
   uint8_t* tag;
+  // saddle tags: tiles that receive a marked inflow (the final pass skips
+  // the others, whose tags the local pass already wrote)
+  uint32_t* tin;
   unsigned long long* prof_cells;  // profiling: cells processed by this launch
 };
 
@@ -412,6 +415,9 @@ __global__ __launch_bounds__(kTB) void k_tile(TileArgs a) {
   __shared__ uint16_t E[(PH == 1) ? TC : 1];
   __shared__ uint32_t S[(W != W_ROOTS) ? TC : 1];
   if (!tile_active(a.tact, a.tiles_x, blockIdx.x, blockIdx.y)) return;  // uniform per block
+  if constexpr (PH == 3 && W == W_SADDLE) {
+    if (a.tin && a.tin[blockIdx.y * a.tiles_x + blockIdx.x] == 0u) return;  // uniform per block
+  }
   const GridD& g = a.g;
   const int x0 = blockIdx.x * TS, y0 = blockIdx.y * TS;
   const int tw = min(TS, g.nx - x0), th = min(TS, g.ny - y0);
@@ -499,6 +505,7 @@ __global__ __launch_bounds__(kTB) void k_tile(TileArgs a) {
         a.S[c] = S[li];
         a.inflow[c] = 0u;
       }
+      if constexpr (W == W_SADDLE) a.tag[c] = S[li] ? 1 : 0;  // final unless the tile gets a marked inflow
       if (e == li) {  // local end
         if (rootmask & (1u << j)) {
           if constexpr (W == W_ROOTS) {
@@ -542,8 +549,18 @@ __global__ __launch_bounds__(kTB) void k_tile(TileArgs a) {
 // For labels, a target in an inactive tile drains to an open edge (label 0):
 // the exit becomes a terminal labelled 0.
 __global__ __launch_bounds__(kBT) void k_exit_link(TileArgs a, const int32_t* __restrict__ exits,
-                                                   const int64_t* count, int labels, int32_t* __restrict__ list_out) {
+                                                   const int64_t* count, int labels, int32_t* __restrict__ list_out,
+                                                   int64_t* list_count, int64_t* first_empty, uint32_t* tin,
+                                                   int ntiles) {
   const int64_t n = *count;
+  // set-up of the rounds that follow: list length, first-empty marker, and
+  // the per-tile inflow flags that k_exit_inflow sets afterwards
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (list_count) *list_count = n;
+    if (first_empty) *first_empty = (int64_t)1 << 40;
+  }
+  if (tin)
+    for (int t = blockIdx.x * kBT + threadIdx.x; t < ntiles; t += gridDim.x * kBT) tin[t] = 0u;
   for (int64_t w = (int64_t)blockIdx.x * kBT + threadIdx.x; w < n; w += (int64_t)gridDim.x * kBT) {
     const int x = exits[w];
     const int r = a.rcv[x];
@@ -563,14 +580,23 @@ __global__ __launch_bounds__(kBT) void k_exit_link(TileArgs a, const int32_t* __
   }
 }
 
-// inflow of entry cells: sum of the totals of the exits draining into them
-__global__ __launch_bounds__(kBT) void k_exit_inflow(const int32_t* __restrict__ exits, const int64_t* count,
-                                                     const int32_t* __restrict__ rcv, const uint32_t* __restrict__ S,
+// inflow of entry cells: sum of the totals of the exits draining into them;
+// with tin, flags the tiles that receive a nonzero inflow
+__global__ __launch_bounds__(kBT) void k_exit_inflow(TileArgs a, const int32_t* __restrict__ exits,
+                                                     const int64_t* count, const uint32_t* __restrict__ S,
                                                      uint32_t* inflow) {
   const int64_t n = *count;
   for (int64_t w = (int64_t)blockIdx.x * kBT + threadIdx.x; w < n; w += (int64_t)gridDim.x * kBT) {
     const int x = exits[w];
-    atomicAdd(&inflow[rcv[x]], S[x]);
+    const uint32_t v = S[x];
+    if (v == 0u) continue;
+    const int r = a.rcv[x];
+    atomicAdd(&inflow[r], v);
+    if (a.tin) {
+      const int rr = (int)fdiv((uint32_t)r, a.fd);
+      const int rc = r - rr * a.g.nx;
+      a.tin[(rr / TS) * a.tiles_x + rc / TS] = 1u;
+    }
   }
 }
 
@@ -598,7 +624,7 @@ __global__ __launch_bounds__(kBT) void k_label_tiles(TileArgs a, int32_t* __rest
 
 struct TileBufs {
   int32_t *E, *P2, *bid2, *exits, *L0, *L1, *t0, *t1, *t2, *pits;
-  uint32_t *S, *inflow, *tact;
+  uint32_t *S, *inflow, *tact, *tin;
 };
 
 static int tile_bufs(pff_ctx* ctx, const GridD& g, TileBufs& b) {
@@ -617,7 +643,8 @@ static int tile_bufs(pff_ctx* ctx, const GridD& g, TileBufs& b) {
   PFF_TRY(scratch(ctx, S_TIN, n4, &p)); b.inflow = (uint32_t*)p;
   PFF_TRY(scratch(ctx, S_TPIT, n4, &p)); b.pits = (int32_t*)p;
   const size_t nt = (size_t)((g.nx + TS - 1) / TS) * ((g.ny + TS - 1) / TS);
-  PFF_TRY(scratch(ctx, S_TACT, nt * 4 * 2, &p)); b.tact = (uint32_t*)p;
+  PFF_TRY(scratch(ctx, S_TACT, nt * 4 * 3, &p)); b.tact = (uint32_t*)p;
+  b.tin = b.tact + 2 * nt;
   return 0;
 }
 
@@ -666,8 +693,8 @@ int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, 
   a.tag = tag;
   a.tact = tact;
   hipStream_t s = ctx->stream;
-  PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_NEXIT, 0, 8, s), "memset"));
-  PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_CYCLE, 0, 8, s), "memset"));
+  static_assert(W_NEXIT == W_CYCLE + 1, "cycle flag and exit count are adjacent words");
+  PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_CYCLE, 0, 16, s), "memset"));
   const dim3 tg = tile_grid(g);
   // algorithmic bytes per cell: rcv in; E, S, inflow out (+ bid, snode for saddle weights)
   a.prof_cells = prof_begin(ctx, "k_tile", wmode == W_UNIT ? 16.0 : 24.0);
@@ -681,10 +708,11 @@ int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, 
   L.l[1] = b.L1;
   L.c[0] = ctx->dev_words + W_COUNT0;
   L.c[1] = ctx->dev_words + W_COUNT1;
-  PFF_TRY(check_hip(hipMemcpyAsync(L.c[0], a.n_exits, 8, hipMemcpyDeviceToDevice, s), "copy"));
-  hipLaunchKernelGGL(k_exit_link, dim3(list_grid()), dim3(kBT), 0, s, a, b.exits, a.n_exits, 0, b.L0);
   // saddle tags only ask whether anything upstream is marked: OR rounds
   const bool flags = wmode == W_SADDLE;
+  int64_t* fe = ctx->dev_words + W_EMPTY2;
+  hipLaunchKernelGGL(k_exit_link, dim3(list_grid()), dim3(kBT), 0, s, a, b.exits, a.n_exits, 0, b.L0, L.c[0], fe,
+                     flags ? b.tin : nullptr, tiles_count(g));
   if (rounds > 0 && !cycle) {
     // the caller knows the depth bound (the labels of the same graph)
     PFF_TRY(count_rounds(ctx, L, b.S, b.P2, b.t0, (uint32_t*)b.t1, b.t2, rounds, flags));
@@ -693,8 +721,6 @@ int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, 
     // then confirm on the host that the list has emptied
     const int cap = ceil_log2i(g.N) + 1;
     int& last = flags ? ctx->last_tag_rounds : ctx->last_count_rounds;
-    int64_t* fe = ctx->dev_words + W_EMPTY2;
-    PFF_TRY(launch_fill_i64(ctx, 1, fe, (int64_t)1 << 40));
     int done_rounds = 0;
     int64_t live = 1;
     int step = rounds > 0 ? rounds : std::max(2, std::min(last + 1, cap));
@@ -703,11 +729,8 @@ int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, 
       PFF_TRY(count_rounds(ctx, L, b.S, b.P2, b.t0, (uint32_t*)b.t1, b.t2, r, flags, done_rounds, fe));
       done_rounds += r;
       step = 3;
-      PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_LIVE, L.c[L.cur], 8, hipMemcpyDeviceToHost, s), "d2h"));
-      PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_CYCLE, a.cycle, 8, hipMemcpyDeviceToHost, s), "d2h"));
-      PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_EMPTY2, fe, 8, hipMemcpyDeviceToHost, s), "d2h"));
-      PFF_TRY(check_hip(hipStreamSynchronize(s), "sync"));
-      live = ctx->host_words[W_LIVE];
+      PFF_TRY(read_words(ctx));
+      live = ctx->host_words[L.cur == 0 ? W_COUNT0 : W_COUNT1];
       if (live == 0 || ctx->host_words[W_CYCLE] != 0) break;
     }
     const int64_t f = ctx->host_words[W_EMPTY2];
@@ -716,7 +739,8 @@ int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, 
     if (cycle) *cycle = cyc;
     if (cyc) return cycle ? 0 : fail(PFF_ENOTCONV, "tiled_upstream: receivers are not a forest");
   }
-  hipLaunchKernelGGL(k_exit_inflow, dim3(list_grid()), dim3(kBT), 0, s, b.exits, a.n_exits, rcv, b.S, b.inflow);
+  if (flags) a.tin = b.tin;  // zeroed by k_exit_link
+  hipLaunchKernelGGL(k_exit_inflow, dim3(list_grid()), dim3(kBT), 0, s, a, b.exits, a.n_exits, b.S, b.inflow);
   // rcv, inflow in; Q (or count) out / rcv, bid, snode, inflow in; tag out
   a.prof_cells = prof_begin(ctx, "k_tile", wmode == W_UNIT ? 12.0 : 17.0);
   if (wmode == W_UNIT) hipLaunchKernelGGL((k_tile<0, 3, W_UNIT>), tg, dim3(kTB), 0, s, a);
@@ -753,11 +777,10 @@ int tiled_labels(pff_ctx* ctx, const GridD& g, const int32_t* rec, const uint32_
   L.l[1] = b.L1;
   L.c[0] = ctx->dev_words + W_COUNT0;
   L.c[1] = ctx->dev_words + W_COUNT1;
-  PFF_TRY(check_hip(hipMemcpyAsync(L.c[0], a.n_exits, 8, hipMemcpyDeviceToDevice, s), "copy"));
-  hipLaunchKernelGGL(k_exit_link, dim3(list_grid()), dim3(kBT), 0, s, a, b.exits, a.n_exits, 1, b.L0);
+  hipLaunchKernelGGL(k_exit_link, dim3(list_grid()), dim3(kBT), 0, s, a, b.exits, a.n_exits, 1, b.L0, L.c[0],
+                     ctx->dev_words + W_EMPTY, nullptr, 0);
   // start from the previous call's round count, then confirm on the host
   const int cap = ceil_log2i(g.N) + 1;
-  PFF_TRY(launch_fill_i64(ctx, 1, ctx->dev_words + W_EMPTY, (int64_t)1 << 40));
   int done_rounds = 0;
   int64_t live = 1;
   int step = std::max(2, std::min(ctx->last_label_rounds + 1, cap));
@@ -767,13 +790,8 @@ int tiled_labels(pff_ctx* ctx, const GridD& g, const int32_t* rec, const uint32_
     done_rounds += r;
     step = 3;
     // one readback: live count (a cycle across tiles), in-tile cycle flag, pits
-    PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_LIVE, L.c[L.cur], 8, hipMemcpyDeviceToHost, s), "d2h"));
-    PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_CYCLE, a.cycle, 8, hipMemcpyDeviceToHost, s), "d2h"));
-    PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_NPIT, a.n_pits, 8, hipMemcpyDeviceToHost, s), "d2h"));
-    PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_EMPTY, ctx->dev_words + W_EMPTY, 8, hipMemcpyDeviceToHost, s),
-                      "d2h"));
-    PFF_TRY(check_hip(hipStreamSynchronize(s), "sync"));
-    live = ctx->host_words[W_LIVE];
+    PFF_TRY(read_words(ctx));
+    live = ctx->host_words[L.cur == 0 ? W_COUNT0 : W_COUNT1];
     if (live == 0 || ctx->host_words[W_CYCLE] != 0) break;
   }
   // rounds that actually had work (the first empty round index, or all)
@@ -880,7 +898,9 @@ __global__ __launch_bounds__(kBT) void k_basin_jump(const int32_t* __restrict__ 
 // bid <- M[bid] on the tiles of tact_in; flags the tiles with a closed basin
 __global__ __launch_bounds__(kBT) void k_relabel_tiles(GridD g, int tiles_x, const uint32_t* __restrict__ tact_in,
                                                        const int32_t* __restrict__ M, int32_t* __restrict__ bid,
-                                                       uint32_t* __restrict__ tact_out) {
+                                                       uint32_t* __restrict__ tact_out, int64_t* npit,
+                                                       const int64_t* n_next) {
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) *npit = *n_next;
   int bx, by;
   xcd_block(bx, by);
   const int t = (by >> 4) * tiles_x + bx;
@@ -917,8 +937,8 @@ int relabel_from_outlets(pff_ctx* ctx, const GridD& g, int32_t* bid, const int64
   int64_t* n_next = ctx->dev_words + W_NPIT2;
   int64_t* chg = ctx->dev_words + W_MCHG;
   *cycle = 0;
-  PFF_TRY(check_hip(hipMemsetAsync(n_next, 0, 8, s), "memset"));
-  PFF_TRY(check_hip(hipMemsetAsync(chg, 0, 8, s), "memset"));
+  static_assert(W_MCHG == W_NPIT2 + 1, "next pit count and change round are adjacent words");
+  PFF_TRY(check_hip(hipMemsetAsync(n_next, 0, 16, s), "memset"));
   const unsigned lg = grid_for(std::max<int64_t>(n_prev, 1), kBT, 4096);
   hipLaunchKernelGGL(k_basin_next, dim3(lg), dim3(kBT), 0, s, (const int32_t*)b.pits, (const int64_t*)npit, outlet,
                      (const int32_t*)bid, g.N, M[0], M[1], next_pits, n_next);
@@ -935,9 +955,7 @@ int relabel_from_outlets(pff_ctx* ctx, const GridD& g, int32_t* bid, const int64
     }
     done += step;
     PFF_TRY(check_hip(hipGetLastError(), "basin_jump"));
-    PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_MCHG, chg, 8, hipMemcpyDeviceToHost, s), "d2h"));
-    PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_NPIT2, n_next, 8, hipMemcpyDeviceToHost, s), "d2h"));
-    PFF_TRY(check_hip(hipStreamSynchronize(s), "sync"));
+    PFF_TRY(read_words(ctx));
     // converged when the last round changed nothing
     if (ctx->host_words[W_MCHG] < done) {
       ctx->last_jump_rounds = std::max<int>((int)ctx->host_words[W_MCHG] + 1, 1);
@@ -952,12 +970,12 @@ int relabel_from_outlets(pff_ctx* ctx, const GridD& g, int32_t* bid, const int64
   const int64_t n2 = ctx->host_words[W_NPIT2];
   PFF_TRY(check_hip(hipMemsetAsync(tact_out, 0, (size_t)tiles_count(g) * 4, s), "memset"));
   hipLaunchKernelGGL(k_relabel_tiles, dim3((g.nx + 63) / 64, (g.ny + 3) / 4), dim3(64, 4), 0, s, g,
-                     (g.nx + TS - 1) / TS, tact_in, (const int32_t*)M[cur], bid, tact_out);
+                     (g.nx + TS - 1) / TS, tact_in, (const int32_t*)M[cur], bid, tact_out, npit,
+                     (const int64_t*)n_next);
   PFF_TRY(check_hip(hipGetLastError(), "relabel_tiles"));
-  // the uncarved basins become the pits list
-  if (n2 > 0)
-    PFF_TRY(check_hip(hipMemcpyAsync(b.pits, next_pits, (size_t)n2 * 4, hipMemcpyDeviceToDevice, s), "copy"));
-  PFF_TRY(check_hip(hipMemcpyAsync(npit, n_next, 8, hipMemcpyDeviceToDevice, s), "copy"));
+  // the uncarved basins become the pits list: swap the two list buffers
+  std::swap(ctx->buf[S_TPIT], ctx->buf[S_TPIT2]);
+  std::swap(ctx->cap[S_TPIT], ctx->cap[S_TPIT2]);
   *ndep = n2;
   return 0;
 }

@@ -221,10 +221,13 @@ int launch_manning(pff_ctx* ctx, const GridD& g, const pff_hydro* p, float* h, f
 // max |a - b| over N cells (both finite) and the number of NaN cells of a;
 // out[0] holds the max as float bits (non-negative floats order as integers),
 // out[1] the NaN count
+// max |a - b| over non-NaN cells, NaN count, and (with out_above) the number
+// of cells with |a - b| > tol or NaN
 __global__ __launch_bounds__(256) void k_maxdiff(int32_t N, const float* __restrict__ a, const float* __restrict__ b,
-                                                 uint32_t* out_max, unsigned long long* out_nan) {
+                                                 uint32_t* out_max, unsigned long long* out_nan, float tol,
+                                                 unsigned long long* out_above) {
   float m = 0.0f;
-  unsigned nan = 0;
+  unsigned nan = 0, above = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
     const float x = a[i];
     if (x != x) {
@@ -233,20 +236,23 @@ __global__ __launch_bounds__(256) void k_maxdiff(int32_t N, const float* __restr
     }
     const float d = fabsf(x - b[i]);
     m = d > m ? d : m;
+    above += d > tol ? 1u : 0u;
   }
   for (int o = 32; o > 0; o >>= 1) {
     m = fmaxf(m, __shfl_down(m, o));
     nan += __shfl_down(nan, o);
+    above += __shfl_down(above, o);
   }
   if ((threadIdx.x & 63) == 0) {
     atomicMax(out_max, __float_as_uint(m));
     if (nan) atomicAdd(out_nan, (unsigned long long)nan);
+    if (out_above && (above + nan)) atomicAdd(out_above, (unsigned long long)(above + nan));
   }
 }
 
 int launch_maxdiff(pff_ctx* ctx, int32_t N, const float* a, const float* b, uint32_t* out_max,
-                   unsigned long long* out_nan) {
-  hipLaunchKernelGGL(k_maxdiff, dim3(1024), dim3(256), 0, ctx->stream, N, a, b, out_max, out_nan);
+                   unsigned long long* out_nan, float tol, unsigned long long* out_above) {
+  hipLaunchKernelGGL(k_maxdiff, dim3(1024), dim3(256), 0, ctx->stream, N, a, b, out_max, out_nan, tol, out_above);
   return check_hip(hipGetLastError(), "maxdiff");
 }
 

@@ -613,6 +613,45 @@ __global__ __launch_bounds__(kBT) void k_cycle_apply_list(int32_t N, const int32
   }
 }
 
+// lake-outlet log: one entry per basin of the trip (its pit holds its own id)
+__global__ __launch_bounds__(kBT) void k_log_outlets(int32_t N, const int32_t* __restrict__ bid,
+                                                     const int64_t* __restrict__ outlet,
+                                                     const int32_t* __restrict__ snode, int32_t* __restrict__ lb,
+                                                     int64_t* __restrict__ lo, int32_t* __restrict__ ls,
+                                                     int64_t capacity, int64_t* count) {
+  for (int64_t base = (int64_t)blockIdx.x * kBT; base < N; base += (int64_t)gridDim.x * kBT) {
+    const int64_t i = base + threadIdx.x;
+    int32_t v[1] = {0};
+    bool take[1] = {false};
+    if (i < N && bid[i] == (int32_t)i + 1) {
+      v[0] = (int32_t)i + 1;
+      take[0] = true;
+    }
+    // positions past the capacity are counted but not written
+    int total;
+    const int off = block_exclusive_scan<kBT>(take[0] ? 1 : 0, &total);
+    __shared__ int64_t base_s;
+    if (threadIdx.x == 0) base_s = total ? (int64_t)atomicAdd((unsigned long long*)count, (unsigned long long)total) : 0;
+    __syncthreads();
+    const int64_t p = base_s + off;
+    if (take[0] && p < capacity) {
+      const int b = v[0];
+      lb[p] = b;
+      lo[p] = b < N ? outlet[b] : INVALID_KEY;
+      ls[p] = b < N ? snode[b] : -1;
+    }
+    __syncthreads();
+  }
+}
+
+int launch_log_outlets(pff_ctx* ctx, const GridD& g, const int32_t* bid, const int64_t* outlet,
+                       const int32_t* snode, int64_t* dev_count) {
+  const auto& L = ctx->lake_log;
+  hipLaunchKernelGGL(k_log_outlets, dim3(grid_for(g.N, kBT, 8192)), dim3(kBT), 0, ctx->stream, g.N, bid, outlet, snode,
+                     L.basin, L.outlet, L.snode, L.capacity, dev_count);
+  return check_hip(hipGetLastError(), "log_outlets");
+}
+
 int saddlesort_active(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t* isb, float* zp, int64_t* saddle,
                       int32_t* snode, int64_t* outlet, const float* z, const uint32_t* tact, const int32_t* pits,
                       const int64_t* npits) {

@@ -35,6 +35,18 @@ struct pff_ctx {
     hipEvent_t start, stop;
     int slot;
   };
+  // armed lake-outlet log for the next reroute_flow call (pff_ctx_lake_log)
+  struct LakeLog {
+    int32_t* basin = nullptr;
+    int64_t* outlet = nullptr;
+    int32_t* snode = nullptr;
+    int64_t capacity = 0;
+    int64_t* trip_counts = nullptr;  // host: entries logged up to the end of each trip
+    int32_t max_trips = 0;
+    int64_t* dev_count = nullptr;    // device running count (8 bytes, owned by the context)
+    bool armed = false;
+  } lake_log;
+  hipEvent_t sync_ev = nullptr;  // spin-wait event of wait_stream
   bool prof_on = false;
   std::vector<ProfRec> prof;
   unsigned long long* prof_cells = nullptr;  // device, kProfSlots counters
@@ -51,6 +63,11 @@ int check_grid(const pff_grid* g, GridD* out);
 // (null when profiling is off); call prof_end after the launch
 unsigned long long* prof_begin(pff_ctx* ctx, const char* name, double bytes_per_cell);
 void prof_end(pff_ctx* ctx);
+// wait for the context stream by polling an event (a short spin instead of
+// the runtime's blocking wait; every lakeflow trip waits once or twice)
+int wait_stream(pff_ctx* ctx);
+// copy the device words to the pinned host words and wait
+int read_words(pff_ctx* ctx);
 // device scratch slot of at least `bytes`
 int scratch(pff_ctx* ctx, int slot, size_t bytes, void** out, bool zero_new = false);
 
@@ -72,7 +89,7 @@ enum Slot : int {
 };
 
 // device word indices (dev_words / host_words)
-enum Word : int { W_COUNT0 = 0, W_COUNT1, W_COUNT2, W_FLAG, W_FLAG2, W_CYCLE, W_NEXIT, W_NDEP, W_NPIT, W_LIVE, W_EMPTY, W_GFMAX, W_GFNAN, W_NPIT2, W_MCHG, W_EMPTY2, W_NWORDS = 16 };
+enum Word : int { W_COUNT0 = 0, W_COUNT1, W_COUNT2, W_FLAG, W_FLAG2, W_CYCLE, W_NEXIT, W_NDEP, W_NPIT, W_LIVE, W_EMPTY, W_GFMAX, W_GFNAN, W_NPIT2, W_MCHG, W_EMPTY2, W_GFABOVE, W_NWORDS = 24 };
 
 constexpr int kBlock = 256;
 inline unsigned grid_for(int64_t n, int block = kBlock, int64_t cap = 1 << 20) {
@@ -131,6 +148,10 @@ int saddlesort_active(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t*
                       int32_t* snode, int64_t* outlet, const float* z, const uint32_t* tact, const int32_t* pits,
                       const int64_t* npits);
 int launch_reroute_jump(pff_ctx* ctx, const GridD& g, int32_t* rec, const int64_t* outlet);
+// append (b, outlet[b], saddlenode[b]) for every basin b of the current labels
+// (cells with bid[i] == i + 1) to the armed lake log; *dev_count counts them
+int launch_log_outlets(pff_ctx* ctx, const GridD& g, const int32_t* bid, const int64_t* outlet,
+                       const int32_t* snode, int64_t* dev_count);
 // reference-shaped carve (lakeflow.py:405-438)
 int reroute_carve(pff_ctx* ctx, const GridD& g, int32_t* rec, int32_t* rec_, int32_t* rec__, uint8_t* tag,
                   uint8_t* tag_, const int32_t* snode, const int64_t* outlet, int32_t* iters);
@@ -191,7 +212,7 @@ int launch_pack_input(pff_ctx* ctx, int64_t npix, int nplanes, int C, const floa
 
 int launch_edge_maps(pff_ctx* ctx, const GridD& g, uint8_t* edges, uint8_t* out);
 int launch_maxdiff(pff_ctx* ctx, int32_t N, const float* a, const float* b, uint32_t* out_max,
-                   unsigned long long* out_nan);
+                   unsigned long long* out_nan, float tol = 0.f, unsigned long long* out_above = nullptr);
 int launch_pack(pff_ctx* ctx, int64_t n, int64_t* arr, const float* ft, const int32_t* it);
 int launch_unpack(pff_ctx* ctx, int64_t n, const int64_t* arr, float* ft, int32_t* it);
 int launch_swap(pff_ctx* ctx, int64_t n, uint32_t* a, uint32_t* b);

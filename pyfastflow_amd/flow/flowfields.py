@@ -121,19 +121,44 @@ class FlowRouter:
         _lib.call("pff_compute_receivers", self._ctx(), self._grid(), _lib.ptr(self.z), _lib.ptr(self.receivers),
                   _lib.ptr(self.gradient))
 
-    def reroute_flow(self, carve=True, bid_first=None):
+    def reroute_flow(self, carve=True, bid_first=None, log_outlets=False):
         """Route closed depressions through their spill saddles
-        (flowfields.py:228-252, lakeflow.py:466-544)."""
+        (flowfields.py:228-252, lakeflow.py:466-544).
+
+        log_outlets=True also records, for every trip, each basin's outlet
+        key and saddle node after the saddle passes (the state the carve
+        uses); they are left in self.last_lake_outlets as a list of
+        (basin ids, outlet keys, saddle nodes) numpy arrays, one per trip."""
         if not self.lakeflow:
             raise RuntimeError("Flow field not compiled for lakeflow")
         st = _lib.PffLakeStatus()
         bf = bid_first.data if isinstance(bid_first, Field) else bid_first
+        log = None
+        if log_outlets:
+            N = self.nx * self.ny
+            log = (torch.empty(N, dtype=torch.int32, device=self.device),
+                   torch.empty(N, dtype=torch.int64, device=self.device),
+                   torch.empty(N, dtype=torch.int32, device=self.device),
+                   (ctypes.c_int64 * 64)())
+            _lib.call("pff_ctx_lake_log", self._ctx(), _lib.ptr(log[0]), _lib.ptr(log[1]), _lib.ptr(log[2]), N,
+                      log[3], 64)
         _lib.call("pff_reroute_flow", self._ctx(), self._grid(), _lib.ptr(self.bid), _lib.ptr(self.receivers),
                   _lib.ptr(self.receivers_), _lib.ptr(self.receivers__), _lib.ptr(self.z), _lib.ptr(self.z_),
                   _lib.ptr(self.is_border), _lib.ptr(self.outlet), _lib.ptr(self.basin_saddle),
                   _lib.ptr(self.basin_saddlenode), _lib.ptr(self.tag), _lib.ptr(self.tag_), 1 if carve else 0,
                   _lib.ptr(bf), ctypes.byref(st))
         self.last_lake_status = st
+        if log is not None:
+            b, o, sn = (t.cpu().numpy() for t in log[:3])
+            trips, prev = [], 0
+            for k in range(64):
+                c = int(log[3][k])
+                if c < 0:
+                    break
+                c = min(c, b.size)
+                trips.append((b[prev:c].copy(), o[prev:c].copy(), sn[prev:c].copy()))
+                prev = c
+            self.last_lake_outlets = trips
         return st
 
     def accumulate_constant_Q(self, value, area=True):

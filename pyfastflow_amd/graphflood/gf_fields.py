@@ -82,15 +82,17 @@ class Flooder:
         self.iterations += int(N)
         return st
 
-    def run_to_steady(self, tol=1e-6, check_every=100, max_iter=1_000_000, n_diffuse=5):
-        """Iterate until max|h(t) - h(t - check_every)| < tol, checked every
-        `check_every` iterations on the GPU, or max_iter iterations.  Returns
-        the PffGfStatus (iterations, residual, converged, NaN cells)."""
+    def run_to_steady(self, tol=1e-6, check_every=100, max_iter=1_000_000, n_diffuse=5, frac=0.0):
+        """Iterate until steady, checked every `check_every` iterations on the
+        GPU on d = |h(t) - h(t - check_every)|: frac=0 asks max d < tol; with
+        0 < frac < 1 at most that fraction of cells may have d > tol.  Stops
+        at max_iter.  Returns the PffGfStatus (iterations, residual (max d),
+        above_frac, converged, NaN cells, and the first check's values)."""
         r = self.router
         st = _lib.PffGfStatus()
         _lib.call("pff_graphflood_to_steady", r._ctx(), r._grid(), self._hydro(), _lib.ptr(r.z), _lib.ptr(self.h),
-                  _lib.ptr(r.Q), _lib.ptr(r.receivers), float(tol), int(check_every), int(max_iter), int(n_diffuse),
-                  1e-3, ctypes.byref(st))
+                  _lib.ptr(r.Q), _lib.ptr(r.receivers), float(tol), float(frac), int(check_every), int(max_iter),
+                  int(n_diffuse), 1e-3, ctypes.byref(st))
         self.iterations += int(st.iterations)
         return st
 

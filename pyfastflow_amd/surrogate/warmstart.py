@@ -140,12 +140,14 @@ def predict_depth(model: UNet, router: FlowRouter, rain: float, manning: float) 
     return depth_of(model(planes))[0]
 
 
-def solve(z: np.ndarray, rain: float, manning: float, model: UNet | None, tol: float, check_every: int = 100,
-          max_iter: int = 200000, update: str = "single", log=print):
-    """GraphFlood to max|h(t) - h(t - check_every)| < tol from a cold start,
-    and from the model's warm start (if a model is given).  The warm start
-    is kept only if its first residual is not worse than the cold start's.
-    Returns a dict with iterations, wall times and residuals of both."""
+def solve(z: np.ndarray, rain: float, manning: float, model: UNet | None, tol: float, frac: float = 0.0,
+          check_every: int = 100, max_iter: int = 200000, update: str = "single", log=print):
+    """GraphFlood to steady (Flooder.run_to_steady with tol and frac) from a
+    cold start, and from the model's warm start (if a model is given).  The
+    warm start is kept only if its first residual (the fraction of cells
+    still moving by more than tol, or the max change when frac is 0) is not
+    worse than the cold start's.  Returns a dict with iterations, wall times
+    (inference included) and residuals of both."""
     ny, nx = z.shape
     out = {}
     for mode in (["cold", "warm"] if model is not None else ["cold"]):
@@ -158,14 +160,16 @@ def solve(z: np.ndarray, rain: float, manning: float, model: UNet | None, tol: f
         if mode == "warm":
             h0 = predict_depth(model, r, rain, manning)
         _reset_start(r, fl, z, h0, rain)
-        st = fl.run_to_steady(tol=tol, check_every=check_every, max_iter=max_iter)
+        st = fl.run_to_steady(tol=tol, check_every=check_every, max_iter=max_iter, frac=frac)
         torch.cuda.synchronize()
         out[mode] = {"iterations": int(st.iterations), "seconds": time.perf_counter() - t0,
                      "residual": float(st.residual), "first_residual": float(st.first_residual),
+                     "above_frac": float(st.above_frac), "first_above_frac": float(st.first_above_frac),
                      "converged": bool(st.converged), "nan_cells": int(st.nan_cells)}
         log(f"[surrogate] {mode}: {out[mode]}")
         del fl, r
-    if "warm" in out and out["warm"]["first_residual"] > out["cold"]["first_residual"]:
+    key = "first_above_frac" if frac > 0 else "first_residual"
+    if "warm" in out and out["warm"][key] > out["cold"][key]:
         out["used"] = "cold (warm start's first residual was worse)"
     else:
         out["used"] = "warm" if "warm" in out else "cold"

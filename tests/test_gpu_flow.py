@@ -226,3 +226,45 @@ def test_reroute_sentinel_and_capped_dems(pf, ora, seed):
     np.testing.assert_array_equal(r.receivers.to_numpy(), lk["rec"])
     st = r.last_lake_status
     assert [int(st.ndep[i]) for i in range(st.n_loop)] == lk["ndep"]
+
+
+INVALID_KEY = 0x4CBEBC200000002A
+
+
+@pytest.mark.parametrize("case", [
+    (64, 64, "periodic_EW", 4, 6),
+    (128, 96, "normal", 10, 8),
+    (256, 256, "periodic_EW", 40, 9),
+    (200, 300, "periodic_NS", 30, 12),
+    "sentinel68", "sentinel447", "capped157",
+])
+def test_lake_outlets_per_trip(pf, ora, case):
+    """Lake outlets bit-exact per trip: after each trip's saddle and cycle
+    passes, every basin's packed outlet key (elevation, node) and saddle
+    node equal the oracle's (lakeflow.py:105-281, 8.P)."""
+    from oracle_util import sentinel_dem
+
+    if isinstance(case, str):
+        seed = int("".join(ch for ch in case if ch.isdigit()))
+        z = sentinel_dem(seed)
+        ny, nx = z.shape
+        mode = "normal"
+    else:
+        ny, nx, mode, pits, seed = case
+        z = _dem(ny, nx, seed, pits, mode)
+    print(f"case lake outlets {case}", flush=True)
+    r = _router(pf, nx, ny, 10.0, mode, z)
+    r.compute_receivers()
+    r.reroute_flow(carve=True, log_outlets=True)
+    g = _ograd(ora, nx, ny, 10.0, mode)
+    rcv, _ = ora.compute_receivers(g, z)
+    lk = ora.reroute_flow(g, z, rcv, carve=True, log_iters=64)
+    trips = r.last_lake_outlets
+    assert len(trips) == lk["n_iter"]
+    for t, (b, o, sn) in enumerate(trips):
+        ol, snl = lk["outlet_log"][t], lk["saddlenode_log"][t]
+        np.testing.assert_array_equal(o, ol[b], err_msg=f"trip {t}: outlet keys")
+        np.testing.assert_array_equal(sn, snl[b], err_msg=f"trip {t}: saddle nodes")
+        rest = np.ones(ol.size, bool)
+        rest[b] = False
+        assert (ol[rest] == INVALID_KEY).all() and (snl[rest] == -1).all(), f"trip {t}: basins missing from the log"
