@@ -175,6 +175,8 @@ typedef struct {
   int64_t nan_cells;       /* cells whose h is NaN at the end (h < 0 makes h^(5/3) NaN) */
   double above_frac;       /* fraction of cells with |h(t) - h(t - check_every)| > tol (or NaN), last check */
   double first_above_frac; /* the same at the first check */
+  double mean_change;      /* mean |h(t) - h(t - check_every)| over non-NaN cells (/N), last check, m */
+  double first_mean_change;/* the same at the first check */
 } pff_gf_status;
 
 /* gf_fields.py:111-141  Flooder.run_graphflood(N): n_iter iterations of

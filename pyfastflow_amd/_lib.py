@@ -51,6 +51,8 @@ class PffGfStatus(ctypes.Structure):
         ("nan_cells", c_i64),
         ("above_frac", c_d),
         ("first_above_frac", c_d),
+        ("mean_change", c_d),
+        ("first_mean_change", c_d),
     ]
 
 

@@ -681,14 +681,19 @@ int gf_iteration(pff_ctx* ctx, const pff_grid* g, const GridD& d, const pff_hydr
 }
 
 int gf_check(pff_ctx* ctx, const GridD& d, const float* h, const float* hprev, double tol, double* res,
-             int64_t* nans, int64_t* above) {
+             int64_t* nans, int64_t* above, double* mean) {
   uint32_t* mx = (uint32_t*)(ctx->dev_words + W_GFMAX);
   unsigned long long* nn = (unsigned long long*)(ctx->dev_words + W_GFNAN);
   unsigned long long* ab = (unsigned long long*)(ctx->dev_words + W_GFABOVE);
+  double* sm = (double*)(ctx->dev_words + W_GFSUM);
+  static_assert(W_GFNAN == W_GFMAX + 1 && W_GFSUM == W_GFABOVE + 1, "residual words are pairs");
   PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_GFMAX, 0, 16, ctx->stream), "memset"));
-  PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_GFABOVE, 0, 8, ctx->stream), "memset"));
-  PFF_TRY(launch_maxdiff(ctx, d.N, h, hprev, mx, nn, (float)tol, ab));
+  PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_GFABOVE, 0, 16, ctx->stream), "memset"));
+  PFF_TRY(launch_maxdiff(ctx, d.N, h, hprev, mx, nn, (float)tol, ab, sm));
   PFF_TRY(read_words(ctx));
+  double sumv;
+  std::memcpy(&sumv, &ctx->host_words[W_GFSUM], 8);
+  *mean = sumv / (double)d.N;
   uint32_t bits = (uint32_t)(ctx->host_words[W_GFMAX] & 0xFFFFFFFFu);
   float m;
   std::memcpy(&m, &bits, 4);
@@ -736,6 +741,8 @@ int pff_graphflood_to_steady(pff_ctx* ctx, const pff_grid* g, const pff_hydro* p
   s.first_residual = -1.0;
   s.above_frac = -1.0;
   s.first_above_frac = -1.0;
+  s.mean_change = -1.0;
+  s.first_mean_change = -1.0;
   const size_t n4 = (size_t)d.N * 4;
   PFF_TRY(check_hip(hipMemcpyAsync(f.hprev, h, n4, hipMemcpyDeviceToDevice, ctx->stream), "copy"));
   while (s.iterations < max_iter) {
@@ -745,14 +752,17 @@ int pff_graphflood_to_steady(pff_ctx* ctx, const pff_grid* g, const pff_hydro* p
     if (n < check_every) break;  // a partial block cannot be judged
     double res = 0;
     int64_t nans = 0, above = 0;
-    PFF_TRY(gf_check(ctx, d, h, f.hprev, tol, &res, &nans, &above));
+    double mean = 0;
+    PFF_TRY(gf_check(ctx, d, h, f.hprev, tol, &res, &nans, &above, &mean));
     s.checks++;
     s.residual = res;
     s.nan_cells = nans;
     s.above_frac = (double)above / (double)d.N;
+    s.mean_change = mean;
     if (s.first_residual < 0) {
       s.first_residual = res;
       s.first_above_frac = s.above_frac;
+      s.first_mean_change = mean;
     }
     if (nans > 0) break;
     // frac = 0: every cell within tol (the max-norm test); otherwise at most

@@ -354,8 +354,8 @@ int npass_for(int32_t N) { return ceil_log2i(N) + 1; }
 // ---------------------------------------------------------------------------
 constexpr int TS = 64;               // tile side (cells)
 constexpr int TC = TS * TS;          // 4096 cells per tile
-constexpr int kTB = 1024;            // threads per tile block
-constexpr int CPT = TC / kTB;        // 4 cells per thread
+constexpr int kTB = 512;             // threads per tile block
+constexpr int CPT = TC / kTB;        // 8 cells per thread
 constexpr uint16_t NONE = 0xFFFF;
 constexpr int kTileRounds = 12;      // 2^12 = TC: every acyclic in-tile path is contracted
 
@@ -424,6 +424,7 @@ __global__ __launch_bounds__(kTB) void k_tile(TileArgs a) {
   if (a.prof_cells && threadIdx.x == 0) atomicAdd(a.prof_cells, (unsigned long long)(tw * th));
   int32_t cell[CPT];
   uint32_t rootmask = 0;
+  int wany = 0;
 #pragma unroll
   for (int j = 0; j < CPT; ++j) {
     const int li = threadIdx.x + j * kTB;
@@ -450,9 +451,16 @@ __global__ __launch_bounds__(kTB) void k_tile(TileArgs a) {
     }
     P[li] = lp;
     if constexpr (PH == 1) E[li] = (lp == NONE) ? (uint16_t)li : lp;
-    if constexpr (W != W_ROOTS) S[li] = w;
+    if constexpr (W != W_ROOTS) {
+      S[li] = w;
+      wany |= w != 0u;
+    }
   }
-  __syncthreads();
+  // saddle weights are sparse: a tile without any nonzero weight keeps S = 0
+  // and skips the sum part of the rounds (uniform per block)
+  bool sums = W == W_UNIT;
+  if constexpr (W == W_SADDLE) sums = __syncthreads_or(wany) != 0;
+  else __syncthreads();
   bool cyc = true;
   for (int k = 0; k < kTileRounds; ++k) {
     uint16_t pm[CPT], pp[CPT], ee[CPT];
@@ -465,7 +473,9 @@ __global__ __launch_bounds__(kTB) void k_tile(TileArgs a) {
       sv[j] = 0;
       if (pm[j] != NONE) {
         pp[j] = P[pm[j]];
-        if constexpr (W != W_ROOTS) sv[j] = S[li];
+        if constexpr (W != W_ROOTS) {
+          if (sums) sv[j] = S[li];
+        }
       }
       if constexpr (PH == 1) ee[j] = E[E[li]];
     }
@@ -475,7 +485,9 @@ __global__ __launch_bounds__(kTB) void k_tile(TileArgs a) {
     for (int j = 0; j < CPT; ++j) {
       const int li = threadIdx.x + j * kTB;
       if (pm[j] != NONE) {
-        if constexpr (W != W_ROOTS) atomicAdd(&S[pm[j]], sv[j]);
+        if constexpr (W != W_ROOTS) {
+          if (sums) atomicAdd(&S[pm[j]], sv[j]);
+        }
         P[li] = pp[j];
         any |= (pp[j] != NONE);
       }
@@ -901,22 +913,25 @@ __global__ __launch_bounds__(kBT) void k_relabel_tiles(GridD g, int tiles_x, con
                                                        uint32_t* __restrict__ tact_out, int64_t* npit,
                                                        const int64_t* n_next) {
   if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) *npit = *n_next;
+  // 64 columns x 16 rows per block, four rows per thread
   int bx, by;
   xcd_block(bx, by);
-  const int t = (by >> 4) * tiles_x + bx;
+  const int t = ((by * 16) / TS) * tiles_x + bx;
   if (tact_in && tact_in[t] == 0u) return;  // uniform per block
   const int col = bx * 64 + threadIdx.x;
-  const int row = by * 4 + threadIdx.y;
   int any = 0;
-  if (col < g.nx && row < g.ny) {
-    const int c = row * g.nx + col;
-    const int b = bid[c];
-    if (b != 0) {
-      const int nb = (b > 0 && b < g.N) ? M[b] : 0;
-      if (nb != b) bid[c] = nb;
-      any = nb != 0;
+  if (col < g.nx)
+    for (int rr = threadIdx.y; rr < 16; rr += 4) {
+      const int row = by * 16 + rr;
+      if (row >= g.ny) break;
+      const int c = row * g.nx + col;
+      const int b = bid[c];
+      if (b != 0) {
+        const int nb = (b > 0 && b < g.N) ? M[b] : 0;
+        if (nb != b) bid[c] = nb;
+        any |= nb != 0;
+      }
     }
-  }
   any = __syncthreads_or(any);
   if (any && threadIdx.x == 0 && threadIdx.y == 0) atomicOr(&tact_out[t], 1u);
 }
@@ -969,7 +984,7 @@ int relabel_from_outlets(pff_ctx* ctx, const GridD& g, int32_t* bid, const int64
   }
   const int64_t n2 = ctx->host_words[W_NPIT2];
   PFF_TRY(check_hip(hipMemsetAsync(tact_out, 0, (size_t)tiles_count(g) * 4, s), "memset"));
-  hipLaunchKernelGGL(k_relabel_tiles, dim3((g.nx + 63) / 64, (g.ny + 3) / 4), dim3(64, 4), 0, s, g,
+  hipLaunchKernelGGL(k_relabel_tiles, dim3((g.nx + 63) / 64, (g.ny + 15) / 16), dim3(64, 4), 0, s, g,
                      (g.nx + TS - 1) / TS, tact_in, (const int32_t*)M[cur], bid, tact_out, npit,
                      (const int64_t*)n_next);
   PFF_TRY(check_hip(hipGetLastError(), "relabel_tiles"));

@@ -225,8 +225,9 @@ int launch_manning(pff_ctx* ctx, const GridD& g, const pff_hydro* p, float* h, f
 // of cells with |a - b| > tol or NaN
 __global__ __launch_bounds__(256) void k_maxdiff(int32_t N, const float* __restrict__ a, const float* __restrict__ b,
                                                  uint32_t* out_max, unsigned long long* out_nan, float tol,
-                                                 unsigned long long* out_above) {
+                                                 unsigned long long* out_above, double* out_sum) {
   float m = 0.0f;
+  double sum = 0.0;
   unsigned nan = 0, above = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
     const float x = a[i];
@@ -237,22 +238,26 @@ __global__ __launch_bounds__(256) void k_maxdiff(int32_t N, const float* __restr
     const float d = fabsf(x - b[i]);
     m = d > m ? d : m;
     above += d > tol ? 1u : 0u;
+    sum += (double)d;
   }
   for (int o = 32; o > 0; o >>= 1) {
     m = fmaxf(m, __shfl_down(m, o));
     nan += __shfl_down(nan, o);
     above += __shfl_down(above, o);
+    sum += __shfl_down(sum, o);
   }
   if ((threadIdx.x & 63) == 0) {
     atomicMax(out_max, __float_as_uint(m));
     if (nan) atomicAdd(out_nan, (unsigned long long)nan);
     if (out_above && (above + nan)) atomicAdd(out_above, (unsigned long long)(above + nan));
+    if (out_sum) atomicAdd(out_sum, sum);
   }
 }
 
 int launch_maxdiff(pff_ctx* ctx, int32_t N, const float* a, const float* b, uint32_t* out_max,
-                   unsigned long long* out_nan, float tol, unsigned long long* out_above) {
-  hipLaunchKernelGGL(k_maxdiff, dim3(1024), dim3(256), 0, ctx->stream, N, a, b, out_max, out_nan, tol, out_above);
+                   unsigned long long* out_nan, float tol, unsigned long long* out_above, double* out_sum) {
+  hipLaunchKernelGGL(k_maxdiff, dim3(1024), dim3(256), 0, ctx->stream, N, a, b, out_max, out_nan, tol, out_above,
+                     out_sum);
   return check_hip(hipGetLastError(), "maxdiff");
 }
 

@@ -421,18 +421,24 @@ int carve_from_levels(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t*
 // ---------------------------------------------------------------------------
 constexpr int TSL = 64;  // tile side, as in the tiled contraction
 
-// One cell per lane: 64x4 blocks, 16 blocks per 64x64 tile; a block whose
-// tile is inactive returns at once.
-#define PFF_CELL_PROLOGUE(tact, tiles_x)                                       \
+// 64x4 threads per block covering 64 columns x kCellRows rows (each thread
+// takes every fourth row), 4 blocks per 64x64 tile; a block whose tile is
+// inactive returns at once.  The loop body runs once per cell of the block;
+// `continue` leaves a cell.
+constexpr int kCellRows = 16;
+#define PFF_CELL_LOOP_BEGIN(tact, tiles_x)                                     \
   int bx_, by_;                                                                \
   xcd_block(bx_, by_);                                                         \
-  if (tact && tact[(by_ >> 4) * (tiles_x) + bx_] == 0u) return;                \
+  if (tact && tact[((by_ * kCellRows) / TSL) * (tiles_x) + bx_] == 0u) return; \
   const int col = bx_ * 64 + threadIdx.x;                                      \
-  const int row = by_ * 4 + threadIdx.y;                                       \
-  if (col >= g.nx || row >= g.ny) return;                                      \
-  const int i = row * g.nx + col;
+  if (col < g.nx)                                                              \
+    for (int rr_ = threadIdx.y; rr_ < kCellRows; rr_ += 4) {                   \
+      const int row = by_ * kCellRows + rr_;                                   \
+      if (row >= g.ny) break;                                                  \
+      const int i = row * g.nx + col;
+#define PFF_CELL_LOOP_END }
 
-static dim3 cell_grid(const GridD& g) { return dim3((g.nx + 63) / 64, (g.ny + 3) / 4); }
+static dim3 cell_grid(const GridD& g) { return dim3((g.nx + 63) / 64, (g.ny + kCellRows - 1) / kCellRows); }
 
 __global__ __launch_bounds__(kBT) void k_saddle_init_list(const int32_t* __restrict__ pits, const int64_t* np,
                                                           int64_t* saddle, int64_t* outlet, int32_t* snode) {
@@ -453,13 +459,16 @@ __global__ __launch_bounds__(kBT) void k_zprime_min_t(GridD g, int tiles_x, cons
                                                       const int32_t* __restrict__ bid, const float* __restrict__ z,
                                                       uint8_t* __restrict__ isb, float* __restrict__ zp,
                                                       int64_t* saddle) {
-  // no early return: the whole wave takes part in the segmented minimum
+  // no early return inside the row loop: the whole wave (one row of 64
+  // cells) takes part in the segmented minimum
   int bx_, by_;
   xcd_block(bx_, by_);
-  if (tact && tact[(by_ >> 4) * tiles_x + bx_] == 0u) return;  // uniform per block
+  if (tact && tact[((by_ * kCellRows) / TSL) * tiles_x + bx_] == 0u) return;  // uniform per block
   const int col = bx_ * 64 + threadIdx.x;
-  const int row = by_ * 4 + threadIdx.y;
-  const bool inside = col < g.nx && row < g.ny;
+  for (int rr_ = threadIdx.y; rr_ < kCellRows; rr_ += 4) {  // uniform per wave
+  const int row = by_ * kCellRows + rr_;
+  if (row >= g.ny) break;
+  const bool inside = col < g.nx;
   const int i = inside ? row * g.nx + col : 0;
   int keyb = -1 - (int)threadIdx.x;  // lanes without a candidate never match another lane
   int64_t res = INVALID_KEY;
@@ -511,6 +520,7 @@ __global__ __launch_bounds__(kBT) void k_zprime_min_t(GridD g, int tiles_x, cons
   const int prevb = __shfl_up(keyb, 1);
   const bool head = keyb >= 0 && (lane == 0 || prevb != keyb);
   if (head && res < saddle[keyb]) atomicMin((long long*)&saddle[keyb], (long long)res);
+  }
 }
 
 // pass 3 (:199-219), largest cell index wins
@@ -519,13 +529,13 @@ __global__ __launch_bounds__(kBT) void k_saddle_node_t(GridD g, int tiles_x, con
                                                        const int32_t* __restrict__ bid, const uint8_t* __restrict__ isb,
                                                        const float* __restrict__ zp, const int64_t* __restrict__ saddle,
                                                        int32_t* snode) {
-  PFF_CELL_PROLOGUE(tact, tiles_x)
+  PFF_CELL_LOOP_BEGIN(tact, tiles_x)
   const int bi = bid[i];
-  if (bi == 0 || bi >= g.N || can_leave<MODE>(g, i, row, col) || !isb[i]) return;
+  if (bi == 0 || bi >= g.N || can_leave<MODE>(g, i, row, col) || !isb[i]) continue;
   const int64_t key = saddle[bi];
   const float tz = key_float(key);
   const int tb = key_index(key);
-  if (zp[i] != tz) return;
+  if (zp[i] != tz) continue;
   bool here = false;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -534,6 +544,7 @@ __global__ __launch_bounds__(kBT) void k_saddle_node_t(GridD g, int tiles_x, con
     if (bid[jn] == tb) here = true;
   }
   if (here) atomicMax(&snode[bi], i);
+  PFF_CELL_LOOP_END
 }
 
 // pass 4 (:226-250) over the pits
@@ -678,11 +689,12 @@ int saddlesort_active(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t*
 __global__ __launch_bounds__(kBT) void k_carve_reverse_t(GridD g, int tiles_x, const uint32_t* __restrict__ tact,
                                                          const uint8_t* __restrict__ tag,
                                                          const int32_t* __restrict__ rec_in, int32_t* rec) {
-  PFF_CELL_PROLOGUE(tact, tiles_x)
+  PFF_CELL_LOOP_BEGIN(tact, tiles_x)
   if (tag[i]) {
     const int r = rec_in[i];
     if (r != i && (unsigned)r < (unsigned)g.N && tag[r]) rec[r] = i;
   }
+  PFF_CELL_LOOP_END
 }
 
 // saddle node -> outlet (lakeflow.py:399-403) over the pits
@@ -701,8 +713,9 @@ __global__ __launch_bounds__(kBT) void k_carve_link_list(const int32_t* __restri
 __global__ __launch_bounds__(kBT) void k_carve_sync_t(GridD g, int tiles_x, const uint32_t* __restrict__ tact,
                                                       const uint8_t* __restrict__ tag, const int32_t* __restrict__ rec,
                                                       int32_t* __restrict__ rec_work) {
-  PFF_CELL_PROLOGUE(tact, tiles_x)
+  PFF_CELL_LOOP_BEGIN(tact, tiles_x)
   if (tag[i]) rec_work[i] = rec[i];
+  PFF_CELL_LOOP_END
 }
 
 // Carve for one trip with the tiled contraction: the cells the reference

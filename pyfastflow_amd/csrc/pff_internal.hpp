@@ -89,7 +89,7 @@ enum Slot : int {
 };
 
 // device word indices (dev_words / host_words)
-enum Word : int { W_COUNT0 = 0, W_COUNT1, W_COUNT2, W_FLAG, W_FLAG2, W_CYCLE, W_NEXIT, W_NDEP, W_NPIT, W_LIVE, W_EMPTY, W_GFMAX, W_GFNAN, W_NPIT2, W_MCHG, W_EMPTY2, W_GFABOVE, W_NWORDS = 24 };
+enum Word : int { W_COUNT0 = 0, W_COUNT1, W_COUNT2, W_FLAG, W_FLAG2, W_CYCLE, W_NEXIT, W_NDEP, W_NPIT, W_LIVE, W_EMPTY, W_GFMAX, W_GFNAN, W_NPIT2, W_MCHG, W_EMPTY2, W_GFABOVE, W_GFSUM, W_NWORDS = 24 };
 
 constexpr int kBlock = 256;
 inline unsigned grid_for(int64_t n, int block = kBlock, int64_t cap = 1 << 20) {
@@ -212,7 +212,8 @@ int launch_pack_input(pff_ctx* ctx, int64_t npix, int nplanes, int C, const floa
 
 int launch_edge_maps(pff_ctx* ctx, const GridD& g, uint8_t* edges, uint8_t* out);
 int launch_maxdiff(pff_ctx* ctx, int32_t N, const float* a, const float* b, uint32_t* out_max,
-                   unsigned long long* out_nan, float tol = 0.f, unsigned long long* out_above = nullptr);
+                   unsigned long long* out_nan, float tol = 0.f, unsigned long long* out_above = nullptr,
+                   double* out_sum = nullptr);
 int launch_pack(pff_ctx* ctx, int64_t n, int64_t* arr, const float* ft, const int32_t* it);
 int launch_unpack(pff_ctx* ctx, int64_t n, const int64_t* arr, float* ft, int32_t* it);
 int launch_swap(pff_ctx* ctx, int64_t n, uint32_t* a, uint32_t* b);

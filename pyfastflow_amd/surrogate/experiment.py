@@ -60,10 +60,12 @@ def main(argv=None):
         with torch.no_grad():
             val = float(((model(X[:n_val]) - Y[:n_val]) ** 2).mean())
             base = float(((Y[:n_val] - Y[n_val:].mean()) ** 2).mean())
+            guess = float((Y[:n_val] ** 2).mean())  # error of the analytic guess alone
         flops = model.flops_per_pixel() * 3 * X[n_val:].shape[0] * X.shape[-1] ** 2 * a.epochs
         dt = time.time() - t0
         res["train"] = {"epochs": a.epochs, "first_loss": losses[0], "last_loss": losses[-1], "val_mse": val,
-                        "val_mse_of_mean_predictor": base, "seconds": round(dt, 1),
+                        "val_mse_of_mean_correction": base, "val_mse_of_analytic_guess": guess,
+                        "seconds": round(dt, 1),
                         "TFLOP_per_s": round(flops / dt / 1e12, 2)}
         log(f"[surrogate] training {res['train']}")
         if a.out:
@@ -77,7 +79,7 @@ def main(argv=None):
     with torch.no_grad():
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        planes = torch.randn(1, 4, n, n, device="cuda")
+        planes = torch.randn(1, model.in_planes, n, n, device="cuda")
         model(planes)
         torch.cuda.synchronize()
         t_inf = time.perf_counter() - t0

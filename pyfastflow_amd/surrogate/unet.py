@@ -11,7 +11,7 @@ from torch import nn
 
 from . import ops
 
-PLANES = ("z", "log_area", "rain", "manning")
+PLANES = ("z", "log_area", "log_slope", "log_fill_depth", "rain", "manning")
 
 
 class UNet(nn.Module):
@@ -40,6 +40,8 @@ class UNet(nn.Module):
             add(ch[k], ch[k])
         self.hw = nn.Parameter(torch.randn(base, generator=g) * math.sqrt(1.0 / base))
         self.hb = nn.Parameter(torch.zeros(1))
+        # range of the training targets (log depth); predictions are clamped to it
+        self.register_buffer("target_range", torch.tensor([-10.0, 10.0]))
 
     def forward(self, planes: torch.Tensor) -> torch.Tensor:
         """planes (n, in_planes, h, w) f32, h and w multiples of 2^(levels-1)

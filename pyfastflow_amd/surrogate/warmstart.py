@@ -4,7 +4,8 @@ Inputs of the network for a DEM with rainfall P and Manning's n: the elevation
 (standardised per tile), the log drainage area after lakeflow rerouting (the
 accumulation of FlowRouter on the DEM), log10 P and n, as four planes.  The
 target is log(h + H_FLOOR) of the GraphFlood water depth the plain solver
-reaches from a cold start.
+reaches from a cold start, learnt as a correction to an analytic first guess
+(Manning depth on hillslopes, fill depth in lakes).
 
 A warm start sets h to the prediction and zh = z + h (Flooder.set_h alone
 does not touch zh, gf_fields.py:143-150), with Q from the accumulation of the
@@ -29,19 +30,43 @@ from .unet import UNet
 H_FLOOR = 1e-3  # m: log(h + H_FLOOR) keeps dry cells finite
 
 
-def _planes(router: FlowRouter, rain: float, manning: float) -> torch.Tensor:
-    """(1, 4, ny, nx) input planes; leaves router.Q = drainage area (m^2)."""
+def _planes(router: FlowRouter, rain: float, manning: float, with_base: bool = False):
+    """(1, 6, ny, nx) input planes: standardised z, log drainage area after
+    lakeflow, log steepest-descent slope, log depth of the filled
+    depressions, log10 rainfall and Manning's n.  Leaves router.Q = drainage
+    area (m^2).  On a hillslope the steady depth is a power law of area,
+    slope, rainfall and n (Manning), and in a filled lake it is the fill
+    depth, so these planes carry what the depth depends on.
+
+    with_base also returns the (ny, nx) analytic first guess the network
+    corrects: log(max(h_manning, fill depth) + H_FLOOR), with h_manning the
+    depth at which Manning outflow over the steepest slope carries the
+    rainfall of the drainage area, (P A n / (dx sqrt(S)))^(3/5)."""
+    from ..flow.fill_topo import topofill
+
+    ny, nx = router.ny, router.nx
     router.compute_receivers()
+    slope = router.gradient.data.view(ny, nx).clone()
     router.reroute_flow(carve=True)
     router.accumulate_constant_Q(1.0, area=True)
-    ny, nx = router.ny, router.nx
     z = router.z.data.view(ny, nx)
+    zf = torch.empty_like(router.z.data)
+    topofill(router, epsilon=1e-4, custom_z=zf)
+    depth = (zf.view(ny, nx) - z).clamp_min(0)
     area = router.Q.data.view(ny, nx)
     zn = (z - z.mean()) / (z.std() + 1e-6)
     la = (torch.log(area) - 10.0) / 3.0
+    ls = (torch.log(slope.clamp_min(1e-4)) + 3.0) / 2.0
+    ld = (torch.log(depth + H_FLOOR) + 3.0) / 3.0
     pr = torch.full_like(z, (math.log10(rain) + 4.65) / 0.5)
     mn = torch.full_like(z, (manning - 0.04) / 0.012)
-    return torch.stack([zn, la, pr, mn])[None].float()
+    planes = torch.stack([zn, la, ls, ld, pr, mn])[None].float()
+    if not with_base:
+        return planes
+    dx = float(router.dx)
+    hm = (rain * area * manning / (dx * torch.sqrt(slope.clamp_min(1e-4)))) ** 0.6
+    base = torch.log(torch.maximum(hm, depth) + H_FLOOR)
+    return planes, base.float()
 
 
 def target_of(h: torch.Tensor) -> torch.Tensor:
@@ -70,7 +95,8 @@ def make_dataset(n_domains: int = 8, size: int = 1024, tile: int = 256, iters: i
     """Plain-solver depth fields on config-2 style DEMs (beta 2.0, relief
     U[300, 1000] m, pits in proportion to area) with rainfall U[5e-6, 5e-5]
     m/s and n U[0.02, 0.06], cut into tile x tile samples.  Returns planes
-    (S, 4, tile, tile) and targets (S, tile, tile) on the GPU."""
+    (S, 6, tile, tile) and targets (S, tile, tile) on the GPU; a target is
+    the log depth minus the analytic first guess of _planes."""
     rng = np.random.Generator(np.random.PCG64(seed))
     X, Y = [], []
     for d in range(n_domains):
@@ -81,7 +107,7 @@ def make_dataset(n_domains: int = 8, size: int = 1024, tile: int = 256, iters: i
         z = synth.dem(size, size, 2.0, relief, int(rng.integers(0, 2**31 - 1)), n_pits=pits, dx=10.0)
         r = FlowRouter(size, size, 10.0, boundary_mode="normal")
         r.set_z(z)
-        planes = _planes(r, rain, man)
+        planes, base = _planes(r, rain, man, with_base=True)
         fl = Flooder(r, precipitation_rates=rain, manning=man, edge_slope=1e-2, update=update)
         _reset_start(r, fl, z, None, rain)
         t0 = time.time()
@@ -90,7 +116,7 @@ def make_dataset(n_domains: int = 8, size: int = 1024, tile: int = 256, iters: i
         h = fl.h.data.view(size, size)
         log(f"[surrogate] domain {d}: rain {rain:.2e} n {man:.3f} relief {relief:.0f} m, {iters} iterations in "
             f"{time.time() - t0:.1f} s, h mean {float(h.mean()):.4f} max {float(h.max()):.2f}")
-        y = target_of(h)
+        y = target_of(h) - base  # the network learns the correction to the analytic guess
         for i in range(0, size, tile):
             for j in range(0, size, tile):
                 X.append(planes[0, :, i:i + tile, j:j + tile])
@@ -104,6 +130,7 @@ def train(model: UNet, X: torch.Tensor, Y: torch.Tensor, epochs: int = 20, batch
     """Adam on the mean squared error of log depth, with random flips and
     transpositions of the tiles.  Returns the per-epoch mean losses."""
     g = torch.Generator(device="cpu").manual_seed(seed)
+    model.target_range.copy_(torch.stack([Y.min(), Y.max()]).to(model.target_range))
     opt = torch.optim.Adam(model.parameters(), lr=lr)
     sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, T_max=max(epochs, 1))
     losses = []
@@ -136,18 +163,22 @@ def train(model: UNet, X: torch.Tensor, Y: torch.Tensor, epochs: int = 20, batch
 
 @torch.no_grad()
 def predict_depth(model: UNet, router: FlowRouter, rain: float, manning: float) -> torch.Tensor:
-    planes = _planes(router, rain, manning)
-    return depth_of(model(planes))[0]
+    """Predicted depth: the analytic first guess corrected by the network,
+    the correction clamped to the range it was trained on
+    (model.target_range)."""
+    planes, base = _planes(router, rain, manning, with_base=True)
+    lo, hi = (float(v) for v in model.target_range)
+    return depth_of(base + model(planes)[0].clamp(lo, hi))
 
 
 def solve(z: np.ndarray, rain: float, manning: float, model: UNet | None, tol: float, frac: float = 0.0,
-          check_every: int = 100, max_iter: int = 200000, update: str = "single", log=print):
+          check_every: int = 100, max_iter: int = 200000, update: str = "single", log=print, chunk: int = 5000):
     """GraphFlood to steady (Flooder.run_to_steady with tol and frac) from a
     cold start, and from the model's warm start (if a model is given).  The
-    warm start is kept only if its first residual (the fraction of cells
-    still moving by more than tol, or the max change when frac is 0) is not
-    worse than the cold start's.  Returns a dict with iterations, wall times
-    (inference included) and residuals of both."""
+    warm start is kept only if its first residual (the mean change of h over
+    the first check block) is not worse than the cold start's.  Returns a
+    dict with iterations, wall times (inference included) and residuals of
+    both."""
     ny, nx = z.shape
     out = {}
     for mode in (["cold", "warm"] if model is not None else ["cold"]):
@@ -160,15 +191,29 @@ def solve(z: np.ndarray, rain: float, manning: float, model: UNet | None, tol: f
         if mode == "warm":
             h0 = predict_depth(model, r, rain, manning)
         _reset_start(r, fl, z, h0, rain)
-        st = fl.run_to_steady(tol=tol, check_every=check_every, max_iter=max_iter, frac=frac)
+        # in chunks (whole check blocks) so that progress is visible
+        its, first, first_frac, first_mean = 0, None, None, None
+        while True:
+            n = min(chunk, max_iter - its)
+            st = fl.run_to_steady(tol=tol, check_every=check_every, max_iter=n, frac=frac)
+            its += int(st.iterations)
+            if first is None:
+                first, first_frac = float(st.first_residual), float(st.first_above_frac)
+                first_mean = float(st.first_mean_change)
+            log(f"[surrogate] {mode}: {its} iterations, max change {float(st.residual):.3e} m, mean change "
+                f"{float(st.mean_change):.3e} m, {100 * float(st.above_frac):.1f}% of cells above tol")
+            if st.converged or st.nan_cells or its >= max_iter:
+                break
         torch.cuda.synchronize()
-        out[mode] = {"iterations": int(st.iterations), "seconds": time.perf_counter() - t0,
-                     "residual": float(st.residual), "first_residual": float(st.first_residual),
-                     "above_frac": float(st.above_frac), "first_above_frac": float(st.first_above_frac),
+        out[mode] = {"iterations": its, "seconds": time.perf_counter() - t0,
+                     "residual": float(st.residual), "first_residual": first,
+                     "above_frac": float(st.above_frac), "first_above_frac": first_frac,
+                     "mean_change": float(st.mean_change), "first_mean_change": first_mean,
                      "converged": bool(st.converged), "nan_cells": int(st.nan_cells)}
         log(f"[surrogate] {mode}: {out[mode]}")
         del fl, r
-    key = "first_above_frac" if frac > 0 else "first_residual"
+    # the initial residual: mean change of h over the first check block
+    key = "first_mean_change"
     if "warm" in out and out["warm"][key] > out["cold"][key]:
         out["used"] = "cold (warm start's first residual was worse)"
     else:
