@@ -459,6 +459,18 @@ __global__ __launch_bounds__(kBT) void k_zprime_min_t(GridD g, int tiles_x, cons
                                                       const int32_t* __restrict__ bid, const float* __restrict__ z,
                                                       uint8_t* __restrict__ isb, float* __restrict__ zp,
                                                       int64_t* saddle) {
+  // Per-basin minima go through a small LDS table per block (one global
+  // atomic per basin and block instead of one per wave); a basin that finds
+  // the table full falls back to a global atomic.
+  constexpr int kSlots = 64;
+  __shared__ int32_t tb[kSlots];
+  __shared__ long long tk[kSlots];
+  const int tid = threadIdx.y * 64 + threadIdx.x;
+  if (tid < kSlots) {
+    tb[tid] = -1;
+    tk[tid] = (long long)INVALID_KEY;
+  }
+  __syncthreads();
   // no early return inside the row loop: the whole wave (one row of 64
   // cells) takes part in the segmented minimum
   int bx_, by_;
@@ -519,7 +531,26 @@ __global__ __launch_bounds__(kBT) void k_zprime_min_t(GridD g, int tiles_x, cons
   }
   const int prevb = __shfl_up(keyb, 1);
   const bool head = keyb >= 0 && (lane == 0 || prevb != keyb);
-  if (head && res < saddle[keyb]) atomicMin((long long*)&saddle[keyb], (long long)res);
+  if (head) {
+    bool done = false;
+    int slot = (int)(((uint32_t)keyb * 2654435761u) >> 26);  // 6-bit hash
+    for (int probe = 0; probe < 8 && !done; ++probe, slot = (slot + 1) & (kSlots - 1)) {
+      const int prev = atomicCAS(&tb[slot], -1, keyb);
+      if (prev == -1 || prev == keyb) {
+        atomicMin(&tk[slot], (long long)res);
+        done = true;
+      }
+    }
+    if (!done && res < saddle[keyb]) atomicMin((long long*)&saddle[keyb], (long long)res);
+  }
+  }
+  __syncthreads();
+  if (tid < kSlots) {
+    const int b = tb[tid];
+    if (b >= 0) {
+      const long long v = tk[tid];
+      if (v < (long long)saddle[b]) atomicMin((long long*)&saddle[b], v);
+    }
   }
 }
 

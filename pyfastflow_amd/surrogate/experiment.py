@@ -23,7 +23,7 @@ import torch
 
 from .. import synth
 from .unet import UNet
-from .warmstart import make_dataset, solve, train
+from .warmstart import load_weights, make_dataset, save_weights, solve, train
 
 
 def log(*a):
@@ -47,7 +47,7 @@ def main(argv=None):
     res = {}
     model = UNet().cuda()
     if a.weights:
-        model.load_state_dict(torch.load(a.weights, map_location="cuda", weights_only=True))
+        load_weights(model, a.weights)
     else:
         t0 = time.time()
         X, Y = make_dataset(n_domains=a.domains, size=a.size, iters=a.iters, log=log)
@@ -69,7 +69,7 @@ def main(argv=None):
                         "TFLOP_per_s": round(flops / dt / 1e12, 2)}
         log(f"[surrogate] training {res['train']}")
         if a.out:
-            torch.save(model.state_dict(), a.out)
+            save_weights(model, a.out)
     model.eval()
 
     # held-out DEM: config-2 style synthesis at eval_size, open edges

@@ -219,3 +219,20 @@ def solve(z: np.ndarray, rain: float, manning: float, model: UNet | None, tol: f
     else:
         out["used"] = "warm" if "warm" in out else "cold"
     return out
+
+
+WEIGHTS = __file__.rsplit("/", 1)[0] + "/unet_warmstart.pt"
+
+
+def save_weights(model: UNet, path: str = WEIGHTS):
+    """bf16 copy of the weights (the kernels use bf16 operands) and the f32
+    target range."""
+    sd = {k: (v.detach().to(torch.bfloat16).cpu() if v.dtype == torch.float32 and k != "target_range"
+              else v.detach().cpu()) for k, v in model.state_dict().items()}
+    torch.save(sd, path)
+
+
+def load_weights(model: UNet, path: str = WEIGHTS) -> UNet:
+    sd = torch.load(path, map_location="cpu", weights_only=True)
+    model.load_state_dict({k: v.float() for k, v in sd.items()})
+    return model
