@@ -461,26 +461,38 @@ __global__ __launch_bounds__(kTB) void k_tile(TileArgs a) {
   bool sums = W == W_UNIT;
   if constexpr (W == W_SADDLE) sums = __syncthreads_or(wany) != 0;
   else __syncthreads();
+  // Saddle tags run on receivers already known to be a forest; a tile with
+  // no weight there only needs its local ends, jumped on E alone until they
+  // stop changing (uniform per block)
+  bool eonly = false;
+  if constexpr (PH == 1 && W == W_SADDLE) eonly = !sums;
   bool cyc = true;
   for (int k = 0; k < kTileRounds; ++k) {
     uint16_t pm[CPT], pp[CPT], ee[CPT];
     uint32_t sv[CPT];
+    int any = 0;
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
       const int li = threadIdx.x + j * kTB;
-      pm[j] = P[li];
+      pm[j] = NONE;
       pp[j] = NONE;
       sv[j] = 0;
-      if (pm[j] != NONE) {
-        pp[j] = P[pm[j]];
-        if constexpr (W != W_ROOTS) {
-          if (sums) sv[j] = S[li];
+      if (!eonly) {
+        pm[j] = P[li];
+        if (pm[j] != NONE) {
+          pp[j] = P[pm[j]];
+          if constexpr (W != W_ROOTS) {
+            if (sums) sv[j] = S[li];
+          }
         }
       }
-      if constexpr (PH == 1) ee[j] = E[E[li]];
+      if constexpr (PH == 1) {
+        const uint16_t e0 = E[li];
+        ee[j] = E[e0];
+        if (eonly) any |= ee[j] != e0;
+      }
     }
     __syncthreads();
-    int any = 0;
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
       const int li = threadIdx.x + j * kTB;
