@@ -187,6 +187,7 @@ int reroute_core(pff_ctx* ctx, const GridD& d, int32_t* bid, int32_t* rec, int32
   int nouter = 1;
   int64_t nprev = 0;
   bool relabelled = false;  // this trip's labels came from the previous trip's outlets
+  int border_cur = -1;      // border list of the previous trip (-1: none yet)
   for (int it = 0; it < nouter; ++it) {
     int64_t ndb = 0;
     relabelled = false;
@@ -248,7 +249,7 @@ int reroute_core(pff_ctx* ctx, const GridD& d, int32_t* bid, int32_t* rec, int32
       PFF_TRY(tiled_pits(ctx, d, &pits, &npits));
       if (bs) PFF_TRY(batch_mark42(ctx, bs, d, bid, 1));
       PFF_TRY(saddlesort_active(ctx, d, bid, is_border, z_prime, basin_saddle, basin_saddlenode, outlet, z, tact, pits,
-                                npits));
+                                npits, &border_cur));
       if (bs) PFF_TRY(batch_mark42(ctx, bs, d, bid, 0));
       PFF_TRY(log_trip(ctx, d, bid, outlet, basin_saddlenode, st ? st->n_iter : it));
       if (st) st->n_iter++;

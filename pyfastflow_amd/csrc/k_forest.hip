@@ -389,6 +389,9 @@ struct TileArgs {
   // saddle tags: tiles that receive a marked inflow (the final pass skips
   // the others, whose tags the local pass already wrote)
   uint32_t* tin;
+  // saddle tags: list of the tagged cells (nullable), each cell once
+  int32_t* tlist;
+  int64_t* n_tlist;
   unsigned long long* prof_cells;  // profiling: cells processed by this launch
 };
 
@@ -530,6 +533,7 @@ __global__ __launch_bounds__(kTB) void k_tile(TileArgs a) {
         a.inflow[c] = 0u;
       }
       if constexpr (W == W_SADDLE) a.tag[c] = S[li] ? 1 : 0;  // final unless the tile gets a marked inflow
+      // (the tagged cells are listed below)
       if (e == li) {  // local end
         if (rootmask & (1u << j)) {
           if constexpr (W == W_ROOTS) {
@@ -553,18 +557,33 @@ __global__ __launch_bounds__(kTB) void k_tile(TileArgs a) {
     }
     if constexpr (W == W_ROOTS) block_append<CPT, kTB>(pv, tpit, a.pits, a.n_pits);
     block_append<CPT, kTB>(ex, take, a.exits, a.n_exits);
+    if constexpr (W == W_SADDLE) {
+      if (sums && a.tlist) {  // uniform per block; a tile without weights has no tag yet
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) tpit[j] = cell[j] >= 0 && S[threadIdx.x + j * kTB] != 0u;
+        block_append<CPT, kTB>(cell, tpit, a.tlist, a.n_tlist);
+      }
+    }
   } else {
+    bool fresh[CPT];
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
       const int li = threadIdx.x + j * kTB;
       const int c = cell[j];
+      fresh[j] = false;
       if (c < 0) continue;
       if constexpr (W == W_UNIT) {
         if (a.cnt) a.cnt[c] = S[li];
         if (a.Q) a.Q[c] = a.q0 * (float)S[li];
       } else if constexpr (W == W_SADDLE) {
-        a.tag[c] = S[li] ? 1 : 0;
+        // the local pass listed the cells it tagged: list the new ones only
+        const bool t = S[li] != 0u;
+        fresh[j] = t && a.tag[c] == 0;
+        a.tag[c] = t ? 1 : 0;
       }
+    }
+    if constexpr (W == W_SADDLE) {
+      if (a.tlist) block_append<CPT, kTB>(cell, fresh, a.tlist, a.n_tlist);
     }
   }
 }
@@ -705,10 +724,14 @@ int tiles_count(const GridD& g) { return ((g.nx + TS - 1) / TS) * ((g.ny + TS - 
 // graph that is not a forest (outputs are then not meaningful).
 int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, const int32_t* bid,
                    const int32_t* snode, float q0, float* Q, uint32_t* cnt, uint8_t* tag, const uint32_t* tact,
-                   int32_t* cycle, int rounds) {
+                   int32_t* cycle, int rounds, int32_t* tag_list) {
   TileBufs b;
   PFF_TRY(tile_bufs(ctx, g, b));
   TileArgs a = tile_args(g, b, ctx, rcv);
+  if (tag_list) {
+    a.tlist = tag_list;
+    a.n_tlist = ctx->dev_words + W_NTAG;
+  }
   a.bid = bid;
   a.snode = snode;
   a.q0 = q0;

@@ -440,8 +440,14 @@ constexpr int kCellRows = 16;
 
 static dim3 cell_grid(const GridD& g) { return dim3((g.nx + 63) / 64, (g.ny + kCellRows - 1) / kCellRows); }
 
+// also zeroes the list counters of this trip (clear0, clear1; nullable)
 __global__ __launch_bounds__(kBT) void k_saddle_init_list(const int32_t* __restrict__ pits, const int64_t* np,
-                                                          int64_t* saddle, int64_t* outlet, int32_t* snode) {
+                                                          int64_t* saddle, int64_t* outlet, int32_t* snode,
+                                                          int64_t* clear0, int64_t* clear1) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (clear0) *clear0 = 0;
+    if (clear1) *clear1 = 0;
+  }
   const int64_t n = *np;
   for (int64_t w = (int64_t)blockIdx.x * kBT + threadIdx.x; w < n; w += (int64_t)gridDim.x * kBT) {
     const int b = pits[w];
@@ -451,78 +457,32 @@ __global__ __launch_bounds__(kBT) void k_saddle_init_list(const int32_t* __restr
   }
 }
 
-// passes 1+2 fused (lakeflow.py:132-194): z', border flag and the packed
-// atomicMin of (z'_i, bid_j); a plain read first skips atomics that cannot
-// lower the current minimum
-template <int MODE>
-__global__ __launch_bounds__(kBT) void k_zprime_min_t(GridD g, int tiles_x, const uint32_t* __restrict__ tact,
-                                                      const int32_t* __restrict__ bid, const float* __restrict__ z,
-                                                      uint8_t* __restrict__ isb, float* __restrict__ zp,
-                                                      int64_t* saddle) {
-  // Per-basin minima go through a small LDS table per block (one global
-  // atomic per basin and block instead of one per wave); a basin that finds
-  // the table full falls back to a global atomic.
-  constexpr int kSlots = 64;
-  __shared__ int32_t tb[kSlots];
-  __shared__ long long tk[kSlots];
-  const int tid = threadIdx.y * 64 + threadIdx.x;
-  if (tid < kSlots) {
-    tb[tid] = -1;
-    tk[tid] = (long long)INVALID_KEY;
+// Per-basin minima of packed saddle keys.  Each lane offers at most one
+// (basin, key) pair; a segmented minimum over runs of lanes of the same basin
+// leaves one offer per run, which goes to a small LDS table per block (one
+// global atomic per basin and block at the flush) or, when the table has no
+// room for the basin, straight to a global atomicMin after a plain read shows
+// that it can lower the stored minimum.
+constexpr int kSaddleSlots = 512;
+
+struct SaddleTable {
+  int32_t b[kSaddleSlots];
+  long long k[kSaddleSlots];
+};
+
+__device__ __forceinline__ void saddle_table_init(SaddleTable& t) {
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  for (int q = tid; q < kSaddleSlots; q += kBT) {
+    t.b[q] = -1;
+    t.k[q] = (long long)INVALID_KEY;
   }
   __syncthreads();
-  // no early return inside the row loop: the whole wave (one row of 64
-  // cells) takes part in the segmented minimum
-  int bx_, by_;
-  xcd_block(bx_, by_);
-  if (tact && tact[((by_ * kCellRows) / TSL) * tiles_x + bx_] == 0u) return;  // uniform per block
-  const int col = bx_ * 64 + threadIdx.x;
-  for (int rr_ = threadIdx.y; rr_ < kCellRows; rr_ += 4) {  // uniform per wave
-  const int row = by_ * kCellRows + rr_;
-  if (row >= g.ny) break;
-  const bool inside = col < g.nx;
-  const int i = inside ? row * g.nx + col : 0;
-  int keyb = -1 - (int)threadIdx.x;  // lanes without a candidate never match another lane
-  int64_t res = INVALID_KEY;
-  if (inside) {
-    const float zi = z[i];
-    if (can_leave<MODE>(g, i, row, col)) {
-      zp[i] = zi;
-    } else {
-      const int bi = bid[i];
-      bool border = false;
-      float zn = 1e9f;
-      int nbj[4];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int jn = neighbour<MODE>(g, i, row, col, k);
-        nbj[k] = -1;
-        if (jn < 0) continue;
-        const int bj = bid[jn];
-        if (bj != bi) {
-          border = true;
-          zn = fminf(zn, z[jn]);
-          nbj[k] = bj;
-        }
-      }
-      isb[i] = border ? 1 : 0;
-      const float zpi = border ? fmaxf(zi, zn) : 1e9f;
-      zp[i] = zpi;
-      if (border && bi != 0 && bi < g.N) {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          if (nbj[k] < 0) continue;
-          const int64_t c = pack_key(zpi, nbj[k]);
-          res = c < res ? c : res;
-        }
-        if (res != INVALID_KEY) keyb = bi;
-      }
-    }
-  }
-  // segmented minimum over runs of lanes of the same basin (64 lanes = 64
-  // consecutive cells of a row); only the first lane of a run issues the
-  // atomic, after a plain read shows it can lower the stored minimum
-  const int lane = threadIdx.x;
+}
+
+// keyb < 0: no offer from this lane (every lane of the wave must call)
+__device__ __forceinline__ void saddle_offer(SaddleTable& t, int keyb, int64_t res, int64_t* saddle) {
+  const int lane = (threadIdx.x + threadIdx.y * blockDim.x) & 63;
+  if (keyb < 0) keyb = -1 - lane;  // lanes without an offer never match another lane
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
     const int ob = __shfl_down(keyb, d);
@@ -530,43 +490,162 @@ __global__ __launch_bounds__(kBT) void k_zprime_min_t(GridD g, int tiles_x, cons
     if (lane + d < 64 && ob == keyb && ov < res) res = ov;
   }
   const int prevb = __shfl_up(keyb, 1);
-  const bool head = keyb >= 0 && (lane == 0 || prevb != keyb);
-  if (head) {
+  if (keyb >= 0 && (lane == 0 || prevb != keyb)) {
     bool done = false;
-    int slot = (int)(((uint32_t)keyb * 2654435761u) >> 26);  // 6-bit hash
-    for (int probe = 0; probe < 8 && !done; ++probe, slot = (slot + 1) & (kSlots - 1)) {
-      const int prev = atomicCAS(&tb[slot], -1, keyb);
+    int slot = (int)(((uint32_t)keyb * 2654435761u) >> 23);  // 9-bit hash
+    for (int probe = 0; probe < 8 && !done; ++probe, slot = (slot + 1) & (kSaddleSlots - 1)) {
+      const int prev = atomicCAS(&t.b[slot], -1, keyb);
       if (prev == -1 || prev == keyb) {
-        atomicMin(&tk[slot], (long long)res);
+        atomicMin(&t.k[slot], (long long)res);
         done = true;
       }
     }
     if (!done && res < saddle[keyb]) atomicMin((long long*)&saddle[keyb], (long long)res);
   }
-  }
+}
+
+__device__ __forceinline__ void saddle_table_flush(SaddleTable& t, int64_t* saddle) {
   __syncthreads();
-  if (tid < kSlots) {
-    const int b = tb[tid];
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  for (int q = tid; q < kSaddleSlots; q += kBT) {
+    const int b = t.b[q];
     if (b >= 0) {
-      const long long v = tk[tid];
+      const long long v = t.k[q];
       if (v < (long long)saddle[b]) atomicMin((long long*)&saddle[b], v);
     }
   }
 }
 
-// pass 3 (:199-219), largest cell index wins
+// z', border flag and the offer of cell i (not an outlet cell); returns true
+// when i is a border cell of a closed basin (the cells the saddle passes use)
 template <int MODE>
-__global__ __launch_bounds__(kBT) void k_saddle_node_t(GridD g, int tiles_x, const uint32_t* __restrict__ tact,
-                                                       const int32_t* __restrict__ bid, const uint8_t* __restrict__ isb,
-                                                       const float* __restrict__ zp, const int64_t* __restrict__ saddle,
-                                                       int32_t* snode) {
-  PFF_CELL_LOOP_BEGIN(tact, tiles_x)
+__device__ __forceinline__ bool zprime_cell(const GridD& g, int i, int row, int col, float zi,
+                                            const int32_t* __restrict__ bid, const float* __restrict__ z,
+                                            uint8_t* __restrict__ isb, float* __restrict__ zp, int& keyb,
+                                            int64_t& res) {
   const int bi = bid[i];
-  if (bi == 0 || bi >= g.N || can_leave<MODE>(g, i, row, col) || !isb[i]) continue;
+  bool border = false;
+  float zn = 1e9f;
+  int nbj[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int jn = neighbour<MODE>(g, i, row, col, k);
+    nbj[k] = -1;
+    if (jn < 0) continue;
+    const int bj = bid[jn];
+    if (bj != bi) {
+      border = true;
+      zn = fminf(zn, z[jn]);
+      nbj[k] = bj;
+    }
+  }
+  isb[i] = border ? 1 : 0;
+  const float zpi = border ? fmaxf(zi, zn) : 1e9f;
+  zp[i] = zpi;
+  if (!border || bi == 0 || bi >= g.N) return false;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (nbj[k] < 0) continue;
+    const int64_t c = pack_key(zpi, nbj[k]);
+    res = c < res ? c : res;
+  }
+  if (res != INVALID_KEY) keyb = bi;
+  return true;
+}
+
+// passes 1+2 fused (lakeflow.py:132-194) over the active tiles: z', border
+// flag and the packed minimum of (z'_i, bid_j) per basin; the border cells of
+// closed basins are appended to blist in row-major order (blist may be null)
+template <int MODE>
+__global__ __launch_bounds__(kBT) void k_zprime_min_t(GridD g, int tiles_x, const uint32_t* __restrict__ tact,
+                                                      const int32_t* __restrict__ bid, const float* __restrict__ z,
+                                                      uint8_t* __restrict__ isb, float* __restrict__ zp,
+                                                      int64_t* saddle, int32_t* blist, int64_t* nblist) {
+  // one block per 64x64 tile, one wave per row: one table flush and one list
+  // append per tile
+  __shared__ SaddleTable t;
+  int bx_, by_;
+  xcd_block(bx_, by_);
+  if (tact && tact[by_ * tiles_x + bx_] == 0u) return;  // uniform per block
+  saddle_table_init(t);
+  constexpr int R = TSL / 4;
+  int32_t lv[R];
+  bool lt[R];
+  const int col = bx_ * 64 + threadIdx.x;
+  // no early exit inside the row loop: the whole wave (one row of 64 cells)
+  // takes part in the segmented minimum
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int row = by_ * TSL + threadIdx.y + 4 * r;
+    lv[r] = 0;
+    lt[r] = false;
+    if (row >= g.ny) continue;  // uniform per wave
+    const bool inside = col < g.nx;
+    const int i = inside ? row * g.nx + col : 0;
+    int keyb = -1;
+    int64_t res = INVALID_KEY;
+    if (inside) {
+      const float zi = z[i];
+      if (can_leave<MODE>(g, i, row, col)) {
+        zp[i] = zi;
+      } else {
+        lt[r] = zprime_cell<MODE>(g, i, row, col, zi, bid, z, isb, zp, keyb, res);
+        lv[r] = i;
+      }
+    }
+    saddle_offer(t, keyb, res, saddle);
+  }
+  if (blist) block_append_ordered<R>(lv, lt, blist, nblist);
+  saddle_table_flush(t, saddle);
+}
+
+// the same over the previous trip's border list: a border cell of this trip
+// was a border cell of the previous one (labels only merge), so the cells
+// that are still border cells of closed basins are the new list
+// R list entries per thread and chunk, one append per chunk: long lists take
+// large chunks (few appends on the shared counter), short ones small chunks
+// (enough blocks to fill the chip)
+template <int MODE, int R>
+__global__ __launch_bounds__(kBT) void k_zprime_list(GridD g, const int32_t* __restrict__ lin, const int64_t* nin,
+                                                     const int32_t* __restrict__ bid, const float* __restrict__ z,
+                                                     uint8_t* __restrict__ isb, float* __restrict__ zp,
+                                                     int64_t* saddle, int32_t* lout, int64_t* nout) {
+  __shared__ SaddleTable t;
+  saddle_table_init(t);
+  const int64_t n = *nin;
+  constexpr int64_t chunk = (int64_t)kBT * R;
+  for (int64_t base = (int64_t)blockIdx.x * chunk; base < n; base += (int64_t)gridDim.x * chunk) {
+    int32_t lv[R];
+    bool lt[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int64_t e = base + r * kBT + threadIdx.x;
+      lv[r] = 0;
+      lt[r] = false;
+      int keyb = -1;
+      int64_t res = INVALID_KEY;
+      if (e < n) {
+        const int i = lin[e];
+        int row, col;
+        rc_of(g, i, row, col);
+        lt[r] = zprime_cell<MODE>(g, i, row, col, z[i], bid, z, isb, zp, keyb, res);
+        lv[r] = i;
+      }
+      saddle_offer(t, keyb, res, saddle);
+    }
+    block_append_ordered<R>(lv, lt, lout, nout);
+  }
+  saddle_table_flush(t, saddle);
+}
+
+// pass 3 (:199-219) for one border cell of a closed basin: largest cell index wins
+template <int MODE>
+__device__ __forceinline__ void saddle_node_cell(const GridD& g, int i, int row, int col, int bi,
+                                                 const int32_t* __restrict__ bid, const float* __restrict__ zp,
+                                                 const int64_t* __restrict__ saddle, int32_t* snode) {
   const int64_t key = saddle[bi];
-  const float tz = key_float(key);
+  if (zp[i] != key_float(key)) return;
   const int tb = key_index(key);
-  if (zp[i] != tz) continue;
   bool here = false;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -575,7 +654,21 @@ __global__ __launch_bounds__(kBT) void k_saddle_node_t(GridD g, int tiles_x, con
     if (bid[jn] == tb) here = true;
   }
   if (here) atomicMax(&snode[bi], i);
-  PFF_CELL_LOOP_END
+}
+
+// over this trip's border list
+template <int MODE>
+__global__ __launch_bounds__(kBT) void k_saddle_node_list(GridD g, const int32_t* __restrict__ list,
+                                                          const int64_t* nlist, const int32_t* __restrict__ bid,
+                                                          const float* __restrict__ zp,
+                                                          const int64_t* __restrict__ saddle, int32_t* snode) {
+  const int64_t n = *nlist;
+  for (int64_t e = (int64_t)blockIdx.x * kBT + threadIdx.x; e < n; e += (int64_t)gridDim.x * kBT) {
+    const int i = list[e];
+    int row, col;
+    rc_of(g, i, row, col);
+    saddle_node_cell<MODE>(g, i, row, col, bid[i], bid, zp, saddle, snode);
+  }
 }
 
 // pass 4 (:226-250) over the pits
@@ -694,19 +787,48 @@ int launch_log_outlets(pff_ctx* ctx, const GridD& g, const int32_t* bid, const i
   return check_hip(hipGetLastError(), "log_outlets");
 }
 
+// Saddles and outlets of one trip on the tiled path.  *border_cur (host) is
+// the border list of the previous trip (0 or 1), or -1 for a full pass over
+// the active tiles (the first trip); on return it names this trip's list.
+// A list longer than half of the grid is also passed over in full (the
+// coalesced pass is cheaper than the gathers then); its length is the one the
+// last read of the device words saw.  Also zeroes the tag-list counter that
+// carve_tiled appends to.
 int saddlesort_active(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t* isb, float* zp, int64_t* saddle,
                       int32_t* snode, int64_t* outlet, const float* z, const uint32_t* tact, const int32_t* pits,
-                      const int64_t* npits) {
-  void* inv;
+                      const int64_t* npits, int* border_cur) {
+  void *inv, *bl0, *bl1;
   PFF_TRY(scratch(ctx, S_B0, (size_t)g.N, &inv));
+  PFF_TRY(scratch(ctx, S_BL0, (size_t)g.N * 4, &bl0));
+  PFF_TRY(scratch(ctx, S_BL1, (size_t)g.N * 4, &bl1));
+  int32_t* BL[2] = {(int32_t*)bl0, (int32_t*)bl1};
+  static_assert(W_NBL1 == W_NBL0 + 1, "border list counters are adjacent words");
+  int64_t* NB[2] = {ctx->dev_words + W_NBL0, ctx->dev_words + W_NBL1};
+  const int in = *border_cur;
+  const int out = in < 0 ? 0 : (in ^ 1);
   hipStream_t s = ctx->stream;
   const int tiles_x = (g.nx + TSL - 1) / TSL;
-  const dim3 tg = cell_grid(g), lg(1024), bl(kBT), cb(64, 4);
-  hipLaunchKernelGGL(k_saddle_init_list, lg, bl, 0, s, pits, npits, saddle, outlet, snode);
-  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_zprime_min_t<M>, tg, cb, 0, s, g, tiles_x, tact, bid, z, isb, zp,
-                                               saddle));
-  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_saddle_node_t<M>, tg, cb, 0, s, g, tiles_x, tact, bid,
-                                               (const uint8_t*)isb, (const float*)zp, (const int64_t*)saddle, snode));
+  const dim3 lg(1024), bl(kBT), cb(64, 4);
+  hipLaunchKernelGGL(k_saddle_init_list, lg, bl, 0, s, pits, npits, saddle, outlet, snode, NB[out],
+                     ctx->dev_words + W_NTAG);
+  const bool full = in < 0 || ctx->host_words[W_NBL0 + in] * 2 > (int64_t)g.N;
+  const dim3 lgz(2048);
+  if (full) {
+    const dim3 tt((g.nx + TSL - 1) / TSL, (g.ny + TSL - 1) / TSL);
+    PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_zprime_min_t<M>, tt, cb, 0, s, g, tiles_x, tact, bid, z, isb, zp,
+                                                 saddle, BL[out], NB[out]));
+  } else {
+    if (ctx->host_words[W_NBL0 + in] > (int64_t)1 << 21)
+      PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL((k_zprime_list<M, 16>), lgz, bl, 0, s, g, (const int32_t*)BL[in],
+                                                   (const int64_t*)NB[in], bid, z, isb, zp, saddle, BL[out], NB[out]));
+    else
+      PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL((k_zprime_list<M, 2>), lgz, bl, 0, s, g, (const int32_t*)BL[in],
+                                                   (const int64_t*)NB[in], bid, z, isb, zp, saddle, BL[out], NB[out]));
+  }
+  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_saddle_node_list<M>, lgz, bl, 0, s, g, (const int32_t*)BL[out],
+                                               (const int64_t*)NB[out], bid, (const float*)zp, (const int64_t*)saddle,
+                                               snode));
+  *border_cur = out;
   PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_outlet_list<M>, lg, bl, 0, s, g, pits, npits, bid, z,
                                                (const int64_t*)saddle, (const int32_t*)snode, outlet));
   hipLaunchKernelGGL(k_cycle_flag_list, lg, bl, 0, s, g.N, pits, npits, bid, (const int64_t*)outlet, (uint8_t*)inv,
@@ -716,16 +838,17 @@ int saddlesort_active(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t*
   return check_hip(hipGetLastError(), "saddlesort_active");
 }
 
-// tiled carve finalise (lakeflow.py:390-397): reverse tagged links
-__global__ __launch_bounds__(kBT) void k_carve_reverse_t(GridD g, int tiles_x, const uint32_t* __restrict__ tact,
-                                                         const uint8_t* __restrict__ tag,
-                                                         const int32_t* __restrict__ rec_in, int32_t* rec) {
-  PFF_CELL_LOOP_BEGIN(tact, tiles_x)
-  if (tag[i]) {
+// tiled carve finalise (lakeflow.py:390-397): reverse the tagged links, over
+// the tagged cells
+__global__ __launch_bounds__(kBT) void k_carve_reverse_list(int32_t N, const int32_t* __restrict__ list,
+                                                            const int64_t* nlist, const uint8_t* __restrict__ tag,
+                                                            const int32_t* __restrict__ rec_in, int32_t* rec) {
+  const int64_t n = *nlist;
+  for (int64_t e = (int64_t)blockIdx.x * kBT + threadIdx.x; e < n; e += (int64_t)gridDim.x * kBT) {
+    const int i = list[e];
     const int r = rec_in[i];
-    if (r != i && (unsigned)r < (unsigned)g.N && tag[r]) rec[r] = i;
+    if (r != i && (unsigned)r < (unsigned)N && tag[r]) rec[r] = i;
   }
-  PFF_CELL_LOOP_END
 }
 
 // saddle node -> outlet (lakeflow.py:399-403) over the pits
@@ -741,12 +864,13 @@ __global__ __launch_bounds__(kBT) void k_carve_link_list(const int32_t* __restri
 }
 
 // every changed receiver belongs to a tagged cell: copy those to the working copy
-__global__ __launch_bounds__(kBT) void k_carve_sync_t(GridD g, int tiles_x, const uint32_t* __restrict__ tact,
-                                                      const uint8_t* __restrict__ tag, const int32_t* __restrict__ rec,
-                                                      int32_t* __restrict__ rec_work) {
-  PFF_CELL_LOOP_BEGIN(tact, tiles_x)
-  if (tag[i]) rec_work[i] = rec[i];
-  PFF_CELL_LOOP_END
+__global__ __launch_bounds__(kBT) void k_carve_sync_list(const int32_t* __restrict__ list, const int64_t* nlist,
+                                                         const int32_t* __restrict__ rec, int32_t* __restrict__ rec_work) {
+  const int64_t n = *nlist;
+  for (int64_t e = (int64_t)blockIdx.x * kBT + threadIdx.x; e < n; e += (int64_t)gridDim.x * kBT) {
+    const int i = list[e];
+    rec_work[i] = rec[i];
+  }
 }
 
 // Carve for one trip with the tiled contraction: the cells the reference
@@ -758,19 +882,23 @@ __global__ __launch_bounds__(kBT) void k_carve_sync_t(GridD g, int tiles_x, cons
 int carve_tiled(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t* rec_in, const int32_t* bid,
                 const int32_t* snode, const int64_t* outlet, int32_t* rec_work, const uint32_t* tact,
                 const int32_t* pits, const int64_t* npits, int rounds, int32_t* cycle) {
-  void* tag;
+  void *tag, *tl;
   PFF_TRY(scratch(ctx, S_B1, (size_t)g.N, &tag));
-  // rec_in was proved a forest by this trip's labels: no cycle check needed
+  PFF_TRY(scratch(ctx, S_TLIST, (size_t)g.N * 4, &tl));
+  // rec_in was proved a forest by this trip's labels: no cycle check needed.
+  // The tiled passes also list the tagged cells (the counter was zeroed by
+  // this trip's saddlesort_active).
+  const int64_t* ntag = ctx->dev_words + W_NTAG;
   PFF_TRY(tiled_upstream(ctx, g, rec_in, 2, bid, snode, 0.0f, nullptr, nullptr, (uint8_t*)tag, tact, nullptr,
-                         rounds));
+                         rounds, (int32_t*)tl));
   *cycle = 0;
-  const int tiles_x = (g.nx + TSL - 1) / TSL;
-  const dim3 tg = cell_grid(g), bl(kBT), cb(64, 4);
-  hipLaunchKernelGGL(k_carve_reverse_t, tg, cb, 0, ctx->stream, g, tiles_x, tact, (const uint8_t*)tag, rec_in, rec);
+  const dim3 lg(2048), bl(kBT);
+  hipLaunchKernelGGL(k_carve_reverse_list, lg, bl, 0, ctx->stream, g.N, (const int32_t*)tl, ntag,
+                     (const uint8_t*)tag, rec_in, rec);
   hipLaunchKernelGGL(k_carve_link_list, dim3(1024), bl, 0, ctx->stream, pits, npits, outlet, snode, rec);
   if (rec_work)
-    hipLaunchKernelGGL(k_carve_sync_t, tg, cb, 0, ctx->stream, g, tiles_x, tact, (const uint8_t*)tag,
-                       (const int32_t*)rec, rec_work);
+    hipLaunchKernelGGL(k_carve_sync_list, lg, bl, 0, ctx->stream, (const int32_t*)tl, ntag, (const int32_t*)rec,
+                       rec_work);
   return check_hip(hipGetLastError(), "carve_tiled");
 }
 

@@ -56,6 +56,46 @@ __device__ __forceinline__ void block_append(const int32_t (&val)[ITEMS], const 
   __syncthreads();
 }
 
+// Append in (round, wave, lane) order with one atomicAdd per block: item r of
+// every thread is one round, so items a block visits row by row (one wave
+// per row) land in the list in row-major order, and items taken from a list
+// keep that list's order.  Must be reached by every thread.
+template <int R, int BT = kBT>
+__device__ __forceinline__ void block_append_ordered(const int32_t (&val)[R], const bool (&take)[R], int32_t* list,
+                                                     int64_t* count) {
+  constexpr int W = BT / 64;
+  __shared__ int off_s[R * W];
+  __shared__ int64_t base_s;
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  const int lane = tid & 63, wid = tid >> 6;
+  uint64_t mask[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    mask[r] = __ballot(take[r]);
+    if (lane == 0) off_s[r * W + wid] = __popcll(mask[r]);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int acc = 0;
+    for (int k = 0; k < R * W; ++k) {
+      const int c = off_s[k];
+      off_s[k] = acc;
+      acc += c;
+    }
+    base_s = acc ? (int64_t)atomicAdd((unsigned long long*)count, (unsigned long long)acc) : 0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (take[r]) {
+      const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask[r] >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)mask[r], 0u));
+      list[base_s + off_s[r * W + wid] + below] = val[r];
+    }
+  }
+  __syncthreads();
+}
+
 // Block-wide sum with one atomicAdd on a 64-bit counter.
 template <int BT = kBT>
 __device__ __forceinline__ void block_count(int mine, int64_t* counter) {

@@ -85,11 +85,13 @@ enum Slot : int {
   S_GF0,    // GraphFlood driver fields (one block, see graphflood_fields)
   S_BATCH,  // tile-canvas arrays of pff_batch_route (one block)
   S_BATCHT, // tile-canvas tables (per tile)
-  S_M0, S_M1, S_TPIT2  // basin-level relabel: label maps (by basin id) and the next pits list
+  S_M0, S_M1, S_TPIT2,  // basin-level relabel: label maps (by basin id) and the next pits list
+  S_BL0, S_BL1,         // border cells of closed basins, this trip's and the next trip's
+  S_TLIST               // cells tagged by the tiled carve
 };
 
 // device word indices (dev_words / host_words)
-enum Word : int { W_COUNT0 = 0, W_COUNT1, W_COUNT2, W_FLAG, W_FLAG2, W_CYCLE, W_NEXIT, W_NDEP, W_NPIT, W_LIVE, W_EMPTY, W_GFMAX, W_GFNAN, W_NPIT2, W_MCHG, W_EMPTY2, W_GFABOVE, W_GFSUM, W_NWORDS = 24 };
+enum Word : int { W_COUNT0 = 0, W_COUNT1, W_COUNT2, W_FLAG, W_FLAG2, W_CYCLE, W_NEXIT, W_NDEP, W_NPIT, W_LIVE, W_EMPTY, W_GFMAX, W_GFNAN, W_NPIT2, W_MCHG, W_EMPTY2, W_GFABOVE, W_GFSUM, W_NBL0, W_NBL1, W_NTAG, W_NWORDS = 24 };
 
 constexpr int kBlock = 256;
 inline unsigned grid_for(int64_t n, int block = kBlock, int64_t cap = 1 << 20) {
@@ -124,7 +126,7 @@ int launch_saddlesort(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t*
 // tiled contraction (forests; *cycle = 1 when the receivers are not a forest)
 int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, const int32_t* bid,
                    const int32_t* snode, float q0, float* Q, uint32_t* cnt, uint8_t* tag, const uint32_t* tact,
-                   int32_t* cycle, int rounds = 0);
+                   int32_t* cycle, int rounds = 0, int32_t* tag_list = nullptr);
 int tiled_labels(pff_ctx* ctx, const GridD& g, const int32_t* rec, const uint32_t* tact_in, int64_t* ndep,
                  int32_t* cycle, int* rounds_used = nullptr);
 int tiled_labels_write(pff_ctx* ctx, const GridD& g, int32_t* bid, const uint32_t* tact_in, uint32_t* tact_out);
@@ -146,7 +148,7 @@ int carve_tiled(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t* rec_i
                 const int32_t* pits, const int64_t* npits, int rounds, int32_t* cycle);
 int saddlesort_active(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t* isb, float* zp, int64_t* saddle,
                       int32_t* snode, int64_t* outlet, const float* z, const uint32_t* tact, const int32_t* pits,
-                      const int64_t* npits);
+                      const int64_t* npits, int* border_cur);
 int launch_reroute_jump(pff_ctx* ctx, const GridD& g, int32_t* rec, const int64_t* outlet);
 // append (b, outlet[b], saddlenode[b]) for every basin b of the current labels
 // (cells with bid[i] == i + 1) to the armed lake log; *dev_count counts them
