@@ -255,8 +255,9 @@ int reroute_core(pff_ctx* ctx, const GridD& d, int32_t* bid, int32_t* rec, int32
       if (st) st->n_iter++;
       // carve (:525-526): rec equals rec_ here; rec is edited, then copied to rec_
       int32_t cyc = 0;
-      PFF_TRY(carve_tiled(ctx, d, rec, rec_, bid, basin_saddlenode, outlet, rec_, tact, pits, npits, label_rounds,
-                          &cyc));
+      // the first trip's labels came from the tiled pass over rec_ (local ends stored)
+      PFF_TRY(carve_tiled(ctx, d, rec, rec_, bid, basin_saddlenode, outlet, rec_, tact, pits, npits, it == 0,
+                          label_rounds, &cyc));
       if (!cyc) continue;
       // not reachable (the labels proved rec_ a forest); fall back regardless
       if (bs) {

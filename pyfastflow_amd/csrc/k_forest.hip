@@ -392,6 +392,13 @@ struct TileArgs {
   // saddle tags: list of the tagged cells (nullable), each cell once
   int32_t* tlist;
   int64_t* n_tlist;
+  // saddle tags across lakeflow trips (nullable): tiles whose receivers the
+  // previous trip's carve changed; the local pass clears its tile's flag.
+  // A tile that is not flagged (or every tile, with ends_valid: the labels
+  // pass of the same trip wrote E) keeps the local ends in E, so a tile
+  // without a weight skips its rounds and reads them from there.
+  uint32_t* tdirty;
+  int ends_valid;
   unsigned long long* prof_cells;  // profiling: cells processed by this launch
 };
 
@@ -428,6 +435,16 @@ __global__ __launch_bounds__(kTB) void k_tile(TileArgs a) {
   int32_t cell[CPT];
   uint32_t rootmask = 0;
   int wany = 0;
+  __shared__ uint32_t dirty_s;
+  if constexpr (PH == 1 && W == W_SADDLE) {
+    if (threadIdx.x == 0) {
+      dirty_s = 1u;
+      if (a.tdirty) {
+        dirty_s = a.ends_valid ? 0u : a.tdirty[blockIdx.y * a.tiles_x + blockIdx.x];
+        a.tdirty[blockIdx.y * a.tiles_x + blockIdx.x] = 0u;
+      }
+    }
+  }
 #pragma unroll
   for (int j = 0; j < CPT; ++j) {
     const int li = threadIdx.x + j * kTB;
@@ -468,9 +485,15 @@ __global__ __launch_bounds__(kTB) void k_tile(TileArgs a) {
   // no weight there only needs its local ends, jumped on E alone until they
   // stop changing (uniform per block)
   bool eonly = false;
-  if constexpr (PH == 1 && W == W_SADDLE) eonly = !sums;
-  bool cyc = true;
-  for (int k = 0; k < kTileRounds; ++k) {
+  // a weightless tile whose receivers did not change since its local ends
+  // were stored takes them from E and runs no rounds (uniform per block)
+  bool stored = false;
+  if constexpr (PH == 1 && W == W_SADDLE) {
+    eonly = !sums;
+    stored = eonly && dirty_s == 0u;
+  }
+  bool cyc = !stored;
+  for (int k = 0; k < (stored ? 0 : kTileRounds); ++k) {
     uint16_t pm[CPT], pp[CPT], ee[CPT];
     uint32_t sv[CPT];
     int any = 0;
@@ -526,15 +549,21 @@ __global__ __launch_bounds__(kTB) void k_tile(TileArgs a) {
       pv[j] = cell[j] + 1;
       const int c = cell[j];
       if (c < 0) continue;
-      const int e = E[li];
-      a.E[c] = (y0 + (e >> 6)) * g.nx + x0 + (e & 63);
+      bool end;
+      if (stored) {
+        end = a.E[c] == c;
+      } else {
+        const int e = E[li];
+        a.E[c] = (y0 + (e >> 6)) * g.nx + x0 + (e & 63);
+        end = e == li;
+      }
       if constexpr (W != W_ROOTS) {
         a.S[c] = S[li];
         a.inflow[c] = 0u;
       }
       if constexpr (W == W_SADDLE) a.tag[c] = S[li] ? 1 : 0;  // final unless the tile gets a marked inflow
       // (the tagged cells are listed below)
-      if (e == li) {  // local end
+      if (end) {  // local end
         if (rootmask & (1u << j)) {
           if constexpr (W == W_ROOTS) {
             const int row = y0 + (li >> 6), col = x0 + (li & 63);
@@ -623,6 +652,19 @@ __global__ __launch_bounds__(kBT) void k_exit_link(TileArgs a, const int32_t* __
   }
 }
 
+// tiles whose receivers this trip's carve changes: those of the tagged cells
+__global__ __launch_bounds__(kBT) void k_mark_dirty(int32_t N, int32_t nx, int tiles_x,
+                                                    const int32_t* __restrict__ list, const int64_t* nlist,
+                                                    uint32_t* tdirty) {
+  const int64_t n = *nlist;
+  for (int64_t w = (int64_t)blockIdx.x * kBT + threadIdx.x; w < n; w += (int64_t)gridDim.x * kBT) {
+    const int c = list[w];
+    if (c < 0 || c >= N) continue;
+    const int row = c / nx;
+    tdirty[(row / TS) * tiles_x + (c - row * nx) / TS] = 1u;
+  }
+}
+
 // inflow of entry cells: sum of the totals of the exits draining into them;
 // with tin, flags the tiles that receive a nonzero inflow
 __global__ __launch_bounds__(kBT) void k_exit_inflow(TileArgs a, const int32_t* __restrict__ exits,
@@ -667,7 +709,7 @@ __global__ __launch_bounds__(kBT) void k_label_tiles(TileArgs a, int32_t* __rest
 
 struct TileBufs {
   int32_t *E, *P2, *bid2, *exits, *L0, *L1, *t0, *t1, *t2, *pits;
-  uint32_t *S, *inflow, *tact, *tin;
+  uint32_t *S, *inflow, *tact, *tin, *tdirty;
 };
 
 static int tile_bufs(pff_ctx* ctx, const GridD& g, TileBufs& b) {
@@ -686,8 +728,9 @@ static int tile_bufs(pff_ctx* ctx, const GridD& g, TileBufs& b) {
   PFF_TRY(scratch(ctx, S_TIN, n4, &p)); b.inflow = (uint32_t*)p;
   PFF_TRY(scratch(ctx, S_TPIT, n4, &p)); b.pits = (int32_t*)p;
   const size_t nt = (size_t)((g.nx + TS - 1) / TS) * ((g.ny + TS - 1) / TS);
-  PFF_TRY(scratch(ctx, S_TACT, nt * 4 * 3, &p)); b.tact = (uint32_t*)p;
+  PFF_TRY(scratch(ctx, S_TACT, nt * 4 * 4, &p)); b.tact = (uint32_t*)p;
   b.tin = b.tact + 2 * nt;
+  b.tdirty = b.tact + 3 * nt;
   return 0;
 }
 
@@ -724,13 +767,18 @@ int tiles_count(const GridD& g) { return ((g.nx + TS - 1) / TS) * ((g.ny + TS - 
 // graph that is not a forest (outputs are then not meaningful).
 int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, const int32_t* bid,
                    const int32_t* snode, float q0, float* Q, uint32_t* cnt, uint8_t* tag, const uint32_t* tact,
-                   int32_t* cycle, int rounds, int32_t* tag_list) {
+                   int32_t* cycle, int rounds, int32_t* tag_list, int ends_valid) {
   TileBufs b;
   PFF_TRY(tile_bufs(ctx, g, b));
   TileArgs a = tile_args(g, b, ctx, rcv);
   if (tag_list) {
     a.tlist = tag_list;
     a.n_tlist = ctx->dev_words + W_NTAG;
+  }
+  const bool stored_ends = wmode == W_SADDLE && tag_list && ends_valid >= 0;
+  if (stored_ends) {
+    a.tdirty = b.tdirty;
+    a.ends_valid = ends_valid;
   }
   a.bid = bid;
   a.snode = snode;
@@ -794,6 +842,11 @@ int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, 
   else hipLaunchKernelGGL((k_tile<0, 3, W_SADDLE>), tg, dim3(kTB), 0, s, a);
   prof_end(ctx);
   a.prof_cells = nullptr;
+  // the next trip's local pass recomputes the local ends where this carve
+  // changes receivers
+  if (stored_ends)
+    hipLaunchKernelGGL(k_mark_dirty, dim3(list_grid()), dim3(kBT), 0, s, g.N, g.nx, a.tiles_x, (const int32_t*)tag_list,
+                       (const int64_t*)a.n_tlist, b.tdirty);
   return check_hip(hipGetLastError(), "tiled_upstream");
 }
 

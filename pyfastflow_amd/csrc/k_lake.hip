@@ -881,7 +881,7 @@ __global__ __launch_bounds__(kBT) void k_carve_sync_list(const int32_t* __restri
 // (nothing is written then).
 int carve_tiled(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t* rec_in, const int32_t* bid,
                 const int32_t* snode, const int64_t* outlet, int32_t* rec_work, const uint32_t* tact,
-                const int32_t* pits, const int64_t* npits, int rounds, int32_t* cycle) {
+                const int32_t* pits, const int64_t* npits, bool first, int rounds, int32_t* cycle) {
   void *tag, *tl;
   PFF_TRY(scratch(ctx, S_B1, (size_t)g.N, &tag));
   PFF_TRY(scratch(ctx, S_TLIST, (size_t)g.N * 4, &tl));
@@ -889,8 +889,10 @@ int carve_tiled(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t* rec_i
   // The tiled passes also list the tagged cells (the counter was zeroed by
   // this trip's saddlesort_active).
   const int64_t* ntag = ctx->dev_words + W_NTAG;
+  // local ends: the first trip's labels pass stored them for rec_in; later
+  // trips keep them where the previous carve changed nothing
   PFF_TRY(tiled_upstream(ctx, g, rec_in, 2, bid, snode, 0.0f, nullptr, nullptr, (uint8_t*)tag, tact, nullptr,
-                         rounds, (int32_t*)tl));
+                         rounds, (int32_t*)tl, first ? 1 : 0));
   *cycle = 0;
   const dim3 lg(2048), bl(kBT);
   hipLaunchKernelGGL(k_carve_reverse_list, lg, bl, 0, ctx->stream, g.N, (const int32_t*)tl, ntag,
