@@ -42,6 +42,7 @@ int check_grid(const pff_grid* g, GridD* out) {
 }
 
 int scratch(pff_ctx* ctx, int slot, size_t bytes, void** out, bool zero_new) {
+  if (slot < 0 || slot >= S_NSLOTS) return fail(PFF_EINVAL, "scratch slot out of range");
   if (bytes == 0) bytes = 4;
   if (ctx->cap[slot] < bytes) {
     if (ctx->buf[slot]) {

@@ -13,8 +13,8 @@
 struct pff_ctx {
   int32_t device = 0;
   hipStream_t stream = nullptr;
-  // growable device scratch, keyed by slot
-  static constexpr int kSlots = 32;
+  // growable device scratch, keyed by slot (enum Slot below; checked there)
+  static constexpr int kSlots = 48;
   void* buf[kSlots] = {};
   size_t cap[kSlots] = {};
   // pinned host words for counter readback
@@ -87,11 +87,14 @@ enum Slot : int {
   S_BATCHT, // tile-canvas tables (per tile)
   S_M0, S_M1, S_TPIT2,  // basin-level relabel: label maps (by basin id) and the next pits list
   S_BL0, S_BL1,         // border cells of closed basins, this trip's and the next trip's
-  S_TLIST               // cells tagged by the tiled carve
+  S_TLIST,              // cells tagged by the tiled carve
+  S_NSLOTS
 };
+static_assert(S_NSLOTS <= pff_ctx::kSlots, "pff_ctx::kSlots must cover every scratch slot");
 
 // device word indices (dev_words / host_words)
-enum Word : int { W_COUNT0 = 0, W_COUNT1, W_COUNT2, W_FLAG, W_FLAG2, W_CYCLE, W_NEXIT, W_NDEP, W_NPIT, W_LIVE, W_EMPTY, W_GFMAX, W_GFNAN, W_NPIT2, W_MCHG, W_EMPTY2, W_GFABOVE, W_GFSUM, W_NBL0, W_NBL1, W_NTAG, W_NWORDS = 24 };
+enum Word : int { W_COUNT0 = 0, W_COUNT1, W_COUNT2, W_FLAG, W_FLAG2, W_CYCLE, W_NEXIT, W_NDEP, W_NPIT, W_LIVE, W_EMPTY, W_GFMAX, W_GFNAN, W_NPIT2, W_MCHG, W_EMPTY2, W_GFABOVE, W_GFSUM, W_NBL0, W_NBL1, W_NTAG, W_LAST_WORD, W_NWORDS = 24 };
+static_assert(W_LAST_WORD <= W_NWORDS, "device word block too small");
 
 constexpr int kBlock = 256;
 inline unsigned grid_for(int64_t n, int block = kBlock, int64_t cap = 1 << 20) {
