@@ -971,6 +971,40 @@ __global__ __launch_bounds__(kBT) void k_basin_next(const int32_t* __restrict__ 
   }
 }
 
+// Chain ends by chasing: every basin follows M until a fixed point (an
+// uncarved basin keeps M[b] = b; 0 is the open edge, and an id out of range
+// leads there too), storing each step into its own entry so that later
+// chasers take longer strides.  An entry only ever moves further along its
+// own chain, so a chaser that reads an entry another thread is updating
+// stays on its chain.  A basin not done after max_steps (a cycle, or a very
+// long chain) sets *unfinished; the synchronous rounds then finish from here.
+__global__ __launch_bounds__(kBT) void k_basin_chase(const int32_t* __restrict__ pits, const int64_t* np, int32_t N,
+                                                     int32_t* M, int max_steps, int64_t* unfinished) {
+  const int64_t n = *np;
+  for (int64_t w = (int64_t)blockIdx.x * kBT + threadIdx.x; w < n; w += (int64_t)gridDim.x * kBT) {
+    const int b = pits[w];
+    if (b <= 0 || b >= N) continue;
+    int m = M[b];
+    bool done = false;
+    for (int k = 0; k < max_steps; ++k) {
+      if (m <= 0 || m >= N) {
+        m = 0;
+        done = true;
+        break;
+      }
+      const int mm = M[m];
+      if (mm == m) {
+        done = true;
+        break;
+      }
+      m = mm;
+      M[b] = m;
+    }
+    M[b] = m;
+    if (!done) *unfinished = 1;
+  }
+}
+
 // one synchronous pointer-jumping round Mout[b] = Min[Min[b]] over the pits;
 // *chg = the latest round (1-based) in which a label changed
 __global__ __launch_bounds__(kBT) void k_basin_jump(const int32_t* __restrict__ pits, const int64_t* np, int32_t N,
@@ -1045,12 +1079,20 @@ int relabel_from_outlets(pff_ctx* ctx, const GridD& g, int32_t* bid, const int64
   const unsigned lg = grid_for(std::max<int64_t>(n_prev, 1), kBT, 4096);
   hipLaunchKernelGGL(k_basin_next, dim3(lg), dim3(kBT), 0, s, (const int32_t*)b.pits, (const int64_t*)npit, outlet,
                      (const int32_t*)bid, g.N, M[0], M[1], next_pits, n_next);
+  // chase the chains to their ends in one pass; the synchronous rounds below
+  // finish what a cycle or a very long chain leaves unfinished
+  hipLaunchKernelGGL(k_basin_chase, dim3(lg), dim3(kBT), 0, s, (const int32_t*)b.pits, (const int64_t*)npit, g.N,
+                     M[0], 64, chg);
+  PFF_TRY(check_hip(hipGetLastError(), "basin_chase"));
+  PFF_TRY(read_words(ctx));
+  const bool chased = ctx->host_words[W_MCHG] == 0;
+  if (!chased) PFF_TRY(check_hip(hipMemsetAsync(chg, 0, 8, s), "memset"));
   // a chain of basins is at most n_prev long: ceil(log2 n_prev) + 1 rounds
   // reach every end; a label still changing then means a cycle
   const int cap = ceil_log2i(std::max<int64_t>(n_prev, 2)) + 1;
   int cur = 0, done = 0;
   int step = std::max(2, std::min(cap + 1, ctx->last_jump_rounds));
-  while (true) {
+  while (!chased) {
     for (int k = 0; k < step; ++k) {
       hipLaunchKernelGGL(k_basin_jump, dim3(lg), dim3(kBT), 0, s, (const int32_t*)b.pits, (const int64_t*)npit, g.N,
                          (const int32_t*)M[cur], M[cur ^ 1], done + k + 1, chg);

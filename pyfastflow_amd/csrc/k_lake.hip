@@ -463,7 +463,7 @@ __global__ __launch_bounds__(kBT) void k_saddle_init_list(const int32_t* __restr
 // global atomic per basin and block at the flush) or, when the table has no
 // room for the basin, straight to a global atomicMin after a plain read shows
 // that it can lower the stored minimum.
-constexpr int kSaddleSlots = 512;
+constexpr int kSaddleSlots = 1024;  // a 64x64 tile holds ~550 basins in the first trip at 4096^2
 
 struct SaddleTable {
   int32_t b[kSaddleSlots];
@@ -492,7 +492,7 @@ __device__ __forceinline__ void saddle_offer(SaddleTable& t, int keyb, int64_t r
   const int prevb = __shfl_up(keyb, 1);
   if (keyb >= 0 && (lane == 0 || prevb != keyb)) {
     bool done = false;
-    int slot = (int)(((uint32_t)keyb * 2654435761u) >> 23);  // 9-bit hash
+    int slot = (int)(((uint32_t)keyb * 2654435761u) >> 22);  // 10-bit hash
     for (int probe = 0; probe < 8 && !done; ++probe, slot = (slot + 1) & (kSaddleSlots - 1)) {
       const int prev = atomicCAS(&t.b[slot], -1, keyb);
       if (prev == -1 || prev == keyb) {
