@@ -1029,22 +1029,26 @@ __global__ __launch_bounds__(kBT) void k_basin_jump(const int32_t* __restrict__ 
   }
 }
 
-// bid <- M[bid] on the tiles of tact_in; flags the tiles with a closed basin
+// bid <- M[bid] on the tiles of tact_in; tact_out[t] = 1 for the tiles that
+// still hold a closed basin, 0 for the others (one block per 64x64 tile, so
+// every flag is written and none needs clearing first)
 __global__ __launch_bounds__(kBT) void k_relabel_tiles(GridD g, int tiles_x, const uint32_t* __restrict__ tact_in,
                                                        const int32_t* __restrict__ M, int32_t* __restrict__ bid,
                                                        uint32_t* __restrict__ tact_out, int64_t* npit,
                                                        const int64_t* n_next) {
   if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) *npit = *n_next;
-  // 64 columns x 16 rows per block, four rows per thread
   int bx, by;
   xcd_block(bx, by);
-  const int t = ((by * 16) / TS) * tiles_x + bx;
-  if (tact_in && tact_in[t] == 0u) return;  // uniform per block
-  const int col = bx * 64 + threadIdx.x;
+  const int t = by * tiles_x + bx;
+  if (tact_in && tact_in[t] == 0u) {  // uniform per block
+    if (threadIdx.x == 0 && threadIdx.y == 0) tact_out[t] = 0u;
+    return;
+  }
+  const int col = bx * TS + threadIdx.x;
   int any = 0;
   if (col < g.nx)
-    for (int rr = threadIdx.y; rr < 16; rr += 4) {
-      const int row = by * 16 + rr;
+    for (int rr = threadIdx.y; rr < TS; rr += 4) {
+      const int row = by * TS + rr;
       if (row >= g.ny) break;
       const int c = row * g.nx + col;
       const int b = bid[c];
@@ -1055,7 +1059,7 @@ __global__ __launch_bounds__(kBT) void k_relabel_tiles(GridD g, int tiles_x, con
       }
     }
   any = __syncthreads_or(any);
-  if (any && threadIdx.x == 0 && threadIdx.y == 0) atomicOr(&tact_out[t], 1u);
+  if (threadIdx.x == 0 && threadIdx.y == 0) tact_out[t] = any ? 1u : 0u;
 }
 
 int relabel_from_outlets(pff_ctx* ctx, const GridD& g, int32_t* bid, const int64_t* outlet, int64_t n_prev,
@@ -1113,10 +1117,8 @@ int relabel_from_outlets(pff_ctx* ctx, const GridD& g, int32_t* bid, const int64
     step = std::min(4, cap + 1 - done);
   }
   const int64_t n2 = ctx->host_words[W_NPIT2];
-  PFF_TRY(check_hip(hipMemsetAsync(tact_out, 0, (size_t)tiles_count(g) * 4, s), "memset"));
-  hipLaunchKernelGGL(k_relabel_tiles, dim3((g.nx + 63) / 64, (g.ny + 15) / 16), dim3(64, 4), 0, s, g,
-                     (g.nx + TS - 1) / TS, tact_in, (const int32_t*)M[cur], bid, tact_out, npit,
-                     (const int64_t*)n_next);
+  hipLaunchKernelGGL(k_relabel_tiles, tile_grid(g), dim3(64, 4), 0, s, g, (g.nx + TS - 1) / TS, tact_in,
+                     (const int32_t*)M[cur], bid, tact_out, npit, (const int64_t*)n_next);
   PFF_TRY(check_hip(hipGetLastError(), "relabel_tiles"));
   // the uncarved basins become the pits list: swap the two list buffers
   std::swap(ctx->buf[S_TPIT], ctx->buf[S_TPIT2]);
