@@ -140,7 +140,9 @@ int pff_reroute_flow(pff_ctx* ctx, const pff_grid* g, int32_t* bid, int32_t* rec
 /* Arms a log of the lake outlets for the next pff_reroute_flow call on this
    context (then disarms; that call syncs).  After each trip's saddle and
    cycle passes (lakeflow.py:520, the state after :255-281), every basin b of
-   the trip appends (b, outlet[b], basin_saddlenode[b]) to the device arrays
+   the trip (the labels held by their own pit; once the receivers contain a
+   cycle, every slot with an outlet or a saddle node) appends
+   (b, outlet[b], basin_saddlenode[b]) to the device arrays
    basin/outlet/saddlenode (capacity entries; entries past it are counted,
    not written).  trip_counts (host, max_trips) receives the cumulative
    number of entries at the end of each trip, -1 for trips not run.  No

@@ -140,10 +140,10 @@ int accumulate_core(pff_ctx* ctx, const GridD& d, const int32_t* rcv, float q0, 
 
 // armed lake-outlet log: entries of this trip, cumulative count per trip
 static int log_trip(pff_ctx* ctx, const GridD& d, const int32_t* bid, const int64_t* outlet, const int32_t* snode,
-                    int trip) {
+                    int trip, bool all_slots) {
   auto& L = ctx->lake_log;
   if (!L.armed || trip >= L.max_trips) return 0;
-  PFF_TRY(launch_log_outlets(ctx, d, bid, outlet, snode, L.dev_count));
+  PFF_TRY(launch_log_outlets(ctx, d, bid, outlet, snode, L.dev_count, all_slots));
   return check_hip(hipMemcpyAsync(L.trip_counts + trip, L.dev_count, 8, hipMemcpyDeviceToHost, ctx->stream), "d2h");
 }
 
@@ -252,7 +252,7 @@ int reroute_core(pff_ctx* ctx, const GridD& d, int32_t* bid, int32_t* rec, int32
       PFF_TRY(saddlesort_active(ctx, d, bid, is_border, z_prime, basin_saddle, basin_saddlenode, outlet, z, tact, pits,
                                 npits, &border_cur));
       if (bs) PFF_TRY(batch_mark42(ctx, bs, d, bid, 0));
-      PFF_TRY(log_trip(ctx, d, bid, outlet, basin_saddlenode, st ? st->n_iter : it));
+      PFF_TRY(log_trip(ctx, d, bid, outlet, basin_saddlenode, st ? st->n_iter : it, false));
       if (st) st->n_iter++;
       // carve (:525-526): rec equals rec_ here; rec is edited, then copied to rec_
       int32_t cyc = 0;
@@ -271,7 +271,7 @@ int reroute_core(pff_ctx* ctx, const GridD& d, int32_t* bid, int32_t* rec, int32
       PFF_TRY(launch_saddlesort(ctx, d, bid, is_border, z_prime, basin_saddle, basin_saddlenode, outlet, z));
     } else {
       PFF_TRY(launch_saddlesort(ctx, d, bid, is_border, z_prime, basin_saddle, basin_saddlenode, outlet, z));
-      PFF_TRY(log_trip(ctx, d, bid, outlet, basin_saddlenode, st ? st->n_iter : it));
+      PFF_TRY(log_trip(ctx, d, bid, outlet, basin_saddlenode, st ? st->n_iter : it, true));
       if (st) st->n_iter++;
     }
     if (carve) {

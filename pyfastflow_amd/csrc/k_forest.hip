@@ -204,7 +204,7 @@ __global__ __launch_bounds__(kBT) void k_label_scatter2(const int32_t* __restric
                                                         int32_t* bid, int32_t* P, const int32_t* __restrict__ nb,
                                                         const int32_t* __restrict__ np, int32_t* __restrict__ out,
                                                         int64_t* next_count, int32_t* __restrict__ level_next,
-                                                        int32_t* __restrict__ fin, int32_t round) {
+                                                        int32_t* __restrict__ fin, int32_t round, int64_t* self_loop) {
   const int64_t n = *count;
   for (int64_t base = (int64_t)blockIdx.x * kChunk; base < n; base += (int64_t)gridDim.x * kChunk) {
     int32_t v[kItems];
@@ -223,6 +223,9 @@ __global__ __launch_bounds__(kBT) void k_label_scatter2(const int32_t* __restric
         P[m] = q;
         if (level_next) level_next[m] = q;
         if (done && fin) fin[m] = round + 1;
+        // a node that jumps onto itself lies on a cycle (on a forest only
+        // roots point to themselves); the lists keep such a node live
+        if (!done && q == m && self_loop) *self_loop = 1;
         v[j] = m;
         take[j] = !done;
       }
@@ -258,14 +261,15 @@ static int count_rounds(pff_ctx* ctx, Lists& L, uint32_t* S, int32_t* P, int32_t
 }
 
 static int label_rounds_run(pff_ctx* ctx, Lists& L, int32_t* bid, int32_t* P, int32_t* nb, int32_t* np,
-                            int32_t* levels, int32_t N, int32_t* fin, int first_round, int rounds) {
+                            int32_t* levels, int32_t N, int32_t* fin, int first_round, int rounds,
+                            int64_t* self_loop = nullptr) {
   for (int k = first_round; k < first_round + rounds; ++k) {
     const int a = L.cur, b = 1 - L.cur;
     hipLaunchKernelGGL(k_label_gather2, dim3(list_grid()), dim3(kBT), 0, ctx->stream, L.l[a], L.c[a], bid, P, nb, np,
                        L.c[b], ctx->dev_words + W_EMPTY, k);
     int32_t* lvl = levels ? levels + (size_t)(k + 1) * N : nullptr;
     hipLaunchKernelGGL(k_label_scatter2, dim3(list_grid()), dim3(kBT), 0, ctx->stream, L.l[a], L.c[a], bid, P, nb,
-                       np, L.l[b], L.c[b], lvl, fin, k);
+                       np, L.l[b], L.c[b], lvl, fin, k, self_loop);
     L.cur = b;
   }
   return check_hip(hipGetLastError(), "label rounds");
@@ -886,7 +890,8 @@ int tiled_labels(pff_ctx* ctx, const GridD& g, const int32_t* rec, const uint32_
   int step = std::max(2, std::min(ctx->last_label_rounds + 1, cap));
   while (done_rounds < cap) {
     const int r = std::min(step, cap - done_rounds);
-    PFF_TRY(label_rounds_run(ctx, L, b.bid2, b.P2, b.t0, b.t1, nullptr, g.N, nullptr, done_rounds, r));
+    PFF_TRY(label_rounds_run(ctx, L, b.bid2, b.P2, b.t0, b.t1, nullptr, g.N, nullptr, done_rounds, r,
+                             ctx->dev_words + W_CYCLE));
     done_rounds += r;
     step = 3;
     // one readback: live count (a cycle across tiles), in-tile cycle flag, pits
@@ -976,8 +981,11 @@ __global__ __launch_bounds__(kBT) void k_basin_next(const int32_t* __restrict__ 
 // leads there too), storing each step into its own entry so that later
 // chasers take longer strides.  An entry only ever moves further along its
 // own chain, so a chaser that reads an entry another thread is updating
-// stays on its chain.  A basin not done after max_steps (a cycle, or a very
-// long chain) sets *unfinished; the synchronous rounds then finish from here.
+// stays on its chain.  A chaser never stores its own id into its entry (a
+// chain that comes back to it is a cycle), so every fixed point it meets is
+// a true one.  A basin on or feeding a cycle, or on a chain longer than
+// max_steps, sets *unfinished; the synchronous rounds then finish from here
+// (and report the cycle).
 __global__ __launch_bounds__(kBT) void k_basin_chase(const int32_t* __restrict__ pits, const int64_t* np, int32_t N,
                                                      int32_t* M, int max_steps, int64_t* unfinished) {
   const int64_t n = *np;
@@ -985,10 +993,11 @@ __global__ __launch_bounds__(kBT) void k_basin_chase(const int32_t* __restrict__
     const int b = pits[w];
     if (b <= 0 || b >= N) continue;
     int m = M[b];
+    if (m == b) continue;  // uncarved: a fixed point
     bool done = false;
     for (int k = 0; k < max_steps; ++k) {
       if (m <= 0 || m >= N) {
-        m = 0;
+        M[b] = 0;
         done = true;
         break;
       }
@@ -997,10 +1006,10 @@ __global__ __launch_bounds__(kBT) void k_basin_chase(const int32_t* __restrict__
         done = true;
         break;
       }
+      if (mm == b) break;  // the chain comes back to b: a cycle
       m = mm;
       M[b] = m;
     }
-    M[b] = m;
     if (!done) *unfinished = 1;
   }
 }
@@ -1026,6 +1035,23 @@ __global__ __launch_bounds__(kBT) void k_basin_jump(const int32_t* __restrict__ 
     // rounds run in order, so a plain store of this round's number is the
     // latest change; one store per block that saw a change
     if (__syncthreads_or(changed) && threadIdx.x == 0) *chg = round;
+  }
+}
+
+// Synchronous jumping on a cycle can settle on an entry that points to
+// itself (a cycle of length 2^k becomes the identity after k rounds, and the
+// chase leaves mixed strides that can do the same sooner): a true chain end
+// is the open edge or a basin without an outlet.  *bad = 1 if some basin
+// ends elsewhere.
+__global__ __launch_bounds__(kBT) void k_basin_check_ends(const int32_t* __restrict__ pits, const int64_t* np,
+                                                          int32_t N, const int32_t* __restrict__ M,
+                                                          const int64_t* __restrict__ outlet, int64_t* bad) {
+  const int64_t n = *np;
+  for (int64_t w = (int64_t)blockIdx.x * kBT + threadIdx.x; w < n; w += (int64_t)gridDim.x * kBT) {
+    const int b = pits[w];
+    if (b <= 0 || b >= N) continue;
+    const int e = M[b];
+    if (e > 0 && e < N && outlet[e] != INVALID_KEY) *bad = 1;
   }
 }
 
@@ -1105,9 +1131,19 @@ int relabel_from_outlets(pff_ctx* ctx, const GridD& g, int32_t* bid, const int64
     done += step;
     PFF_TRY(check_hip(hipGetLastError(), "basin_jump"));
     PFF_TRY(read_words(ctx));
-    // converged when the last round changed nothing
+    // converged when the last round changed nothing; the ends must then be
+    // true ends, or the chains run round a cycle
     if (ctx->host_words[W_MCHG] < done) {
       ctx->last_jump_rounds = std::max<int>((int)ctx->host_words[W_MCHG] + 1, 1);
+      PFF_TRY(check_hip(hipMemsetAsync(chg, 0, 8, s), "memset"));
+      hipLaunchKernelGGL(k_basin_check_ends, dim3(lg), dim3(kBT), 0, s, (const int32_t*)b.pits, (const int64_t*)npit,
+                         g.N, (const int32_t*)M[cur], outlet, chg);
+      PFF_TRY(check_hip(hipGetLastError(), "basin_check_ends"));
+      PFF_TRY(read_words(ctx));
+      if (ctx->host_words[W_MCHG] != 0) {
+        *cycle = 1;
+        return 0;
+      }
       break;
     }
     if (done >= cap + 1) {

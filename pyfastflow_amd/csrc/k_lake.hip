@@ -81,7 +81,11 @@ __global__ void k_zprime(GridD g, const int32_t* __restrict__ bid, const float* 
   if (i >= g.N) return;
   int row, col;
   rc_of(g, i, row, col);
-  if (can_leave<MODE>(g, i, row, col)) {  // is_border keeps its value (never set for these cells)
+  if (can_leave<MODE>(g, i, row, col)) {
+    // the reference never sets is_border here, so its zero-initialised
+    // field reads False; the flag is written so that a reused buffer reads
+    // the same
+    isb[i] = 0;
     zp[i] = z[i];
     return;
   }
@@ -346,15 +350,29 @@ __global__ __launch_bounds__(kBT) void k_carve_round(int32_t N, const int32_t* _
   }
 }
 
-// finalise_reroute_carve (:390-397): reverse every tagged link i -> rec_in[i]
-__global__ void k_carve_reverse(int32_t N, const int32_t* __restrict__ list, const int64_t* w,
-                                const uint32_t* __restrict__ tagw,
-                                const int32_t* __restrict__ rec_in, int32_t* rec) {
+// finalise_reroute_carve (:390-397): reverse every tagged link i -> rec_in[i].
+// On a receiver graph with a cycle a tagged cell can have two tagged donors;
+// the reference's parallel loop then keeps one of them, the rule here is the
+// largest donor index (the last writer in index order).  Pass 1 records the
+// largest donor + 2 in the tag word (tagged words are nonzero either way),
+// pass 2 writes it.
+__global__ void k_carve_reverse(int32_t N, const int32_t* __restrict__ list, const int64_t* w, uint32_t* tagw,
+                                const int32_t* __restrict__ rec_in) {
   const int64_t n = w[0];
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
     const int i = list[e];
     const int r = rec_in[i];
-    if (r != i && (unsigned)r < (unsigned)N && tagw[r]) rec[r] = i;
+    if (r != i && (unsigned)r < (unsigned)N && tagw[r]) atomicMax(&tagw[r], (uint32_t)i + 2u);
+  }
+}
+
+__global__ void k_carve_reverse_apply(const int32_t* __restrict__ list, const int64_t* w,
+                                      const uint32_t* __restrict__ tagw, int32_t* rec) {
+  const int64_t n = w[0];
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = list[e];
+    const uint32_t v = tagw[i];
+    if (v >= 2u) rec[i] = (int32_t)(v - 2u);
   }
 }
 
@@ -404,7 +422,9 @@ int carve_from_levels(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t*
                        (uint32_t*)tagw, (int32_t*)list, w, t);
   }
   hipLaunchKernelGGL(k_carve_reverse, dim3(stride_grid()), dim3(kBlock), 0, s, g.N, (const int32_t*)list, w,
-                     (const uint32_t*)tagw, rec_in, rec);
+                     (uint32_t*)tagw, rec_in);
+  hipLaunchKernelGGL(k_carve_reverse_apply, dim3(stride_grid()), dim3(kBlock), 0, s, (const int32_t*)list, w,
+                     (const uint32_t*)tagw, rec);
   hipLaunchKernelGGL(k_carve_link, dim3(grid_for(g.N)), dim3(kBlock), 0, s, g.N, outlet, snode, rec);
   hipLaunchKernelGGL(k_carve_sync, dim3(stride_grid()), dim3(kBlock), 0, s, (const int32_t*)list, w,
                      (uint32_t*)tagw, rec, rec_work, tag_bytes);
@@ -587,6 +607,7 @@ __global__ __launch_bounds__(kBT) void k_zprime_min_t(GridD g, int tiles_x, cons
     if (inside) {
       const float zi = z[i];
       if (can_leave<MODE>(g, i, row, col)) {
+        isb[i] = 0;
         zp[i] = zi;
       } else {
         lt[r] = zprime_cell<MODE>(g, i, row, col, zi, bid, z, isb, zp, keyb, res);
@@ -748,19 +769,30 @@ __global__ __launch_bounds__(kBT) void k_cycle_apply_list(int32_t N, const int32
   }
 }
 
-// lake-outlet log: one entry per basin of the trip (its pit holds its own id)
+// lake-outlet log: one entry per basin of the trip.  On a forest every basin
+// is the label of its pit, which holds its own id; with all_slots (receivers
+// with a cycle, where labels of cycle cells need not be held by their own
+// cell, and every basin slot was reset this trip) every slot with an outlet
+// or a saddle node is a basin.
 __global__ __launch_bounds__(kBT) void k_log_outlets(int32_t N, const int32_t* __restrict__ bid,
                                                      const int64_t* __restrict__ outlet,
                                                      const int32_t* __restrict__ snode, int32_t* __restrict__ lb,
                                                      int64_t* __restrict__ lo, int32_t* __restrict__ ls,
-                                                     int64_t capacity, int64_t* count) {
+                                                     int64_t capacity, int64_t* count, int all_slots) {
   for (int64_t base = (int64_t)blockIdx.x * kBT; base < N; base += (int64_t)gridDim.x * kBT) {
     const int64_t i = base + threadIdx.x;
     int32_t v[1] = {0};
     bool take[1] = {false};
-    if (i < N && bid[i] == (int32_t)i + 1) {
-      v[0] = (int32_t)i + 1;
-      take[0] = true;
+    if (i < N) {
+      if (all_slots) {
+        if (outlet[i] != INVALID_KEY || snode[i] != -1) {
+          v[0] = (int32_t)i;
+          take[0] = true;
+        }
+      } else if (bid[i] == (int32_t)i + 1) {
+        v[0] = (int32_t)i + 1;
+        take[0] = true;
+      }
     }
     // positions past the capacity are counted but not written
     int total;
@@ -780,10 +812,10 @@ __global__ __launch_bounds__(kBT) void k_log_outlets(int32_t N, const int32_t* _
 }
 
 int launch_log_outlets(pff_ctx* ctx, const GridD& g, const int32_t* bid, const int64_t* outlet,
-                       const int32_t* snode, int64_t* dev_count) {
+                       const int32_t* snode, int64_t* dev_count, bool all_slots) {
   const auto& L = ctx->lake_log;
   hipLaunchKernelGGL(k_log_outlets, dim3(grid_for(g.N, kBT, 8192)), dim3(kBT), 0, ctx->stream, g.N, bid, outlet, snode,
-                     L.basin, L.outlet, L.snode, L.capacity, dev_count);
+                     L.basin, L.outlet, L.snode, L.capacity, dev_count, all_slots ? 1 : 0);
   return check_hip(hipGetLastError(), "log_outlets");
 }
 

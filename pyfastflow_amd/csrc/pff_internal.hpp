@@ -156,7 +156,7 @@ int launch_reroute_jump(pff_ctx* ctx, const GridD& g, int32_t* rec, const int64_
 // append (b, outlet[b], saddlenode[b]) for every basin b of the current labels
 // (cells with bid[i] == i + 1) to the armed lake log; *dev_count counts them
 int launch_log_outlets(pff_ctx* ctx, const GridD& g, const int32_t* bid, const int64_t* outlet,
-                       const int32_t* snode, int64_t* dev_count);
+                       const int32_t* snode, int64_t* dev_count, bool all_slots);
 // reference-shaped carve (lakeflow.py:405-438)
 int reroute_carve(pff_ctx* ctx, const GridD& g, int32_t* rec, int32_t* rec_, int32_t* rec__, uint8_t* tag,
                   uint8_t* tag_, const int32_t* snode, const int64_t* outlet, int32_t* iters);

@@ -5,6 +5,8 @@ Integer outputs (receivers) bit-exact; Q bit-exact against the single-grid
 GPU run and within 1e-5 relative of f32(q0) * exact upstream count.
 """
 
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -116,3 +118,20 @@ def test_batch_reuse_and_empty(pf):
         assert torch.equal(x, y)
     e = pf.flow.route_batch([], dx=DX)
     assert e.status["cells"] == 0 and e.receivers == []
+
+
+def test_batch_with_a_receiver_cycle_runs_tile_by_tile(pf, ora):
+    """A tile whose lakeflow carve leaves a receiver cycle (only 2-cycles are
+    removed; tests/golden/cycle_dem_32x32.npy) sends the batch down the
+    tile-by-tile path; every tile's receivers still match the oracle and its
+    Q the single-grid run."""
+    cyc = np.load(os.path.join(os.path.dirname(__file__), "golden", "cycle_dem_32x32.npy")).astype(np.float32)
+    tiles = _mixed_tiles()[:6] + [cyc]
+    print(f"case batch with a cycle tile, {len(tiles)} tiles", flush=True)
+    res = pf.flow.route_batch(tiles, dx=DX, lakeflow=True, value=1e-5)
+    assert res.status["per_tile"]
+    for k, z in enumerate(tiles):
+        g, rcv = _oracle(ora, z)
+        np.testing.assert_array_equal(res.receivers[k].cpu().numpy(), rcv, err_msg=f"tile {k} {z.shape}")
+        _, q1 = _single(pf, z)
+        np.testing.assert_array_equal(res.Q[k].cpu().numpy().view(np.uint32), q1.view(np.uint32))

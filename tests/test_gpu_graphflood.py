@@ -138,3 +138,21 @@ def test_graphflood_iterations_vs_oracle(pf, ora, update):
         _close(r.z.to_numpy(), zh_n, f"zh it{it}")
         # continue both from the GPU state so the routing stays shared
         zh, h, Q = r.z.to_numpy().copy(), fl.h.to_numpy().copy(), r.Q.to_numpy().copy()
+
+
+def test_graphflood_survives_carve_cycles(pf):
+    """On the 2048x2048 GraphFlood workload the lakeflow carve of the water
+    surface leaves a receiver cycle within the first 20 iterations (only
+    2-cycles are removed); the reroute must detect it and take the exact
+    path rather than fail."""
+    from pyfastflow_amd import synth
+
+    cg = synth.graphflood_plain()
+    r = pf.flow.FlowRouter(cg["nx"], cg["ny"], cg["dx"], boundary_mode="normal")
+    r.set_z(cg["z"])
+    fl = pf.graphflood.Flooder(r, precipitation_rates=cg["precipitation"], manning=cg["manning"], edge_slope=1e-2)
+    print("case graphflood 2048^2, 25 iterations", flush=True)
+    fl.run_graphflood(25)
+    h = fl.get_h()
+    assert np.isfinite(h).all()
+    assert float(h.max()) > 0.0
