@@ -169,14 +169,19 @@ int reroute_core(pff_ctx* ctx, const GridD& d, int32_t* bid, int32_t* rec, int32
   // cycle appears (possible only through rerouting, e.g. a basin-graph cycle
   // longer than 2) every later step runs the exact synchronous semantics.
   bool exact = !carve;  // jump mode can leave non-forest receivers: keep exact semantics
-  int32_t *levels = nullptr, *fin = nullptr;
+  int32_t *levels = nullptr, *fin = nullptr, *rcvp = nullptr;
+  uint8_t* mark = nullptr;
   auto ensure_levels = [&]() -> int {
     if (levels) return 0;
-    void *lv, *fv;
+    void *lv, *fv, *rp, *mk;
     PFF_TRY(scratch(ctx, S_LEVELS, (size_t)(npass + 1) * n4, &lv));
     PFF_TRY(scratch(ctx, S_I4, n4, &fv));
+    PFF_TRY(scratch(ctx, S_RCVP, n4, &rp));
+    PFF_TRY(scratch(ctx, S_MARK, (size_t)d.N, &mk));
     levels = (int32_t*)lv;
     fin = (int32_t*)fv;
+    rcvp = (int32_t*)rp;
+    mark = (uint8_t*)mk;
     return 0;
   };
   // tile activity flags: trip t labels, sorts and carves only the tiles that
@@ -235,7 +240,7 @@ int reroute_core(pff_ctx* ctx, const GridD& d, int32_t* bid, int32_t* rec, int32
       std::swap(flag_a, flag_b);
     } else if (carve) {
       PFF_TRY(ensure_levels());
-      PFF_TRY(labels_with_levels(ctx, d, rec_, bid, rec__, levels, fin));
+      PFF_TRY(labels_split(ctx, d, rec_, bid, rec__, levels, fin, rcvp, mark));
     } else {
       PFF_TRY(basin_labels(ctx, d, rec_, bid, rec__));
     }
@@ -267,7 +272,7 @@ int reroute_core(pff_ctx* ctx, const GridD& d, int32_t* bid, int32_t* rec, int32
       }
       exact = true;
       PFF_TRY(ensure_levels());
-      PFF_TRY(labels_with_levels(ctx, d, rec_, bid, rec__, levels, fin));
+      PFF_TRY(labels_split(ctx, d, rec_, bid, rec__, levels, fin, rcvp, mark));
       PFF_TRY(launch_saddlesort(ctx, d, bid, is_border, z_prime, basin_saddle, basin_saddlenode, outlet, z));
     } else {
       PFF_TRY(launch_saddlesort(ctx, d, bid, is_border, z_prime, basin_saddle, basin_saddlenode, outlet, z));
@@ -275,9 +280,8 @@ int reroute_core(pff_ctx* ctx, const GridD& d, int32_t* bid, int32_t* rec, int32
       if (st) st->n_iter++;
     }
     if (carve) {
-      int32_t iters = 0;
-      PFF_TRY(carve_from_levels(ctx, d, rec, rec_, levels, fin, rec__, basin_saddlenode, outlet, rec_, nullptr, npass,
-                                &iters));
+      PFF_TRY(carve_split(ctx, d, rec, rec_, bid, basin_saddlenode, outlet, rec_, levels, fin, rec__, rcvp, mark,
+                          npass));
     } else {
       PFF_TRY(launch_reroute_jump(ctx, d, rec_, outlet));  // :529
     }

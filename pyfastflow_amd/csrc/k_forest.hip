@@ -46,6 +46,10 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, FastDiv f) {
 constexpr int kItems = 16;
 constexpr int kChunk = kItems * kBT;  // 4096
 
+// label of the cells that tiled labels with cycle marking finds on or
+// draining into a receiver cycle
+constexpr int32_t kCycleLabel = -1;
+
 // counts ---------------------------------------------------------------------
 // init from all cells: S = 1, P = rcv (or -1 at roots), list = non-roots
 __global__ __launch_bounds__(kBT) void k_count_init_all(int32_t N, const int32_t* __restrict__ rcv,
@@ -353,6 +357,79 @@ int labels_with_levels(pff_ctx* ctx, const GridD& g, const int32_t* rec, int32_t
 
 int npass_for(int32_t N) { return ceil_log2i(N) + 1; }
 
+// After tiled labels with cycle marking: the cells labelled kCycleLabel (on
+// or draining into a receiver cycle) get the set-up of the synchronous
+// passes (init label, pointer, level 0, not finished) and are listed; every
+// other cell keeps its root's label.  rcvp = rec with the listed cells made
+// roots (-1), mark = 1 on the listed cells.
+template <int MODE>
+__global__ __launch_bounds__(kBT) void k_cycle_split(GridD g, const int32_t* __restrict__ rec, int32_t* __restrict__ bid,
+                                                     int32_t* __restrict__ P, int32_t* __restrict__ level0,
+                                                     int32_t* __restrict__ fin, int32_t* __restrict__ rcvp,
+                                                     uint8_t* __restrict__ mark, int32_t* __restrict__ list,
+                                                     int64_t* count) {
+  int32_t val[kItems];
+  bool take[kItems];
+  const int64_t base = (int64_t)blockIdx.x * kChunk;
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const int64_t t = base + j * kBT + threadIdx.x;
+    const int i = (int)t;
+    take[j] = false;
+    val[j] = i;
+    if (t < g.N) {
+      int r = rec[i];
+      if ((unsigned)r >= (unsigned)g.N) r = i;
+      const bool m = bid[i] == kCycleLabel;
+      rcvp[i] = m ? -1 : r;
+      mark[i] = m ? 1 : 0;
+      if (m) {
+        int row, col;
+        rc_of(g, i, row, col);
+        bid[i] = can_leave<MODE>(g, i, row, col) ? 0 : i + 1;
+        P[i] = r;
+        level0[i] = r;
+        fin[i] = 0x7fffffff;
+        take[j] = true;
+      }
+    }
+  }
+  block_append<kItems>(val, take, list, count);
+}
+
+// Labels with the semantics of ceil(log2 N)+1 synchronous passes on
+// receivers that may hold cycles.  A cell whose path reaches a root gets its
+// root's label from the tiled contraction (the passes give the same); only
+// the cells on or draining into a cycle run the passes, and their successors
+// are such cells too, so the passes over them alone give the labels, levels
+// and final pointers that the passes over every cell would.  Pfinal, levels
+// and fin are written for those cells only; rcvp and mark as in
+// k_cycle_split.
+int labels_split(pff_ctx* ctx, const GridD& g, const int32_t* rec, int32_t* bid, int32_t* Pfinal, int32_t* levels,
+                 int32_t* fin, int32_t* rcvp, uint8_t* mark) {
+  int64_t ndep_unused = 0;
+  int32_t cyc_unused = 0;
+  PFF_TRY(tiled_labels(ctx, g, rec, nullptr, &ndep_unused, &cyc_unused, nullptr, true));
+  PFF_TRY(tiled_labels_write(ctx, g, bid, nullptr, nullptr));
+  const size_t n4 = (size_t)g.N * 4;
+  void *L0, *L1, *nbv, *npv;
+  PFF_TRY(scratch(ctx, S_WL0, n4, &L0));
+  PFF_TRY(scratch(ctx, S_WL1, n4, &L1));
+  PFF_TRY(scratch(ctx, S_I2, n4, &nbv));
+  PFF_TRY(scratch(ctx, S_I3, n4, &npv));
+  Lists L;
+  L.l[0] = (int32_t*)L0;
+  L.l[1] = (int32_t*)L1;
+  L.c[0] = ctx->dev_words + W_COUNT0;
+  L.c[1] = ctx->dev_words + W_COUNT1;
+  PFF_TRY(check_hip(hipMemsetAsync(L.c[0], 0, 8, ctx->stream), "memset"));
+  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_cycle_split<M>, dim3((g.N + kChunk - 1) / kChunk), dim3(kBT), 0,
+                                               ctx->stream, g, rec, bid, Pfinal, levels, fin, rcvp, mark, L.l[0],
+                                               L.c[0]));
+  return label_rounds_run(ctx, L, bid, Pfinal, (int32_t*)nbv, (int32_t*)npv, levels, g.N, fin, 0,
+                          ceil_log2i(g.N) + 1);
+}
+
 // ---------------------------------------------------------------------------
 // Tiled contraction
 // ---------------------------------------------------------------------------
@@ -403,6 +480,10 @@ struct TileArgs {
   // without a weight skips its rounds and reads them from there.
   uint32_t* tdirty;
   int ends_valid;
+  // labels on receivers that may hold cycles: a cell on or draining into a
+  // cycle gets E = -1 (in-tile cycle) or the label kCycleLabel (through the
+  // reduced graph) instead of failing the pass
+  int mark_cycles;
   unsigned long long* prof_cells;  // profiling: cells processed by this launch
 };
 
@@ -557,6 +638,15 @@ __global__ __launch_bounds__(kTB) void k_tile(TileArgs a) {
       if (stored) {
         end = a.E[c] == c;
       } else {
+        if constexpr (W == W_ROOTS) {
+          // the in-tile pointer of a cell whose path ends in the tile has
+          // reached NONE after kTileRounds rounds; one that has not is on or
+          // drains into a cycle inside the tile
+          if (a.mark_cycles && P[li] != NONE) {
+            a.E[c] = -1;
+            continue;
+          }
+        }
         const int e = E[li];
         a.E[c] = (y0 + (e >> 6)) * g.nx + x0 + (e & 63);
         end = e == li;
@@ -650,8 +740,16 @@ __global__ __launch_bounds__(kBT) void k_exit_link(TileArgs a, const int32_t* __
         continue;
       }
     }
-    a.P2[x] = a.E[r];
-    if (labels) a.bid2[x] = 0;
+    const int er = a.E[r];
+    if (labels && er < 0) {  // drains into an in-tile cycle (cycle marking only)
+      a.P2[x] = x;
+      a.bid2[x] = kCycleLabel;
+      list_out[w] = x;
+      continue;
+    }
+    a.P2[x] = er;
+    // an exit keeps its initial label only if its chain never reaches a root
+    if (labels) a.bid2[x] = a.mark_cycles ? kCycleLabel : 0;
     list_out[w] = x;
   }
 }
@@ -703,12 +801,13 @@ __global__ __launch_bounds__(kBT) void k_label_tiles(TileArgs a, int32_t* __rest
   int any = 0;
   if (col < a.g.nx && row < a.g.ny) {
     const int c = row * a.g.nx + col;
-    const int b = a.bid2[a.E[c]];
+    const int e = a.E[c];
+    const int b = e >= 0 ? a.bid2[e] : kCycleLabel;
     bid[c] = b;
     any = b != 0;
   }
   any = __syncthreads_or(any);
-  if (any && threadIdx.x == 0 && threadIdx.y == 0) atomicOr(&tact_out[t], 1u);
+  if (tact_out && any && threadIdx.x == 0 && threadIdx.y == 0) atomicOr(&tact_out[t], 1u);
 }
 
 struct TileBufs {
@@ -862,11 +961,12 @@ int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, 
 // basin.  *ndep, *npits, *cycle are host outputs; syncs once (per six
 // reduced-graph rounds).
 int tiled_labels(pff_ctx* ctx, const GridD& g, const int32_t* rec, const uint32_t* tact_in, int64_t* ndep,
-                 int32_t* cycle, int* rounds_used) {
+                 int32_t* cycle, int* rounds_used, bool mark_cycles) {
   TileBufs b;
   PFF_TRY(tile_bufs(ctx, g, b));
   TileArgs a = tile_args(g, b, ctx, rec);
   a.tact = tact_in;
+  a.mark_cycles = mark_cycles ? 1 : 0;
   hipStream_t s = ctx->stream;
   PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_NEXIT, 0, 8, s), "memset"));
   PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_CYCLE, 0, 8, s), "memset"));
@@ -883,8 +983,14 @@ int tiled_labels(pff_ctx* ctx, const GridD& g, const int32_t* rec, const uint32_
   L.c[1] = ctx->dev_words + W_COUNT1;
   hipLaunchKernelGGL(k_exit_link, dim3(list_grid()), dim3(kBT), 0, s, a, b.exits, a.n_exits, 1, b.L0, L.c[0],
                      ctx->dev_words + W_EMPTY, nullptr, 0);
-  // start from the previous call's round count, then confirm on the host
   const int cap = ceil_log2i(g.N) + 1;
+  if (mark_cycles) {
+    // chains that reach a root do so within cap rounds; the others keep
+    // kCycleLabel.  Nothing is read back.
+    PFF_TRY(label_rounds_run(ctx, L, b.bid2, b.P2, b.t0, b.t1, nullptr, g.N, nullptr, 0, cap));
+    return 0;
+  }
+  // start from the previous call's round count, then confirm on the host
   int done_rounds = 0;
   int64_t live = 1;
   int step = std::max(2, std::min(ctx->last_label_rounds + 1, cap));
@@ -915,7 +1021,8 @@ int tiled_labels_write(pff_ctx* ctx, const GridD& g, int32_t* bid, const uint32_
   PFF_TRY(tile_bufs(ctx, g, b));
   TileArgs a = tile_args(g, b, ctx, nullptr);
   a.tact = tact_in;
-  PFF_TRY(check_hip(hipMemsetAsync(tact_out, 0, (size_t)tiles_count(g) * 4, ctx->stream), "memset"));
+  if (tact_out)
+    PFF_TRY(check_hip(hipMemsetAsync(tact_out, 0, (size_t)tiles_count(g) * 4, ctx->stream), "memset"));
   hipLaunchKernelGGL(k_label_tiles, dim3((g.nx + 63) / 64, (g.ny + 3) / 4), dim3(64, 4), 0, ctx->stream, a, bid,
                      tact_out);
   return check_hip(hipGetLastError(), "tiled_labels_write");

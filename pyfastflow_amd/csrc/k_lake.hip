@@ -289,9 +289,10 @@ int launch_reroute_jump(pff_ctx* ctx, const GridD& g, int32_t* rec, const int64_
 // path lengths instead of N per round.
 // ---------------------------------------------------------------------------
 
-// seed: tag every valid saddle node (init_reroute_carve, :323-330)
+// seed: tag every valid saddle node (init_reroute_carve, :323-330); with
+// mark, only the saddle nodes on marked cells
 __global__ __launch_bounds__(kBT) void k_carve_seed(int32_t N, const int32_t* __restrict__ snode, uint32_t* tagw,
-                                                    int32_t* list, int64_t* count) {
+                                                    int32_t* list, int64_t* count, const uint8_t* __restrict__ mark) {
   constexpr int IT = 16;
   int32_t v[IT];
   bool take[IT];
@@ -304,7 +305,7 @@ __global__ __launch_bounds__(kBT) void k_carve_seed(int32_t N, const int32_t* __
     if (b < N) {
       const int s = snode[b];
       v[j] = s;
-      if (s >= 0) take[j] = atomicCAS(&tagw[s], 0u, 1u) == 0u;
+      if (s >= 0 && (!mark || mark[s])) take[j] = atomicCAS(&tagw[s], 0u, 1u) == 0u;
     }
   }
   block_append<IT>(v, take, list, count);
@@ -415,7 +416,7 @@ int carve_from_levels(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t*
   PFF_TRY(check_hip(hipMemsetAsync(w, 0, 3 * 8, ctx->stream), "memset"));
   hipStream_t s = ctx->stream;
   hipLaunchKernelGGL(k_carve_seed, dim3((g.N + 16 * kBT - 1) / (16 * kBT)), dim3(kBT), 0, s, g.N, snode, (uint32_t*)tagw,
-                     (int32_t*)list, w);
+                     (int32_t*)list, w, (const uint8_t*)nullptr);
   for (int t = 0; t <= npass; ++t) {
     hipLaunchKernelGGL(k_carve_prep, dim3(1), dim3(1), 0, s, w);
     hipLaunchKernelGGL(k_carve_round, dim3(stride_grid()), dim3(kBT), 0, s, g.N, levels, fin, Pfinal,
@@ -934,6 +935,55 @@ int carve_tiled(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t* rec_i
     hipLaunchKernelGGL(k_carve_sync_list, lg, bl, 0, ctx->stream, (const int32_t*)tl, ntag, (const int32_t*)rec,
                        rec_work);
   return check_hip(hipGetLastError(), "carve_tiled");
+}
+
+// Carve of a trip whose receivers may hold cycles (labels from
+// labels_split).  The reference's tag loop runs until the tags stop changing
+// (:431), so the tagged cells are every cell reachable from a saddle node.
+// From a saddle node whose path reaches a root that is the path (rcvp keeps
+// those links): the tiled saddle tags.  From a saddle node on a marked cell
+// it stays among the marked cells: the jumping rounds of carve_from_levels
+// over those cells alone.  A cell of the first kind has one tagged donor at
+// most; the reversal of the second takes the largest donor, as
+// carve_from_levels does.  Then every saddle node links to its outlet.
+int carve_split(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t* rec_in, const int32_t* bid,
+                const int32_t* snode, const int64_t* outlet, int32_t* rec_work, const int32_t* levels,
+                const int32_t* fin, const int32_t* Pfinal, const int32_t* rcvp, const uint8_t* mark, int npass) {
+  void *tag, *tl, *tagw, *list;
+  PFF_TRY(scratch(ctx, S_B1, (size_t)g.N, &tag));
+  PFF_TRY(scratch(ctx, S_TLIST, (size_t)g.N * 4, &tl));
+  PFF_TRY(scratch(ctx, S_TAG, (size_t)g.N * 4, &tagw, true));
+  PFF_TRY(scratch(ctx, S_MISC, (size_t)g.N * 4, &list));
+  hipStream_t s = ctx->stream;
+  int64_t* ntag = ctx->dev_words + W_NTAG;
+  PFF_TRY(check_hip(hipMemsetAsync(ntag, 0, 8, s), "memset"));
+  // paths that reach a root (a forest once the marked cells are roots)
+  PFF_TRY(tiled_upstream(ctx, g, rcvp, 2, bid, snode, 0.0f, nullptr, nullptr, (uint8_t*)tag, nullptr, nullptr, 0,
+                         (int32_t*)tl, -1));
+  const dim3 lg(2048), bl(kBT);
+  hipLaunchKernelGGL(k_carve_reverse_list, lg, bl, 0, s, g.N, (const int32_t*)tl, (const int64_t*)ntag,
+                     (const uint8_t*)tag, rcvp, rec);
+  // the marked cells
+  int64_t* w = ctx->dev_words + W_COUNT0;  // 3 consecutive words
+  PFF_TRY(check_hip(hipMemsetAsync(w, 0, 3 * 8, s), "memset"));
+  hipLaunchKernelGGL(k_carve_seed, dim3((g.N + 16 * kBT - 1) / (16 * kBT)), dim3(kBT), 0, s, g.N, snode,
+                     (uint32_t*)tagw, (int32_t*)list, w, mark);
+  for (int t = 0; t <= npass; ++t) {
+    hipLaunchKernelGGL(k_carve_prep, dim3(1), dim3(1), 0, s, w);
+    hipLaunchKernelGGL(k_carve_round, dim3(stride_grid()), dim3(kBT), 0, s, g.N, levels, fin, Pfinal,
+                       (uint32_t*)tagw, (int32_t*)list, w, t);
+  }
+  hipLaunchKernelGGL(k_carve_reverse, dim3(stride_grid()), dim3(kBlock), 0, s, g.N, (const int32_t*)list, w,
+                     (uint32_t*)tagw, rec_in);
+  hipLaunchKernelGGL(k_carve_reverse_apply, dim3(stride_grid()), dim3(kBlock), 0, s, (const int32_t*)list, w,
+                     (const uint32_t*)tagw, rec);
+  hipLaunchKernelGGL(k_carve_link, dim3(grid_for(g.N)), dim3(kBlock), 0, s, g.N, outlet, snode, rec);
+  hipLaunchKernelGGL(k_carve_sync, dim3(stride_grid()), dim3(kBlock), 0, s, (const int32_t*)list, w,
+                     (uint32_t*)tagw, rec, rec_work, (uint8_t*)nullptr);
+  if (rec_work)
+    hipLaunchKernelGGL(k_carve_sync_list, lg, bl, 0, s, (const int32_t*)tl, (const int64_t*)ntag, (const int32_t*)rec,
+                       rec_work);
+  return check_hip(hipGetLastError(), "carve_split");
 }
 
 // Reference-shaped carve (lakeflow.py:405-438) for the module-level API.

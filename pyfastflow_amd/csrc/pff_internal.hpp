@@ -88,6 +88,7 @@ enum Slot : int {
   S_M0, S_M1, S_TPIT2,  // basin-level relabel: label maps (by basin id) and the next pits list
   S_BL0, S_BL1,         // border cells of closed basins, this trip's and the next trip's
   S_TLIST,              // cells tagged by the tiled carve
+  S_RCVP, S_MARK,       // receivers with the cycle cells made roots, cycle-cell flags
   S_NSLOTS
 };
 static_assert(S_NSLOTS <= pff_ctx::kSlots, "pff_ctx::kSlots must cover every scratch slot");
@@ -130,8 +131,11 @@ int launch_saddlesort(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t*
 int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, const int32_t* bid,
                    const int32_t* snode, float q0, float* Q, uint32_t* cnt, uint8_t* tag, const uint32_t* tact,
                    int32_t* cycle, int rounds = 0, int32_t* tag_list = nullptr, int ends_valid = -1);
+// mark_cycles: the receivers may hold cycles; the cells on or draining into
+// one get the label -1 in tiled_labels_write (no host read; *ndep, *cycle and
+// *rounds_used are not written)
 int tiled_labels(pff_ctx* ctx, const GridD& g, const int32_t* rec, const uint32_t* tact_in, int64_t* ndep,
-                 int32_t* cycle, int* rounds_used = nullptr);
+                 int32_t* cycle, int* rounds_used = nullptr, bool mark_cycles = false);
 int tiled_labels_write(pff_ctx* ctx, const GridD& g, int32_t* bid, const uint32_t* tact_in, uint32_t* tact_out);
 int tiled_pits(pff_ctx* ctx, const GridD& g, int32_t** pits, int64_t** npits);
 // Labels of the next lakeflow trip from this trip's labels and outlets,
@@ -166,6 +170,17 @@ int labels_with_levels(pff_ctx* ctx, const GridD& g, const int32_t* rec, int32_t
 int carve_from_levels(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t* rec_in, const int32_t* levels,
                       const int32_t* fin, const int32_t* Pfinal, const int32_t* snode, const int64_t* outlet,
                       int32_t* rec_work, uint8_t* tag_bytes, int npass, int32_t* iters);
+// Labels and carve of a trip whose receivers may hold cycles, split between
+// the cells whose path reaches a root (tiled contraction) and the cells on or
+// draining into a cycle (synchronous passes over those cells only).  Same
+// results as labels_with_levels + carve_from_levels.  labels_split writes
+// rcvp (rec with the cycle cells made roots) and mark (1 on the cycle cells)
+// for carve_split; Pfinal, levels and fin only on the cycle cells.
+int labels_split(pff_ctx* ctx, const GridD& g, const int32_t* rec, int32_t* bid, int32_t* Pfinal, int32_t* levels,
+                 int32_t* fin, int32_t* rcvp, uint8_t* mark);
+int carve_split(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t* rec_in, const int32_t* bid,
+                const int32_t* snode, const int64_t* outlet, int32_t* rec_work, const int32_t* levels,
+                const int32_t* fin, const int32_t* Pfinal, const int32_t* rcvp, const uint8_t* mark, int npass);
 
 int fill_passes(pff_ctx* ctx, const GridD& g, const float* zh, float* z_, int32_t* r0, int32_t* r1, double eps,
                 int first_exp);

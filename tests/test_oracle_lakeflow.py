@@ -8,6 +8,8 @@ import pytest
 from oracle_util import neighbour_table, outlets, roots_by_walking, spill_elevation
 from pyfastflow_amd import synth
 
+INVALID_KEY = 0x4CBEBC200000002A
+
 INVALID = 0x4CBEBC200000002A
 
 
@@ -188,3 +190,49 @@ def test_capped_dems_keep_a_depression(ora):
                 for k in range(4):
                     j = nb[k, c]
                     assert j < 0 or bid[j] == p + 1
+
+
+def test_labels_on_cycles_are_the_jumped_pointer(ora):
+    """ceil(log2 N)+1 synchronous passes bid <- bid[rec], rec <- rec[rec]
+    give bid0[f^(2^P - 1)(i)] on any receiver graph (by induction on the
+    passes); on a forest that is the root's label.  Checked on graphs with
+    loops of 720, 128 (a power of two, which the doubling folds onto
+    itself), 4 and 2 cells."""
+    from cycle_graphs import cycle_case, jumped
+
+    z, rcv, _ = cycle_case()
+    g = ora.Grid(300, 200, 10.0, 0)
+    npass = int(np.ceil(np.log2(g.N))) + 1
+    bid0 = ora.basin_id_init(g)
+    want = bid0[jumped(rcv, npass)]
+    np.testing.assert_array_equal(ora.basin_identification(g, rcv), want)
+
+
+def test_carve_on_cycles_tags_the_forward_closure(ora):
+    """The carve's tag loop runs until nothing changes, so it tags every cell
+    reachable from a saddle node, loops included; the reversal then points
+    each tagged cell at its largest tagged donor and the saddle nodes at
+    their outlets."""
+    from cycle_graphs import cycle_case, forward_closure
+
+    z, rcv, loops = cycle_case()
+    g = ora.Grid(300, 200, 10.0, 0)
+    bid = ora.basin_identification(g, rcv)
+    ss = ora.saddlesort(g, bid, z)
+    sn, ol = ss["saddlenode"], ss["outlet"]
+    seeds = sn[sn >= 0]
+    tag = forward_closure(rcv, seeds)
+    loop_cells = {r * 300 + c for cells in loops for r, c in cells}
+    assert any(tag[c] for c in loop_cells), "no saddle path reaches a loop"
+    rec, otag, _ = ora.reroute_carve(g, rcv, sn, ol)
+    np.testing.assert_array_equal(otag.astype(bool), tag)
+    want = rcv.astype(np.int64).copy()
+    i = np.flatnonzero(tag & (rcv != np.arange(g.N)))
+    i = i[tag[rcv[i]]]
+    best = np.full(g.N, -1, np.int64)
+    np.maximum.at(best, rcv[i], i)
+    hit = best >= 0
+    want[hit] = best[hit]
+    b = np.flatnonzero(ol != INVALID_KEY)
+    want[sn[b]] = (ol[b] & 0xFFFFFFFF).astype(np.int64)
+    np.testing.assert_array_equal(rec, want)
