@@ -520,6 +520,7 @@ __global__ __launch_bounds__(kTB) void k_tile(TileArgs a) {
   int32_t cell[CPT];
   uint32_t rootmask = 0;
   int wany = 0;
+  int farlink = 0;  // a receiver that is not a D4 neighbour (labels pass only)
   __shared__ uint32_t dirty_s;
   if constexpr (PH == 1 && W == W_SADDLE) {
     if (threadIdx.x == 0) {
@@ -548,6 +549,12 @@ __global__ __launch_bounds__(kTB) void k_tile(TileArgs a) {
         const int rr = (int)fdiv((uint32_t)r, a.fd);
         const int dy = rr - y0, dxx = r - rr * g.nx - x0;
         if ((unsigned)dy < (unsigned)th && (unsigned)dxx < (unsigned)tw) lp = (uint16_t)(dy * TS + dxx);
+        if constexpr (PH == 1 && W == W_ROOTS) {
+          // D4 links, wrapping ones included: a receiver in another tile
+          // then lies on that tile's ring, which the carve's tag walks use
+          const int ady = abs(dy - ly), adx = abs(dxx - lx);
+          if (!(ady + adx == 1 || (ady == 0 && adx == g.nx - 1) || (adx == 0 && ady == g.ny - 1))) farlink = 1;
+        }
       }
       if constexpr (W != W_ROOTS) {
         w = tile_weight<W>(a, c);
@@ -564,8 +571,15 @@ __global__ __launch_bounds__(kTB) void k_tile(TileArgs a) {
   // saddle weights are sparse: a tile without any nonzero weight keeps S = 0
   // and skips the sum part of the rounds (uniform per block)
   bool sums = W == W_UNIT;
-  if constexpr (W == W_SADDLE) sums = __syncthreads_or(wany) != 0;
-  else __syncthreads();
+  if constexpr (W == W_SADDLE) {
+    sums = __syncthreads_or(wany) != 0;
+  } else if constexpr (W == W_ROOTS && PH == 1) {
+    // receivers that are not D4 links take the exact path (reported as a
+    // cycle; uniform per block)
+    if (__syncthreads_or(farlink) && threadIdx.x == 0) atomicOr((unsigned long long*)a.cycle, 1ull);
+  } else {
+    __syncthreads();
+  }
   // Saddle tags run on receivers already known to be a forest; a tile with
   // no weight there only needs its local ends, jumped on E alone until they
   // stop changing (uniform per block)
@@ -810,6 +824,221 @@ __global__ __launch_bounds__(kBT) void k_label_tiles(TileArgs a, int32_t* __rest
   if (tact_out && any && threadIdx.x == 0 && threadIdx.y == 0) atomicOr(&tact_out[t], 1u);
 }
 
+// ---------------------------------------------------------------------------
+// Carve tags by walks.  A tile's carve tags only come from its saddle nodes
+// (the paths from them down to the tile's local ends) and from the tagged
+// flow that enters it through ring cells (a D4 receiver in another tile lies
+// on that tile's ring).  So per tile only those paths are followed, one lane
+// per start cell over LDS local pointers, instead of contracting every cell:
+// the local pass walks from the saddle nodes (tagging) and from every ring
+// cell (finding its local end, which the reduced graph needs), the final
+// pass walks from the ring cells that received a tag (tagging).  A walk
+// stops at a cell already tagged: that cell's walker carries on downstream.
+// ---------------------------------------------------------------------------
+constexpr int TW = TC / 32;  // bitmap words per tile
+
+__device__ __forceinline__ bool bit_test_set(uint32_t* bm, int li) {
+  const uint32_t b = 1u << (li & 31);
+  return (atomicOr(&bm[li >> 5], b) & b) != 0u;
+}
+__device__ __forceinline__ bool bit_test(const uint32_t* bm, int li) { return (bm[li >> 5] >> (li & 31)) & 1u; }
+
+// k-th ring cell (local index) of a tw x th tile; ring_count cells in all
+__device__ __forceinline__ int ring_count(int tw, int th) {
+  if (th == 1) return tw;
+  if (tw == 1) return th;
+  return 2 * tw + 2 * (th - 2);
+}
+__device__ __forceinline__ int ring_cell(int k, int tw, int th) {
+  if (th == 1) return k;
+  if (tw == 1) return k * TS;
+  if (k < tw) return k;
+  k -= tw;
+  if (k < tw) return (th - 1) * TS + k;
+  k -= tw;
+  if (k < th - 2) return (k + 1) * TS;
+  k -= th - 2;
+  return (k + 1) * TS + tw - 1;
+}
+
+// local pointers of the tile into P (NONE: root or leaves the tile), roots
+// into the bitmap R (zeroed by the caller)
+__device__ __forceinline__ void tile_load_pointers(const TileArgs& a, int x0, int y0, int tw, int th, uint16_t* P,
+                                                   uint32_t* R) {
+#pragma unroll
+  for (int j = 0; j < CPT; ++j) {
+    const int li = threadIdx.x + j * kTB;
+    const int ly = li >> 6, lx = li & 63;
+    uint16_t lp = NONE;
+    if (lx < tw && ly < th) {
+      const int c = (y0 + ly) * a.g.nx + x0 + lx;
+      int r = a.rcv[c];
+      if ((unsigned)r >= (unsigned)a.g.N) r = c;
+      if (r == c) {
+        atomicOr(&R[li >> 5], 1u << (li & 31));
+      } else {
+        const int rr = (int)fdiv((uint32_t)r, a.fd);
+        const int dy = rr - y0, dxx = r - rr * a.g.nx - x0;
+        if ((unsigned)dy < (unsigned)th && (unsigned)dxx < (unsigned)tw) lp = (uint16_t)(dy * TS + dxx);
+      }
+    }
+    P[li] = lp;
+  }
+}
+
+// tag the path from li down to the first tagged cell or the local end; the
+// newly tagged cells go to the staging list.  Returns the local end if the
+// walk reached it (the end is then tagged by this walk), else -1.
+__device__ __forceinline__ int walk_tag(int li, const uint16_t* P, uint32_t* T, uint16_t* staged, int* nstaged,
+                                        int64_t* cycle) {
+  int c = li;
+  for (int s = 0; s < TC; ++s) {
+    if (bit_test_set(T, c)) return -1;
+    staged[atomicAdd(nstaged, 1)] = (uint16_t)c;
+    const uint16_t n = P[c];
+    if (n == NONE) return c;
+    c = n;
+  }
+  atomicOr((unsigned long long*)cycle, 1ull);  // not a forest
+  return -1;
+}
+
+__device__ __forceinline__ int walk_end(int li, const uint16_t* P, int64_t* cycle) {
+  int c = li;
+  for (int s = 0; s < TC; ++s) {
+    const uint16_t n = P[c];
+    if (n == NONE) return c;
+    c = n;
+  }
+  atomicOr((unsigned long long*)cycle, 1ull);
+  return li;
+}
+
+// append the staged local cells (global ids) to list, one atomic per block
+__device__ __forceinline__ void block_emit(const uint16_t* staged, int n, int x0, int y0, int nx, int32_t* list,
+                                           int64_t* count, uint8_t* tag) {
+  __shared__ int64_t base_s;
+  if (threadIdx.x == 0) base_s = n ? (int64_t)atomicAdd((unsigned long long*)count, (unsigned long long)n) : 0;
+  __syncthreads();
+  for (int k = threadIdx.x; k < n; k += kTB) {
+    const int li = staged[k];
+    const int c = (y0 + (li >> 6)) * nx + x0 + (li & 63);
+    if (tag) tag[c] = 1;
+    list[base_s + k] = c;
+  }
+}
+
+// local pass: tags below the tile's saddle nodes (wbits, cleared here), the
+// local end E of every ring cell, the reduced-graph nodes (the ends those
+// walks reach: listed exits with S = 1 if a saddle path reaches them, or
+// terminals P2 = -1), inflow = 0 on the ring
+__global__ __launch_bounds__(kTB) void k_tag_local(TileArgs a, uint32_t* __restrict__ wbits) {
+  __shared__ uint16_t P[TC];
+  __shared__ uint16_t staged[TC];
+  __shared__ uint16_t ends[TC];
+  __shared__ uint32_t T[TW], Lb[TW], F[TW], R[TW], Wb[TW];
+  __shared__ int ntag, nend, nexit;
+  if (!tile_active(a.tact, a.tiles_x, blockIdx.x, blockIdx.y)) return;  // uniform per block
+  const GridD& g = a.g;
+  const int tile = blockIdx.y * a.tiles_x + blockIdx.x;
+  const int x0 = blockIdx.x * TS, y0 = blockIdx.y * TS;
+  const int tw = min(TS, g.nx - x0), th = min(TS, g.ny - y0);
+  if (a.prof_cells && threadIdx.x == 0) atomicAdd(a.prof_cells, (unsigned long long)(tw * th));
+  if (threadIdx.x < TW) {
+    T[threadIdx.x] = 0u;
+    Lb[threadIdx.x] = 0u;
+    F[threadIdx.x] = 0u;
+    R[threadIdx.x] = 0u;
+    const uint32_t w = wbits[(size_t)tile * TW + threadIdx.x];
+    Wb[threadIdx.x] = w;
+    if (w) wbits[(size_t)tile * TW + threadIdx.x] = 0u;
+  }
+  if (threadIdx.x == 0) {
+    ntag = 0;
+    nend = 0;
+    nexit = 0;
+  }
+  __syncthreads();
+  tile_load_pointers(a, x0, y0, tw, th, P, R);
+  __syncthreads();
+  // saddle paths
+#pragma unroll 1
+  for (int j = 0; j < CPT; ++j) {
+    const int li = threadIdx.x + j * kTB;
+    if (!bit_test(Wb, li)) continue;
+    const int e = walk_tag(li, P, T, staged, &ntag, a.cycle);
+    if (e >= 0) {
+      atomicOr(&F[e >> 5], 1u << (e & 31));
+      if (!bit_test_set(Lb, e)) ends[atomicAdd(&nend, 1)] = (uint16_t)e;
+    }
+  }
+  // ring cells
+  const int nr = ring_count(tw, th);
+  for (int k = threadIdx.x; k < nr; k += kTB) {
+    const int li = ring_cell(k, tw, th);
+    const int e = walk_end(li, P, a.cycle);
+    const int c = (y0 + (li >> 6)) * g.nx + x0 + (li & 63);
+    a.E[c] = (y0 + (e >> 6)) * g.nx + x0 + (e & 63);
+    a.inflow[c] = 0u;
+    if (!bit_test_set(Lb, e)) ends[atomicAdd(&nend, 1)] = (uint16_t)e;
+  }
+  __syncthreads();
+  block_emit(staged, ntag, x0, y0, g.nx, a.tlist, a.n_tlist, a.tag);
+  // ends: roots and ends outside closed basins are terminals; the others
+  // are exits (their receiver is in another tile) with their saddle flag
+  __syncthreads();
+  const int ne = nend;
+  for (int k = threadIdx.x; k < ne; k += kTB) {
+    const int e = ends[k];
+    const int c = (y0 + (e >> 6)) * g.nx + x0 + (e & 63);
+    if (bit_test(R, e) || a.bid[c] == 0) {
+      a.P2[c] = -1;
+    } else {
+      a.S[c] = bit_test(F, e) ? 1u : 0u;
+      staged[atomicAdd(&nexit, 1)] = (uint16_t)e;
+    }
+  }
+  __syncthreads();
+  block_emit(staged, nexit, x0, y0, g.nx, a.exits, a.n_exits, nullptr);
+}
+
+// final pass on the tiles that receive a tag: walk from the ring cells with
+// a tagged inflow, tagging until a tagged cell; list the new tags
+__global__ __launch_bounds__(kTB) void k_tag_final(TileArgs a) {
+  __shared__ uint16_t P[TC];
+  __shared__ uint16_t staged[TC];
+  __shared__ uint32_t T[TW], R[TW];
+  __shared__ int ntag;
+  if (!tile_active(a.tact, a.tiles_x, blockIdx.x, blockIdx.y)) return;  // uniform per block
+  if (a.tin[blockIdx.y * a.tiles_x + blockIdx.x] == 0u) return;      // uniform per block
+  const GridD& g = a.g;
+  const int x0 = blockIdx.x * TS, y0 = blockIdx.y * TS;
+  const int tw = min(TS, g.nx - x0), th = min(TS, g.ny - y0);
+  if (a.prof_cells && threadIdx.x == 0) atomicAdd(a.prof_cells, (unsigned long long)(tw * th));
+  if (threadIdx.x < TW) {
+    T[threadIdx.x] = 0u;
+    R[threadIdx.x] = 0u;
+  }
+  if (threadIdx.x == 0) ntag = 0;
+  __syncthreads();
+  tile_load_pointers(a, x0, y0, tw, th, P, R);
+#pragma unroll
+  for (int j = 0; j < CPT; ++j) {
+    const int li = threadIdx.x + j * kTB;
+    const int ly = li >> 6, lx = li & 63;
+    if (lx < tw && ly < th && a.tag[(y0 + ly) * g.nx + x0 + lx]) atomicOr(&T[li >> 5], 1u << (li & 31));
+  }
+  __syncthreads();
+  const int nr = ring_count(tw, th);
+  for (int k = threadIdx.x; k < nr; k += kTB) {
+    const int li = ring_cell(k, tw, th);
+    const int c = (y0 + (li >> 6)) * g.nx + x0 + (li & 63);
+    if (a.inflow[c] != 0u) (void)walk_tag(li, P, T, staged, &ntag, a.cycle);
+  }
+  __syncthreads();
+  block_emit(staged, ntag, x0, y0, g.nx, a.tlist, a.n_tlist, a.tag);
+}
+
 struct TileBufs {
   int32_t *E, *P2, *bid2, *exits, *L0, *L1, *t0, *t1, *t2, *pits;
   uint32_t *S, *inflow, *tact, *tin, *tdirty;
@@ -951,6 +1180,80 @@ int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, 
     hipLaunchKernelGGL(k_mark_dirty, dim3(list_grid()), dim3(kBT), 0, s, g.N, g.nx, a.tiles_x, (const int32_t*)tag_list,
                        (const int64_t*)a.n_tlist, b.tdirty);
   return check_hip(hipGetLastError(), "tiled_upstream");
+}
+
+int tag_state(pff_ctx* ctx, const GridD& g, uint32_t** wbits, uint8_t** tag, bool begin) {
+  void *w, *t;
+  const size_t wb = (size_t)tiles_count(g) * TW * 4;
+  PFF_TRY(scratch(ctx, S_WBITS, wb, &w, true));
+  PFF_TRY(scratch(ctx, S_B1, (size_t)g.N, &t, true));
+  if (begin) {
+    if (ctx->tag_state_dirty) {
+      PFF_TRY(check_hip(hipMemsetAsync(w, 0, wb, ctx->stream), "memset"));
+      PFF_TRY(check_hip(hipMemsetAsync(t, 0, (size_t)g.N, ctx->stream), "memset"));
+    }
+    ctx->tag_state_dirty = true;  // until the carve has cleared what it sets
+  }
+  *wbits = (uint32_t*)w;
+  *tag = (uint8_t*)t;
+  return 0;
+}
+
+int saddle_tags_walk(pff_ctx* ctx, const GridD& g, const int32_t* rcv, const int32_t* bid, uint32_t* wbits,
+                     uint8_t* tag, const uint32_t* tact, int rounds, int32_t* tag_list) {
+  TileBufs b;
+  PFF_TRY(tile_bufs(ctx, g, b));
+  TileArgs a = tile_args(g, b, ctx, rcv);
+  a.tlist = tag_list;
+  a.n_tlist = ctx->dev_words + W_NTAG;
+  a.bid = bid;
+  a.tag = tag;
+  a.tact = tact;
+  hipStream_t s = ctx->stream;
+  static_assert(W_NEXIT == W_CYCLE + 1, "cycle flag and exit count are adjacent words");
+  PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_CYCLE, 0, 16, s), "memset"));
+  const dim3 tg = tile_grid(g);
+  // rcv in; tags and the ring's E and inflow out (the ring is ~6% of a tile)
+  a.prof_cells = prof_begin(ctx, "k_tag_local", 4.0);
+  hipLaunchKernelGGL(k_tag_local, tg, dim3(kTB), 0, s, a, wbits);
+  prof_end(ctx);
+  a.prof_cells = nullptr;
+  Lists L;
+  L.l[0] = b.L0;
+  L.l[1] = b.L1;
+  L.c[0] = ctx->dev_words + W_COUNT0;
+  L.c[1] = ctx->dev_words + W_COUNT1;
+  int64_t* fe = ctx->dev_words + W_EMPTY2;
+  hipLaunchKernelGGL(k_exit_link, dim3(list_grid()), dim3(kBT), 0, s, a, b.exits, a.n_exits, 0, b.L0, L.c[0], fe, b.tin,
+                     tiles_count(g));
+  if (rounds > 0) {
+    PFF_TRY(count_rounds(ctx, L, b.S, b.P2, b.t0, (uint32_t*)b.t1, b.t2, rounds, true));
+  } else {
+    const int cap = ceil_log2i(g.N) + 1;
+    int done_rounds = 0;
+    int64_t live = 1;
+    int step = std::max(2, std::min(ctx->last_tag_rounds + 1, cap));
+    while (done_rounds < cap) {
+      const int r = std::min(step, cap - done_rounds);
+      PFF_TRY(count_rounds(ctx, L, b.S, b.P2, b.t0, (uint32_t*)b.t1, b.t2, r, true, done_rounds, fe));
+      done_rounds += r;
+      step = 3;
+      PFF_TRY(read_words(ctx));
+      live = ctx->host_words[L.cur == 0 ? W_COUNT0 : W_COUNT1];
+      if (live == 0 || ctx->host_words[W_CYCLE] != 0) break;
+    }
+    const int64_t f = ctx->host_words[W_EMPTY2];
+    ctx->last_tag_rounds = (live == 0 && f < done_rounds) ? std::max<int>((int)f, 1) : done_rounds;
+    if (live != 0 || ctx->host_words[W_CYCLE] != 0) return fail(PFF_ENOTCONV, "saddle tags: receivers are not a forest");
+  }
+  a.tin = b.tin;
+  hipLaunchKernelGGL(k_exit_inflow, dim3(list_grid()), dim3(kBT), 0, s, a, b.exits, a.n_exits, b.S, b.inflow);
+  // rcv and the tile's tags in; new tags out
+  a.prof_cells = prof_begin(ctx, "k_tag_final", 5.0);
+  hipLaunchKernelGGL(k_tag_final, tg, dim3(kTB), 0, s, a);
+  prof_end(ctx);
+  a.prof_cells = nullptr;
+  return check_hip(hipGetLastError(), "saddle_tags_walk");
 }
 
 // Labels bid = init(root) with the tiled contraction, plus the depression

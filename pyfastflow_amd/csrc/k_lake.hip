@@ -747,11 +747,14 @@ __global__ __launch_bounds__(kBT) void k_cycle_flag_list(int32_t N, const int32_
   }
 }
 
+// (also sets the bit of every remaining saddle node in the tile-ordered
+// bitmap wbits, 128 words per 64x64 tile, for the carve's tag walks)
 __global__ __launch_bounds__(kBT) void k_cycle_apply_list(int32_t N, const int32_t* __restrict__ pits,
                                                           const int64_t* np, const int32_t* __restrict__ bid,
                                                           int64_t* outlet, int64_t* saddle, int32_t* snode,
                                                           const uint8_t* __restrict__ inv,
-                                                          const uint8_t* __restrict__ f42) {
+                                                          const uint8_t* __restrict__ f42, int32_t nx, int tiles_x,
+                                                          uint32_t* wbits) {
   const int64_t n = *np;
   for (int64_t w = (int64_t)blockIdx.x * kBT + threadIdx.x; w < n; w += (int64_t)gridDim.x * kBT) {
     const int b = pits[w];
@@ -766,6 +769,14 @@ __global__ __launch_bounds__(kBT) void k_cycle_apply_list(int32_t N, const int32
       outlet[b] = INVALID_KEY;
       saddle[b] = INVALID_KEY;
       snode[b] = -1;
+    } else {
+      const int sn = snode[b];
+      if (sn >= 0 && sn < N) {
+        const int row = sn / nx, col = sn - row * nx;
+        const int li = (row % TSL) * TSL + col % TSL;
+        const size_t word = (size_t)((row / TSL) * tiles_x + col / TSL) * (TSL * TSL / 32) + (li >> 5);
+        atomicOr(&wbits[word], 1u << (li & 31));
+      }
     }
   }
 }
@@ -866,8 +877,11 @@ int saddlesort_active(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t*
                                                (const int64_t*)saddle, (const int32_t*)snode, outlet));
   hipLaunchKernelGGL(k_cycle_flag_list, lg, bl, 0, s, g.N, pits, npits, bid, (const int64_t*)outlet, (uint8_t*)inv,
                      g.F42);
+  uint32_t* wbits;
+  uint8_t* tag;
+  PFF_TRY(tag_state(ctx, g, &wbits, &tag, true));
   hipLaunchKernelGGL(k_cycle_apply_list, lg, bl, 0, s, g.N, pits, npits, bid, outlet, saddle, snode,
-                     (const uint8_t*)inv, g.F42);
+                     (const uint8_t*)inv, g.F42, g.nx, tiles_x, wbits);
   return check_hip(hipGetLastError(), "saddlesort_active");
 }
 
@@ -897,12 +911,15 @@ __global__ __launch_bounds__(kBT) void k_carve_link_list(const int32_t* __restri
 }
 
 // every changed receiver belongs to a tagged cell: copy those to the working copy
+// (and clears the tag bytes of the listed cells: zero between carves)
 __global__ __launch_bounds__(kBT) void k_carve_sync_list(const int32_t* __restrict__ list, const int64_t* nlist,
-                                                         const int32_t* __restrict__ rec, int32_t* __restrict__ rec_work) {
+                                                         const int32_t* __restrict__ rec, int32_t* __restrict__ rec_work,
+                                                         uint8_t* __restrict__ tag) {
   const int64_t n = *nlist;
   for (int64_t e = (int64_t)blockIdx.x * kBT + threadIdx.x; e < n; e += (int64_t)gridDim.x * kBT) {
     const int i = list[e];
-    rec_work[i] = rec[i];
+    if (rec_work) rec_work[i] = rec[i];
+    tag[i] = 0;
   }
 }
 
@@ -915,25 +932,25 @@ __global__ __launch_bounds__(kBT) void k_carve_sync_list(const int32_t* __restri
 int carve_tiled(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t* rec_in, const int32_t* bid,
                 const int32_t* snode, const int64_t* outlet, int32_t* rec_work, const uint32_t* tact,
                 const int32_t* pits, const int64_t* npits, bool first, int rounds, int32_t* cycle) {
-  void *tag, *tl;
-  PFF_TRY(scratch(ctx, S_B1, (size_t)g.N, &tag));
+  void* tl;
+  uint32_t* wbits;
+  uint8_t* tag;
+  PFF_TRY(tag_state(ctx, g, &wbits, &tag, false));
   PFF_TRY(scratch(ctx, S_TLIST, (size_t)g.N * 4, &tl));
   // rec_in was proved a forest by this trip's labels: no cycle check needed.
-  // The tiled passes also list the tagged cells (the counter was zeroed by
-  // this trip's saddlesort_active).
+  // The tag walks also list the tagged cells (the counter was zeroed by
+  // this trip's saddlesort_active, which also set the saddle-node bits).
   const int64_t* ntag = ctx->dev_words + W_NTAG;
-  // local ends: the first trip's labels pass stored them for rec_in; later
-  // trips keep them where the previous carve changed nothing
-  PFF_TRY(tiled_upstream(ctx, g, rec_in, 2, bid, snode, 0.0f, nullptr, nullptr, (uint8_t*)tag, tact, nullptr,
-                         rounds, (int32_t*)tl, first ? 1 : 0));
+  (void)first;
+  PFF_TRY(saddle_tags_walk(ctx, g, rec_in, bid, wbits, tag, tact, rounds, (int32_t*)tl));
   *cycle = 0;
   const dim3 lg(2048), bl(kBT);
   hipLaunchKernelGGL(k_carve_reverse_list, lg, bl, 0, ctx->stream, g.N, (const int32_t*)tl, ntag,
                      (const uint8_t*)tag, rec_in, rec);
   hipLaunchKernelGGL(k_carve_link_list, dim3(1024), bl, 0, ctx->stream, pits, npits, outlet, snode, rec);
-  if (rec_work)
-    hipLaunchKernelGGL(k_carve_sync_list, lg, bl, 0, ctx->stream, (const int32_t*)tl, ntag, (const int32_t*)rec,
-                       rec_work);
+  hipLaunchKernelGGL(k_carve_sync_list, lg, bl, 0, ctx->stream, (const int32_t*)tl, ntag, (const int32_t*)rec,
+                     rec_work, tag);
+  ctx->tag_state_dirty = false;  // k_tag_local cleared the saddle bits, k_carve_sync_list the tags
   return check_hip(hipGetLastError(), "carve_tiled");
 }
 
@@ -949,8 +966,10 @@ int carve_tiled(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t* rec_i
 int carve_split(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t* rec_in, const int32_t* bid,
                 const int32_t* snode, const int64_t* outlet, int32_t* rec_work, const int32_t* levels,
                 const int32_t* fin, const int32_t* Pfinal, const int32_t* rcvp, const uint8_t* mark, int npass) {
-  void *tag, *tl, *tagw, *list;
-  PFF_TRY(scratch(ctx, S_B1, (size_t)g.N, &tag));
+  void *tl, *tagw, *list;
+  uint32_t* wbits;
+  uint8_t* tag;
+  PFF_TRY(tag_state(ctx, g, &wbits, &tag, true));
   PFF_TRY(scratch(ctx, S_TLIST, (size_t)g.N * 4, &tl));
   PFF_TRY(scratch(ctx, S_TAG, (size_t)g.N * 4, &tagw, true));
   PFF_TRY(scratch(ctx, S_MISC, (size_t)g.N * 4, &list));
@@ -980,9 +999,9 @@ int carve_split(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t* rec_i
   hipLaunchKernelGGL(k_carve_link, dim3(grid_for(g.N)), dim3(kBlock), 0, s, g.N, outlet, snode, rec);
   hipLaunchKernelGGL(k_carve_sync, dim3(stride_grid()), dim3(kBlock), 0, s, (const int32_t*)list, w,
                      (uint32_t*)tagw, rec, rec_work, (uint8_t*)nullptr);
-  if (rec_work)
-    hipLaunchKernelGGL(k_carve_sync_list, lg, bl, 0, s, (const int32_t*)tl, (const int64_t*)ntag, (const int32_t*)rec,
-                       rec_work);
+  hipLaunchKernelGGL(k_carve_sync_list, lg, bl, 0, s, (const int32_t*)tl, (const int64_t*)ntag, (const int32_t*)rec,
+                     rec_work, tag);
+  ctx->tag_state_dirty = false;
   return check_hip(hipGetLastError(), "carve_split");
 }
 

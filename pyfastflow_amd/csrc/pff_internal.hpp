@@ -27,6 +27,10 @@ struct pff_ctx {
   int last_count_rounds = 8;
   int last_tag_rounds = 8;
   int last_jump_rounds = 4;
+  // the saddle-node bitmap and the carve tag bytes are all zero between
+  // carves (each carve clears what it set); a carve that stopped half-way
+  // leaves this set and the next one clears both in full
+  bool tag_state_dirty = true;
   // opt-in per-kernel timing (pff_ctx_profile): HIP events around each
   // instrumented launch on the launch stream, plus the cells it processed
   struct ProfRec {
@@ -89,6 +93,7 @@ enum Slot : int {
   S_BL0, S_BL1,         // border cells of closed basins, this trip's and the next trip's
   S_TLIST,              // cells tagged by the tiled carve
   S_RCVP, S_MARK,       // receivers with the cycle cells made roots, cycle-cell flags
+  S_WBITS,              // saddle nodes of the trip, one bit per cell in tile order (128 words per tile)
   S_NSLOTS
 };
 static_assert(S_NSLOTS <= pff_ctx::kSlots, "pff_ctx::kSlots must cover every scratch slot");
@@ -149,6 +154,18 @@ int tiled_pits(pff_ctx* ctx, const GridD& g, int32_t** pits, int64_t** npits);
 int relabel_from_outlets(pff_ctx* ctx, const GridD& g, int32_t* bid, const int64_t* outlet, int64_t n_prev,
                          const uint32_t* tact_in, uint32_t* tact_out, int64_t* ndep, int32_t* cycle);
 int tile_flags(pff_ctx* ctx, const GridD& g, uint32_t** flags_a, uint32_t** flags_b);
+// saddle-node bitmap (S_WBITS) and carve tag bytes (S_B1), zero between
+// carves.  begin: a trip starts setting them (cleared first if the last carve
+// did not finish); the carve that clears them resets tag_state_dirty.
+int tag_state(pff_ctx* ctx, const GridD& g, uint32_t** wbits, uint8_t** tag, bool begin);
+// Carve tags of one trip on a forest of D4 receivers whose saddle nodes are
+// set in wbits (the bits are cleared): tag[c] = 1 for every cell with a saddle
+// node upstream, each listed once in tag_list (count W_NTAG, zeroed by the
+// caller).  Per tile, the paths from the saddle nodes and from the tile's
+// ring cells are walked in LDS; only their ends enter the reduced graph.
+// rounds > 0: known bound of the reduced rounds (no host read).
+int saddle_tags_walk(pff_ctx* ctx, const GridD& g, const int32_t* rcv, const int32_t* bid, uint32_t* wbits,
+                     uint8_t* tag, const uint32_t* tact, int rounds, int32_t* tag_list);
 int tiles_count(const GridD& g);
 int carve_tiled(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t* rec_in, const int32_t* bid,
                 const int32_t* snode, const int64_t* outlet, int32_t* rec_work, const uint32_t* tact,
