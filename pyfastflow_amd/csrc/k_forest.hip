@@ -471,6 +471,14 @@ struct TileArgs {
   // the others, whose tags the local pass already wrote)
   uint32_t* tin;
   // saddle tags: list of the tagged cells (nullable), each cell once
+  // carve tag walks: the reduced graph lives on ring slots (256 per tile,
+  // slot = tile * 256 + ring index); ringE[slot] = slot of the ring cell's
+  // local end if that is an exit, else the terminal slot; ringIn = tagged
+  // inflow of the ring cell; ringS = saddle flag of an exit slot
+  int32_t* ringE;
+  uint32_t* ringIn;
+  uint32_t* ringS;
+  int ring_term;  // terminal slot (ntiles * 256): its pointer is -1
   int32_t* tlist;
   int64_t* n_tlist;
   // saddle tags across lakeflow trips (nullable): tiles whose receivers the
@@ -825,6 +833,46 @@ __global__ __launch_bounds__(kBT) void k_label_tiles(TileArgs a, int32_t* __rest
 }
 
 // ---------------------------------------------------------------------------
+// Per-tile list segments: a tile block writes its entries to its own 4096
+// slots (seg + tile * TC) and its count; k_seg_scan turns the counts into
+// offsets (after the list's current length when appending) and the new
+// length, k_seg_copy moves the segments into the contiguous list.  One
+// block of k_seg_scan per list (blockIdx.x), one block of k_seg_copy per
+// tile and list.
+// ---------------------------------------------------------------------------
+struct SegLists {
+  int32_t* seg[2];
+  int32_t* cnt[2];  // per tile
+  int32_t* off[2];  // per tile
+  int32_t* list[2];
+  int64_t* count[2];
+  int append[2];
+};
+
+__global__ __launch_bounds__(1024) void k_seg_scan(SegLists s, int ntiles) {
+  const int l = blockIdx.x;
+  const int per = (ntiles + 1023) / 1024;
+  const int t0 = min((int)threadIdx.x * per, ntiles), t1 = min(t0 + per, ntiles);
+  const int64_t base = s.append[l] ? *s.count[l] : 0;
+  int sum = 0;
+  for (int t = t0; t < t1; ++t) sum += s.cnt[l][t];
+  int total;
+  int acc = (int)base + block_exclusive_scan<1024>(sum, &total);
+  for (int t = t0; t < t1; ++t) {
+    s.off[l][t] = acc;
+    acc += s.cnt[l][t];
+  }
+  if (threadIdx.x == 0) *s.count[l] = base + total;  // every thread read base before the scan's barriers
+}
+
+__global__ __launch_bounds__(kBT) void k_seg_copy(SegLists s) {
+  const int t = blockIdx.x, l = blockIdx.y;
+  const int n = s.cnt[l][t], o = s.off[l][t];
+  const int32_t* src = s.seg[l] + (size_t)t * TC;
+  for (int k = threadIdx.x; k < n; k += kBT) s.list[l][o + k] = src[k];
+}
+
+// ---------------------------------------------------------------------------
 // Carve tags by walks.  A tile's carve tags only come from its saddle nodes
 // (the paths from them down to the tile's local ends) and from the tagged
 // flow that enters it through ring cells (a D4 receiver in another tile lies
@@ -836,12 +884,32 @@ __global__ __launch_bounds__(kBT) void k_label_tiles(TileArgs a, int32_t* __rest
 // stops at a cell already tagged: that cell's walker carries on downstream.
 // ---------------------------------------------------------------------------
 constexpr int TW = TC / 32;  // bitmap words per tile
+constexpr int kWB = 256;     // threads per tile block of the walk passes (more tiles in flight per CU)
+constexpr int CPW = TC / kWB;
 
 __device__ __forceinline__ bool bit_test_set(uint32_t* bm, int li) {
   const uint32_t b = 1u << (li & 31);
   return (atomicOr(&bm[li >> 5], b) & b) != 0u;
 }
 __device__ __forceinline__ bool bit_test(const uint32_t* bm, int li) { return (bm[li >> 5] >> (li & 31)) & 1u; }
+
+// slot of this lane in an LDS staging list: one atomic per wave for the
+// lanes active at the call
+__device__ __forceinline__ int wave_slot(int* count) {
+  const uint64_t m = __ballot(1);
+  const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  int base = 0;
+  if ((int)(threadIdx.x & 63) == leader) base = atomicAdd(count, __popcll(m));
+  return __shfl(base, leader) + (int)below;
+}
+
+// store a 64-cell row flag (one wave, lanes in cell order) into a bitmap
+__device__ __forceinline__ void row_bits(uint32_t* bm, int li_lane0, bool flag) {
+  const uint64_t m = __ballot(flag);
+  const int lane = threadIdx.x & 63;
+  if (lane < 2) bm[(li_lane0 >> 5) + lane] = (uint32_t)(m >> (32 * lane));
+}
 
 // k-th ring cell (local index) of a tw x th tile; ring_count cells in all
 __device__ __forceinline__ int ring_count(int tw, int th) {
@@ -861,21 +929,70 @@ __device__ __forceinline__ int ring_cell(int k, int tw, int th) {
   return (k + 1) * TS + tw - 1;
 }
 
-// local pointers of the tile into P (NONE: root or leaves the tile), roots
-// into the bitmap R (zeroed by the caller)
-__device__ __forceinline__ void tile_load_pointers(const TileArgs& a, int x0, int y0, int tw, int th, uint16_t* P,
-                                                   uint32_t* R) {
+// A tile's receivers, loaded before any of them is used: on gfx9 a wait for
+// a load also waits for every earlier store, so each pass of the walk
+// kernels is ordered loads, then decisions, then stores.
+__device__ __forceinline__ void tile_load_rcv(const TileArgs& a, int x0, int y0, int tw, int th, int (&rv)[CPW]) {
 #pragma unroll
-  for (int j = 0; j < CPT; ++j) {
-    const int li = threadIdx.x + j * kTB;
+  for (int j = 0; j < CPW; ++j) {
+    const int li = threadIdx.x + j * kWB;
+    const int ly = li >> 6, lx = li & 63;
+    rv[j] = -2;
+    if (lx < tw && ly < th) rv[j] = a.rcv[(y0 + ly) * a.g.nx + x0 + lx];
+  }
+}
+
+// inverse of ring_cell for a ring cell (ly, lx)
+__device__ __forceinline__ int ring_index(int ly, int lx, int tw, int th) {
+  if (th == 1) return lx;
+  if (tw == 1) return ly;
+  if (ly == 0) return lx;
+  if (ly == th - 1) return tw + lx;
+  if (lx == 0) return 2 * tw + ly - 1;
+  return 2 * tw + (th - 2) + ly - 1;
+}
+constexpr int kRing = 256;  // ring slots per tile (2*64 + 2*62 = 252 used)
+
+__device__ __forceinline__ void tile_dims(const GridD& g, int tx, int ty, int& tw, int& th) {
+  tw = min(TS, g.nx - tx * TS);
+  th = min(TS, g.ny - ty * TS);
+}
+
+// ring slot of a cell that lies on its tile's ring
+__device__ __forceinline__ int slot_of(const TileArgs& a, int c) {
+  const int row = (int)fdiv((uint32_t)c, a.fd), col = c - row * a.g.nx;
+  const int tx = col / TS, ty = row / TS;
+  int tw, th;
+  tile_dims(a.g, tx, ty, tw, th);
+  return (ty * a.tiles_x + tx) * kRing + ring_index(row - ty * TS, col - tx * TS, tw, th);
+}
+
+// cell of a ring slot
+__device__ __forceinline__ int slot_cell(const TileArgs& a, int x) {
+  const int tile = x / kRing, k = x - tile * kRing;
+  const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
+  int tw, th;
+  tile_dims(a.g, tx, ty, tw, th);
+  const int li = ring_cell(k, tw, th);
+  return (ty * TS + (li >> 6)) * a.g.nx + tx * TS + (li & 63);
+}
+
+// local pointers of the tile into P (NONE: root or leaves the tile), roots
+// into the bitmap R (each wave writes the words of its row)
+__device__ __forceinline__ void tile_build_pointers(const TileArgs& a, int x0, int y0, int tw, int th,
+                                                    const int (&rv)[CPW], uint16_t* P, uint32_t* R) {
+#pragma unroll
+  for (int j = 0; j < CPW; ++j) {
+    const int li = threadIdx.x + j * kWB;
     const int ly = li >> 6, lx = li & 63;
     uint16_t lp = NONE;
-    if (lx < tw && ly < th) {
+    bool root = false;
+    if (rv[j] != -2) {
       const int c = (y0 + ly) * a.g.nx + x0 + lx;
-      int r = a.rcv[c];
+      int r = rv[j];
       if ((unsigned)r >= (unsigned)a.g.N) r = c;
       if (r == c) {
-        atomicOr(&R[li >> 5], 1u << (li & 31));
+        root = true;
       } else {
         const int rr = (int)fdiv((uint32_t)r, a.fd);
         const int dy = rr - y0, dxx = r - rr * a.g.nx - x0;
@@ -883,18 +1000,16 @@ __device__ __forceinline__ void tile_load_pointers(const TileArgs& a, int x0, in
       }
     }
     P[li] = lp;
+    row_bits(R, li - lx, root);
   }
 }
 
-// tag the path from li down to the first tagged cell or the local end; the
-// newly tagged cells go to the staging list.  Returns the local end if the
-// walk reached it (the end is then tagged by this walk), else -1.
-__device__ __forceinline__ int walk_tag(int li, const uint16_t* P, uint32_t* T, uint16_t* staged, int* nstaged,
-                                        int64_t* cycle) {
+// tag the path from li down to the first tagged cell or the local end.
+// Returns the local end if this walk tagged it, else -1.
+__device__ __forceinline__ int walk_tag(int li, const uint16_t* P, uint32_t* T, int64_t* cycle) {
   int c = li;
   for (int s = 0; s < TC; ++s) {
     if (bit_test_set(T, c)) return -1;
-    staged[atomicAdd(nstaged, 1)] = (uint16_t)c;
     const uint16_t n = P[c];
     if (n == NONE) return c;
     c = n;
@@ -914,129 +1029,191 @@ __device__ __forceinline__ int walk_end(int li, const uint16_t* P, int64_t* cycl
   return li;
 }
 
-// append the staged local cells (global ids) to list, one atomic per block
-__device__ __forceinline__ void block_emit(const uint16_t* staged, int n, int x0, int y0, int nx, int32_t* list,
-                                           int64_t* count, uint8_t* tag) {
-  __shared__ int64_t base_s;
-  if (threadIdx.x == 0) base_s = n ? (int64_t)atomicAdd((unsigned long long*)count, (unsigned long long)n) : 0;
-  __syncthreads();
-  for (int k = threadIdx.x; k < n; k += kTB) {
-    const int li = staged[k];
-    const int c = (y0 + (li >> 6)) * nx + x0 + (li & 63);
-    if (tag) tag[c] = 1;
-    list[base_s + k] = c;
-  }
-}
-
 // local pass: tags below the tile's saddle nodes (wbits, cleared here), the
 // local end E of every ring cell, the reduced-graph nodes (the ends those
 // walks reach: listed exits with S = 1 if a saddle path reaches them, or
 // terminals P2 = -1), inflow = 0 on the ring
-__global__ __launch_bounds__(kTB) void k_tag_local(TileArgs a, uint32_t* __restrict__ wbits) {
+__global__ __launch_bounds__(kWB) void k_tag_local(TileArgs a, uint32_t* __restrict__ wbits, SegLists sl) {
   __shared__ uint16_t P[TC];
-  __shared__ uint16_t staged[TC];
-  __shared__ uint16_t ends[TC];
+  __shared__ uint16_t wl[TC];  // saddle nodes of the tile
   __shared__ uint32_t T[TW], Lb[TW], F[TW], R[TW], Wb[TW];
-  __shared__ int ntag, nend, nexit;
-  if (!tile_active(a.tact, a.tiles_x, blockIdx.x, blockIdx.y)) return;  // uniform per block
-  const GridD& g = a.g;
+  __shared__ int nw;
   const int tile = blockIdx.y * a.tiles_x + blockIdx.x;
+  if (!tile_active(a.tact, a.tiles_x, blockIdx.x, blockIdx.y)) {  // uniform per block
+    if (threadIdx.x == 0) {
+      sl.cnt[0][tile] = 0;
+      sl.cnt[1][tile] = 0;
+    }
+    return;
+  }
+  const GridD& g = a.g;
   const int x0 = blockIdx.x * TS, y0 = blockIdx.y * TS;
   const int tw = min(TS, g.nx - x0), th = min(TS, g.ny - y0);
   if (a.prof_cells && threadIdx.x == 0) atomicAdd(a.prof_cells, (unsigned long long)(tw * th));
+  int rv[CPW];
+  tile_load_rcv(a, x0, y0, tw, th, rv);
+  const uint32_t w = threadIdx.x < TW ? wbits[(size_t)tile * TW + threadIdx.x] : 0u;
   if (threadIdx.x < TW) {
     T[threadIdx.x] = 0u;
     Lb[threadIdx.x] = 0u;
     F[threadIdx.x] = 0u;
-    R[threadIdx.x] = 0u;
-    const uint32_t w = wbits[(size_t)tile * TW + threadIdx.x];
     Wb[threadIdx.x] = w;
-    if (w) wbits[(size_t)tile * TW + threadIdx.x] = 0u;
   }
-  if (threadIdx.x == 0) {
-    ntag = 0;
-    nend = 0;
-    nexit = 0;
+  if (threadIdx.x == 0) nw = 0;
+  tile_build_pointers(a, x0, y0, tw, th, rv, P, R);
+  __syncthreads();
+  if (w) wbits[(size_t)tile * TW + threadIdx.x] = 0u;
+  // saddle nodes into a list, so that their walks spread over all lanes
+#pragma unroll
+  for (int j = 0; j < CPW; ++j) {
+    const int li = threadIdx.x + j * kWB;
+    if (bit_test(Wb, li)) wl[wave_slot(&nw)] = (uint16_t)li;
   }
   __syncthreads();
-  tile_load_pointers(a, x0, y0, tw, th, P, R);
-  __syncthreads();
-  // saddle paths
-#pragma unroll 1
-  for (int j = 0; j < CPT; ++j) {
-    const int li = threadIdx.x + j * kTB;
-    if (!bit_test(Wb, li)) continue;
-    const int e = walk_tag(li, P, T, staged, &ntag, a.cycle);
+  const int nwl = nw;
+  for (int k = threadIdx.x; k < nwl; k += kWB) {
+    const int e = walk_tag(wl[k], P, T, a.cycle);
     if (e >= 0) {
       atomicOr(&F[e >> 5], 1u << (e & 31));
-      if (!bit_test_set(Lb, e)) ends[atomicAdd(&nend, 1)] = (uint16_t)e;
+      atomicOr(&Lb[e >> 5], 1u << (e & 31));
     }
   }
-  // ring cells
+  // ring cells: their local ends
+  __shared__ uint16_t rend[kRing];
+  __shared__ uint32_t Xb[TW];
   const int nr = ring_count(tw, th);
-  for (int k = threadIdx.x; k < nr; k += kTB) {
+  for (int k = threadIdx.x; k < nr; k += kWB) {
+    const int e = walk_end(ring_cell(k, tw, th), P, a.cycle);
+    rend[k] = (uint16_t)e;
+    atomicOr(&Lb[e >> 5], 1u << (e & 31));
+  }
+  __syncthreads();
+  // reached ends: roots and ends outside closed basins are terminals, the
+  // others exits (on the ring: the receiver is in another tile)
+  auto cell_of = [&](int j) {
+    const int li = threadIdx.x + j * kWB;
+    return (y0 + (li >> 6)) * g.nx + x0 + (li & 63);
+  };
+  const int base = tile * kRing;
+  int bv[CPW];
+#pragma unroll
+  for (int j = 0; j < CPW; ++j) {
+    const int li = threadIdx.x + j * kWB;
+    bv[j] = 1;
+    if (bit_test(Lb, li) && !bit_test(R, li)) bv[j] = a.bid[cell_of(j)];
+  }
+  uint32_t tmask = 0u;
+#pragma unroll
+  for (int j = 0; j < CPW; ++j) {
+    const int li = threadIdx.x + j * kWB;
+    if (bit_test(T, li)) tmask |= 1u << j;
+    row_bits(Xb, li - (li & 63), bit_test(Lb, li) && !bit_test(R, li) && bv[j] != 0);
+  }
+  __syncthreads();
+  // one ring slot per thread (nr <= 252 < kWB): its end's slot, its inflow,
+  // and, if the ring cell is itself an exit, its saddle flag
+  int32_t xs[1] = {0};
+  bool xt[1] = {false};
+  if ((int)threadIdx.x < nr) {
+    const int k = threadIdx.x;
+    const int e = rend[k];
+    a.ringE[base + k] = bit_test(Xb, e) ? base + ring_index(e >> 6, e & 63, tw, th) : a.ring_term;
+    a.ringIn[base + k] = 0u;
     const int li = ring_cell(k, tw, th);
-    const int e = walk_end(li, P, a.cycle);
-    const int c = (y0 + (li >> 6)) * g.nx + x0 + (li & 63);
-    a.E[c] = (y0 + (e >> 6)) * g.nx + x0 + (e & 63);
-    a.inflow[c] = 0u;
-    if (!bit_test_set(Lb, e)) ends[atomicAdd(&nend, 1)] = (uint16_t)e;
+    xt[0] = bit_test(Xb, li);
+    xs[0] = base + k;
+    a.ringS[base + k] = (xt[0] && bit_test(F, li)) ? 1u : 0u;
   }
-  __syncthreads();
-  block_emit(staged, ntag, x0, y0, g.nx, a.tlist, a.n_tlist, a.tag);
-  // ends: roots and ends outside closed basins are terminals; the others
-  // are exits (their receiver is in another tile) with their saddle flag
-  __syncthreads();
-  const int ne = nend;
-  for (int k = threadIdx.x; k < ne; k += kTB) {
-    const int e = ends[k];
-    const int c = (y0 + (e >> 6)) * g.nx + x0 + (e & 63);
-    if (bit_test(R, e) || a.bid[c] == 0) {
-      a.P2[c] = -1;
-    } else {
-      a.S[c] = bit_test(F, e) ? 1u : 0u;
-      staged[atomicAdd(&nexit, 1)] = (uint16_t)e;
-    }
-  }
-  __syncthreads();
-  block_emit(staged, nexit, x0, y0, g.nx, a.exits, a.n_exits, nullptr);
+#pragma unroll
+  for (int j = 0; j < CPW; ++j)
+    if ((tmask >> j) & 1u) a.tag[cell_of(j)] = 1;
+  seg_append_mask<CPW, kWB>(tmask, cell_of, sl.seg[0] + (size_t)tile * TC, &sl.cnt[0][tile]);
+  seg_append<1, kWB>(xs, xt, sl.seg[1] + (size_t)tile * TC, &sl.cnt[1][tile]);
 }
 
 // final pass on the tiles that receive a tag: walk from the ring cells with
 // a tagged inflow, tagging until a tagged cell; list the new tags
-__global__ __launch_bounds__(kTB) void k_tag_final(TileArgs a) {
+__global__ __launch_bounds__(kWB, 8) void k_tag_final(TileArgs a, SegLists sl) {
   __shared__ uint16_t P[TC];
-  __shared__ uint16_t staged[TC];
-  __shared__ uint32_t T[TW], R[TW];
-  __shared__ int ntag;
-  if (!tile_active(a.tact, a.tiles_x, blockIdx.x, blockIdx.y)) return;  // uniform per block
-  if (a.tin[blockIdx.y * a.tiles_x + blockIdx.x] == 0u) return;      // uniform per block
+  __shared__ uint32_t T[TW], T0[TW], R[TW];
+  const int tile = blockIdx.y * a.tiles_x + blockIdx.x;
+  if (!tile_active(a.tact, a.tiles_x, blockIdx.x, blockIdx.y) || a.tin[tile] == 0u) {  // uniform per block
+    if (threadIdx.x == 0) sl.cnt[0][tile] = 0;
+    return;
+  }
   const GridD& g = a.g;
   const int x0 = blockIdx.x * TS, y0 = blockIdx.y * TS;
   const int tw = min(TS, g.nx - x0), th = min(TS, g.ny - y0);
   if (a.prof_cells && threadIdx.x == 0) atomicAdd(a.prof_cells, (unsigned long long)(tw * th));
-  if (threadIdx.x < TW) {
-    T[threadIdx.x] = 0u;
-    R[threadIdx.x] = 0u;
-  }
-  if (threadIdx.x == 0) ntag = 0;
-  __syncthreads();
-  tile_load_pointers(a, x0, y0, tw, th, P, R);
+  int rv[CPW];
+  tile_load_rcv(a, x0, y0, tw, th, rv);
+  uint8_t tb[CPW];
 #pragma unroll
-  for (int j = 0; j < CPT; ++j) {
-    const int li = threadIdx.x + j * kTB;
+  for (int j = 0; j < CPW; ++j) {
+    const int li = threadIdx.x + j * kWB;
     const int ly = li >> 6, lx = li & 63;
-    if (lx < tw && ly < th && a.tag[(y0 + ly) * g.nx + x0 + lx]) atomicOr(&T[li >> 5], 1u << (li & 31));
+    tb[j] = (lx < tw && ly < th) ? a.tag[(y0 + ly) * g.nx + x0 + lx] : (uint8_t)0;
+  }
+  tile_build_pointers(a, x0, y0, tw, th, rv, P, R);
+#pragma unroll
+  for (int j = 0; j < CPW; ++j) {
+    const int li = threadIdx.x + j * kWB;
+    row_bits(T, li - (li & 63), tb[j] != 0);
+    row_bits(T0, li - (li & 63), tb[j] != 0);
   }
   __syncthreads();
   const int nr = ring_count(tw, th);
-  for (int k = threadIdx.x; k < nr; k += kTB) {
-    const int li = ring_cell(k, tw, th);
-    const int c = (y0 + (li >> 6)) * g.nx + x0 + (li & 63);
-    if (a.inflow[c] != 0u) (void)walk_tag(li, P, T, staged, &ntag, a.cycle);
-  }
+  for (int k = threadIdx.x; k < nr; k += kWB)
+    if (a.ringIn[tile * kRing + k] != 0u) (void)walk_tag(ring_cell(k, tw, th), P, T, a.cycle);
   __syncthreads();
-  block_emit(staged, ntag, x0, y0, g.nx, a.tlist, a.n_tlist, a.tag);
+  auto cell_of = [&](int j) {
+    const int li = threadIdx.x + j * kWB;
+    return (y0 + (li >> 6)) * g.nx + x0 + (li & 63);
+  };
+  uint32_t tmask = 0u;
+#pragma unroll
+  for (int j = 0; j < CPW; ++j) {
+    const int li = threadIdx.x + j * kWB;
+    if (bit_test(T, li) && !bit_test(T0, li)) {
+      tmask |= 1u << j;
+      a.tag[cell_of(j)] = 1;
+    }
+  }
+  seg_append_mask<CPW, kWB>(tmask, cell_of, sl.seg[0] + (size_t)tile * TC, &sl.cnt[0][tile]);
+}
+
+// reduced graph of the tag walks: exit slot x -> the slot of the local end
+// its receiver's ring cell reaches (or the terminal slot), list set-up
+__global__ __launch_bounds__(kBT) void k_ring_link(TileArgs a, const int32_t* __restrict__ exits, const int64_t* count,
+                                                   int32_t* __restrict__ P2, int32_t* __restrict__ list_out,
+                                                   int64_t* list_count, int64_t* first_empty, uint32_t* tin,
+                                                   int ntiles) {
+  const int64_t n = *count;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *list_count = n;
+    *first_empty = (int64_t)1 << 40;
+    P2[a.ring_term] = -1;
+  }
+  for (int t = blockIdx.x * kBT + threadIdx.x; t < ntiles; t += gridDim.x * kBT) tin[t] = 0u;
+  for (int64_t w = (int64_t)blockIdx.x * kBT + threadIdx.x; w < n; w += (int64_t)gridDim.x * kBT) {
+    const int x = exits[w];
+    P2[x] = a.ringE[slot_of(a, a.rcv[slot_cell(a, x)])];
+    list_out[w] = x;
+  }
+}
+
+// flagged exits pass their flag to the ring cell they drain into
+__global__ __launch_bounds__(kBT) void k_ring_inflow(TileArgs a, const int32_t* __restrict__ exits, const int64_t* count,
+                                                     uint32_t* tin) {
+  const int64_t n = *count;
+  for (int64_t w = (int64_t)blockIdx.x * kBT + threadIdx.x; w < n; w += (int64_t)gridDim.x * kBT) {
+    const int x = exits[w];
+    const uint32_t v = a.ringS[x];
+    if (v == 0u) continue;
+    const int slot = slot_of(a, a.rcv[slot_cell(a, x)]);
+    atomicAdd(&a.ringIn[slot], v);
+    tin[slot / kRing] = 1u;
+  }
 }
 
 struct TileBufs {
@@ -1213,21 +1390,54 @@ int saddle_tags_walk(pff_ctx* ctx, const GridD& g, const int32_t* rcv, const int
   static_assert(W_NEXIT == W_CYCLE + 1, "cycle flag and exit count are adjacent words");
   PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_CYCLE, 0, 16, s), "memset"));
   const dim3 tg = tile_grid(g);
+  const int nt = tiles_count(g);
+  const int nslots = nt * kRing;
+  int32_t* ringP2;
+  {
+    void* rg;
+    PFF_TRY(scratch(ctx, S_RING, (size_t)(nslots + 1) * 4 * 4, &rg));
+    a.ringE = (int32_t*)rg;
+    a.ringIn = (uint32_t*)(a.ringE + (nslots + 1));
+    a.ringS = a.ringIn + (nslots + 1);
+    ringP2 = (int32_t*)(a.ringS + (nslots + 1));
+    a.ring_term = nslots;
+  }
+  SegLists sl{};
+  {
+    void *d0, *d1, *co;
+    PFF_TRY(scratch(ctx, S_SEGD0, (size_t)nt * TC * 4, &d0));
+    PFF_TRY(scratch(ctx, S_SEGD1, (size_t)nt * TC * 4, &d1));
+    PFF_TRY(scratch(ctx, S_SEGCO, (size_t)nt * 4 * 4, &co));
+    sl.seg[0] = (int32_t*)d0;
+    sl.seg[1] = (int32_t*)d1;
+    sl.cnt[0] = (int32_t*)co;
+    sl.cnt[1] = sl.cnt[0] + nt;
+    sl.off[0] = sl.cnt[1] + nt;
+    sl.off[1] = sl.off[0] + nt;
+    sl.list[0] = tag_list;
+    sl.count[0] = a.n_tlist;
+    sl.append[0] = 1;
+    sl.list[1] = b.exits;
+    sl.count[1] = a.n_exits;
+    sl.append[1] = 0;
+  }
   // rcv in; tags and the ring's E and inflow out (the ring is ~6% of a tile)
   a.prof_cells = prof_begin(ctx, "k_tag_local", 4.0);
-  hipLaunchKernelGGL(k_tag_local, tg, dim3(kTB), 0, s, a, wbits);
+  hipLaunchKernelGGL(k_tag_local, tg, dim3(kWB), 0, s, a, wbits, sl);
   prof_end(ctx);
   a.prof_cells = nullptr;
+  hipLaunchKernelGGL(k_seg_scan, dim3(2), dim3(1024), 0, s, sl, nt);
+  hipLaunchKernelGGL(k_seg_copy, dim3(nt, 2), dim3(kBT), 0, s, sl);
   Lists L;
   L.l[0] = b.L0;
   L.l[1] = b.L1;
   L.c[0] = ctx->dev_words + W_COUNT0;
   L.c[1] = ctx->dev_words + W_COUNT1;
   int64_t* fe = ctx->dev_words + W_EMPTY2;
-  hipLaunchKernelGGL(k_exit_link, dim3(list_grid()), dim3(kBT), 0, s, a, b.exits, a.n_exits, 0, b.L0, L.c[0], fe, b.tin,
-                     tiles_count(g));
+  hipLaunchKernelGGL(k_ring_link, dim3(list_grid()), dim3(kBT), 0, s, a, (const int32_t*)b.exits,
+                     (const int64_t*)a.n_exits, ringP2, b.L0, L.c[0], fe, b.tin, nt);
   if (rounds > 0) {
-    PFF_TRY(count_rounds(ctx, L, b.S, b.P2, b.t0, (uint32_t*)b.t1, b.t2, rounds, true));
+    PFF_TRY(count_rounds(ctx, L, a.ringS, ringP2, b.t0, (uint32_t*)b.t1, b.t2, rounds, true));
   } else {
     const int cap = ceil_log2i(g.N) + 1;
     int done_rounds = 0;
@@ -1235,7 +1445,7 @@ int saddle_tags_walk(pff_ctx* ctx, const GridD& g, const int32_t* rcv, const int
     int step = std::max(2, std::min(ctx->last_tag_rounds + 1, cap));
     while (done_rounds < cap) {
       const int r = std::min(step, cap - done_rounds);
-      PFF_TRY(count_rounds(ctx, L, b.S, b.P2, b.t0, (uint32_t*)b.t1, b.t2, r, true, done_rounds, fe));
+      PFF_TRY(count_rounds(ctx, L, a.ringS, ringP2, b.t0, (uint32_t*)b.t1, b.t2, r, true, done_rounds, fe));
       done_rounds += r;
       step = 3;
       PFF_TRY(read_words(ctx));
@@ -1247,12 +1457,15 @@ int saddle_tags_walk(pff_ctx* ctx, const GridD& g, const int32_t* rcv, const int
     if (live != 0 || ctx->host_words[W_CYCLE] != 0) return fail(PFF_ENOTCONV, "saddle tags: receivers are not a forest");
   }
   a.tin = b.tin;
-  hipLaunchKernelGGL(k_exit_inflow, dim3(list_grid()), dim3(kBT), 0, s, a, b.exits, a.n_exits, b.S, b.inflow);
+  hipLaunchKernelGGL(k_ring_inflow, dim3(list_grid()), dim3(kBT), 0, s, a, (const int32_t*)b.exits,
+                     (const int64_t*)a.n_exits, b.tin);
   // rcv and the tile's tags in; new tags out
   a.prof_cells = prof_begin(ctx, "k_tag_final", 5.0);
-  hipLaunchKernelGGL(k_tag_final, tg, dim3(kTB), 0, s, a);
+  hipLaunchKernelGGL(k_tag_final, tg, dim3(kWB), 0, s, a, sl);
   prof_end(ctx);
   a.prof_cells = nullptr;
+  hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(1024), 0, s, sl, nt);
+  hipLaunchKernelGGL(k_seg_copy, dim3(nt, 1), dim3(kBT), 0, s, sl);
   return check_hip(hipGetLastError(), "saddle_tags_walk");
 }
 
@@ -1481,19 +1694,25 @@ __global__ __launch_bounds__(kBT) void k_relabel_tiles(GridD g, int tiles_x, con
     return;
   }
   const int col = bx * TS + threadIdx.x;
+  constexpr int RR = TS / 4;  // rows per thread
+  // all loads, then all gathers, then the stores (a wait for a load also
+  // waits for every earlier store)
+  int b[RR], nb[RR];
+#pragma unroll
+  for (int k = 0; k < RR; ++k) {
+    const int row = by * TS + threadIdx.y + 4 * k;
+    b[k] = (col < g.nx && row < g.ny) ? bid[row * g.nx + col] : 0;
+  }
+#pragma unroll
+  for (int k = 0; k < RR; ++k) nb[k] = (b[k] > 0 && b[k] < g.N) ? M[b[k]] : 0;
   int any = 0;
-  if (col < g.nx)
-    for (int rr = threadIdx.y; rr < TS; rr += 4) {
-      const int row = by * TS + rr;
-      if (row >= g.ny) break;
-      const int c = row * g.nx + col;
-      const int b = bid[c];
-      if (b != 0) {
-        const int nb = (b > 0 && b < g.N) ? M[b] : 0;
-        if (nb != b) bid[c] = nb;
-        any |= nb != 0;
-      }
-    }
+#pragma unroll
+  for (int k = 0; k < RR; ++k) {
+    if (b[k] == 0) continue;
+    const int row = by * TS + threadIdx.y + 4 * k;
+    if (nb[k] != b[k]) bid[row * g.nx + col] = nb[k];
+    any |= nb[k] != 0;
+  }
   any = __syncthreads_or(any);
   if (threadIdx.x == 0 && threadIdx.y == 0) tact_out[t] = any ? 1u : 0u;
 }

@@ -96,6 +96,38 @@ __device__ __forceinline__ void block_append_ordered(const int32_t (&val)[R], co
   __syncthreads();
 }
 
+// Compact the taken values of every thread into this block's own segment
+// (seg, room for every item of the block) and store the count: no global
+// atomic.  Appends of thousands of blocks to one shared counter serialise at
+// the counter's cache line (about 12 ns each); per-block segments are
+// joined afterwards by k_seg_scan and a copy.  Must be reached by every
+// thread.
+template <int ITEMS, int BT = kBT>
+__device__ __forceinline__ void seg_append(const int32_t (&val)[ITEMS], const bool (&take)[ITEMS], int32_t* seg,
+                                           int32_t* count) {
+  int mine = 0;
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) mine += take[j] ? 1 : 0;
+  int total;
+  int p = block_exclusive_scan<BT>(mine, &total);
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j)
+    if (take[j]) seg[p++] = val[j];
+  if (threadIdx.x == 0 && threadIdx.y == 0) *count = total;
+}
+
+// seg_append with the taken items as bits of a mask and their values from
+// val(j)
+template <int ITEMS, int BT, class F>
+__device__ __forceinline__ void seg_append_mask(uint32_t take, F val, int32_t* seg, int32_t* count) {
+  int total;
+  int p = block_exclusive_scan<BT>(__popc(take), &total);
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j)
+    if ((take >> j) & 1u) seg[p++] = val(j);
+  if (threadIdx.x == 0 && threadIdx.y == 0) *count = total;
+}
+
 // Block-wide sum with one atomicAdd on a 64-bit counter.
 template <int BT = kBT>
 __device__ __forceinline__ void block_count(int mine, int64_t* counter) {

@@ -94,6 +94,9 @@ enum Slot : int {
   S_TLIST,              // cells tagged by the tiled carve
   S_RCVP, S_MARK,       // receivers with the cycle cells made roots, cycle-cell flags
   S_WBITS,              // saddle nodes of the trip, one bit per cell in tile order (128 words per tile)
+  S_SEGD0, S_SEGD1,     // per-tile list segments (4096 entries per tile), two lists
+  S_SEGCO,              // their per-tile counts and offsets
+  S_RING,               // ring-slot arrays of the carve tag walks (4 x (256 per tile + 1))
   S_NSLOTS
 };
 static_assert(S_NSLOTS <= pff_ctx::kSlots, "pff_ctx::kSlots must cover every scratch slot");
