@@ -513,11 +513,20 @@ __device__ __forceinline__ bool tile_active(const uint32_t* tact, int tiles_x, i
 
 // PH 1: local pass; PH 3: final pass (weights += inflow, emit results)
 template <int MODE, int PH, int W>
-__global__ __launch_bounds__(kTB) void k_tile(TileArgs a) {
+__global__ __launch_bounds__(kTB) void k_tile(TileArgs a, SegLists sl) {
   __shared__ uint16_t P[TC];
   __shared__ uint16_t E[(PH == 1) ? TC : 1];
   __shared__ uint32_t S[(W != W_ROOTS) ? TC : 1];
-  if (!tile_active(a.tact, a.tiles_x, blockIdx.x, blockIdx.y)) return;  // uniform per block
+  const int tile_id = blockIdx.y * a.tiles_x + blockIdx.x;
+  if (!tile_active(a.tact, a.tiles_x, blockIdx.x, blockIdx.y)) {  // uniform per block
+    if constexpr (PH == 1) {
+      if (threadIdx.x == 0) {
+        sl.cnt[0][tile_id] = 0;
+        if constexpr (W == W_ROOTS) sl.cnt[1][tile_id] = 0;
+      }
+    }
+    return;
+  }
   if constexpr (PH == 3 && W == W_SADDLE) {
     if (a.tin && a.tin[blockIdx.y * a.tiles_x + blockIdx.x] == 0u) return;  // uniform per block
   }
@@ -700,8 +709,8 @@ __global__ __launch_bounds__(kTB) void k_tile(TileArgs a) {
         }
       }
     }
-    if constexpr (W == W_ROOTS) block_append<CPT, kTB>(pv, tpit, a.pits, a.n_pits);
-    block_append<CPT, kTB>(ex, take, a.exits, a.n_exits);
+    if constexpr (W == W_ROOTS) seg_append<CPT, kTB>(pv, tpit, sl.seg[1] + (size_t)tile_id * TC, &sl.cnt[1][tile_id]);
+    seg_append<CPT, kTB>(ex, take, sl.seg[0] + (size_t)tile_id * TC, &sl.cnt[0][tile_id]);
     if constexpr (W == W_SADDLE) {
       if (sums && a.tlist) {  // uniform per block; a tile without weights has no tag yet
 #pragma unroll
@@ -840,15 +849,6 @@ __global__ __launch_bounds__(kBT) void k_label_tiles(TileArgs a, int32_t* __rest
 // block of k_seg_scan per list (blockIdx.x), one block of k_seg_copy per
 // tile and list.
 // ---------------------------------------------------------------------------
-struct SegLists {
-  int32_t* seg[2];
-  int32_t* cnt[2];  // per tile
-  int32_t* off[2];  // per tile
-  int32_t* list[2];
-  int64_t* count[2];
-  int append[2];
-};
-
 __global__ __launch_bounds__(1024) void k_seg_scan(SegLists s, int ntiles) {
   const int l = blockIdx.x;
   const int per = (ntiles + 1023) / 1024;
@@ -1266,6 +1266,35 @@ static dim3 tile_grid(const GridD& g) { return dim3((g.nx + TS - 1) / TS, (g.ny 
 
 int tiles_count(const GridD& g) { return ((g.nx + TS - 1) / TS) * ((g.ny + TS - 1) / TS); }
 
+// per-tile segments for up to two lists of a tile pass (S_SEGD0/1, S_SEGCO)
+int seg_lists(pff_ctx* ctx, const GridD& g, SegLists& sl, int32_t* list0, int64_t* count0, int32_t* list1,
+                     int64_t* count1) {
+  const int nt = tiles_count(g);
+  void *d0, *d1, *co;
+  PFF_TRY(scratch(ctx, S_SEGD0, (size_t)nt * TC * 4, &d0));
+  PFF_TRY(scratch(ctx, S_SEGD1, (size_t)nt * TC * 4, &d1));
+  PFF_TRY(scratch(ctx, S_SEGCO, (size_t)nt * 4 * 4, &co));
+  sl = SegLists{};
+  sl.seg[0] = (int32_t*)d0;
+  sl.seg[1] = (int32_t*)d1;
+  sl.cnt[0] = (int32_t*)co;
+  sl.cnt[1] = sl.cnt[0] + nt;
+  sl.off[0] = sl.cnt[1] + nt;
+  sl.off[1] = sl.off[0] + nt;
+  sl.list[0] = list0;
+  sl.count[0] = count0;
+  sl.list[1] = list1;
+  sl.count[1] = count1;
+  return 0;
+}
+
+// join the segments of nlists lists into their contiguous lists
+void seg_join(pff_ctx* ctx, const GridD& g, const SegLists& sl, int nlists) {
+  const int nt = tiles_count(g);
+  hipLaunchKernelGGL(k_seg_scan, dim3(nlists), dim3(1024), 0, ctx->stream, sl, nt);
+  hipLaunchKernelGGL(k_seg_copy, dim3(nt, nlists), dim3(kBT), 0, ctx->stream, sl);
+}
+
 // Upstream sums with the tiled contraction.  W_UNIT: counts -> cnt and/or
 // Q = q0*count; W_SADDLE: tag[c] = 1 where a saddle node lies upstream
 // (tiles flagged inactive in tact are skipped: they hold no closed basin).
@@ -1302,10 +1331,13 @@ int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, 
   const dim3 tg = tile_grid(g);
   // algorithmic bytes per cell: rcv in; E, S, inflow out (+ bid, snode for saddle weights)
   a.prof_cells = prof_begin(ctx, "k_tile", wmode == W_UNIT ? 16.0 : 24.0);
-  if (wmode == W_UNIT) hipLaunchKernelGGL((k_tile<0, 1, W_UNIT>), tg, dim3(kTB), 0, s, a);
-  else hipLaunchKernelGGL((k_tile<0, 1, W_SADDLE>), tg, dim3(kTB), 0, s, a);
+  SegLists sl;
+  PFF_TRY(seg_lists(ctx, g, sl, b.exits, a.n_exits, nullptr, nullptr));
+  if (wmode == W_UNIT) hipLaunchKernelGGL((k_tile<0, 1, W_UNIT>), tg, dim3(kTB), 0, s, a, sl);
+  else hipLaunchKernelGGL((k_tile<0, 1, W_SADDLE>), tg, dim3(kTB), 0, s, a, sl);
   prof_end(ctx);
   a.prof_cells = nullptr;
+  seg_join(ctx, g, sl, 1);
   // reduced graph over the exits: P2 (roots = -1), S holds the in-tile sums
   Lists L;
   L.l[0] = b.L0;
@@ -1347,8 +1379,8 @@ int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, 
   hipLaunchKernelGGL(k_exit_inflow, dim3(list_grid()), dim3(kBT), 0, s, a, b.exits, a.n_exits, b.S, b.inflow);
   // rcv, inflow in; Q (or count) out / rcv, bid, snode, inflow in; tag out
   a.prof_cells = prof_begin(ctx, "k_tile", wmode == W_UNIT ? 12.0 : 17.0);
-  if (wmode == W_UNIT) hipLaunchKernelGGL((k_tile<0, 3, W_UNIT>), tg, dim3(kTB), 0, s, a);
-  else hipLaunchKernelGGL((k_tile<0, 3, W_SADDLE>), tg, dim3(kTB), 0, s, a);
+  if (wmode == W_UNIT) hipLaunchKernelGGL((k_tile<0, 3, W_UNIT>), tg, dim3(kTB), 0, s, a, sl);
+  else hipLaunchKernelGGL((k_tile<0, 3, W_SADDLE>), tg, dim3(kTB), 0, s, a, sl);
   prof_end(ctx);
   a.prof_cells = nullptr;
   // the next trip's local pass recomputes the local ends where this carve
@@ -1489,9 +1521,12 @@ int tiled_labels(pff_ctx* ctx, const GridD& g, const int32_t* rec, const uint32_
   PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_NPIT, 0, 8, s), "memset"));
   const dim3 tg = tile_grid(g);
   a.prof_cells = prof_begin(ctx, "k_tile", 8.0);  // rcv in, E out
-  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL((k_tile<M, 1, W_ROOTS>), tg, dim3(kTB), 0, s, a));
+  SegLists sl;
+  PFF_TRY(seg_lists(ctx, g, sl, b.exits, a.n_exits, b.pits, a.n_pits));
+  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL((k_tile<M, 1, W_ROOTS>), tg, dim3(kTB), 0, s, a, sl));
   prof_end(ctx);
   a.prof_cells = nullptr;
+  seg_join(ctx, g, sl, 2);
   Lists L;
   L.l[0] = b.L0;
   L.l[1] = b.L1;

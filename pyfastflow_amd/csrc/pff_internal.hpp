@@ -157,6 +157,22 @@ int tiled_pits(pff_ctx* ctx, const GridD& g, int32_t** pits, int64_t** npits);
 int relabel_from_outlets(pff_ctx* ctx, const GridD& g, int32_t* bid, const int64_t* outlet, int64_t n_prev,
                          const uint32_t* tact_in, uint32_t* tact_out, int64_t* ndep, int32_t* cycle);
 int tile_flags(pff_ctx* ctx, const GridD& g, uint32_t** flags_a, uint32_t** flags_b);
+// Per-tile list segments (see k_seg_scan): a tile block writes its entries
+// to seg + tile * TC and its count, instead of appending with an atomic.
+struct SegLists {
+  int32_t* seg[2];
+  int32_t* cnt[2];  // per tile
+  int32_t* off[2];  // per tile
+  int32_t* list[2];
+  int64_t* count[2];
+  int append[2];
+};
+// segments of up to two lists of a tile pass (slots S_SEGD0/1, S_SEGCO);
+// seg_join turns them into the contiguous lists and their counts
+int seg_lists(pff_ctx* ctx, const GridD& g, SegLists& sl, int32_t* list0, int64_t* count0, int32_t* list1,
+              int64_t* count1);
+void seg_join(pff_ctx* ctx, const GridD& g, const SegLists& sl, int nlists);
+
 // saddle-node bitmap (S_WBITS) and carve tag bytes (S_B1), zero between
 // carves.  begin: a trip starts setting them (cleared first if the last carve
 // did not finish); the carve that clears them resets tag_state_dirty.
