@@ -122,31 +122,42 @@ def main():
     stages = {"receivers_ms": round(t_rcv, 4), "lakeflow_ms": round(t_lake, 4), "accumulate_ms": round(t_acc, 4)}
     log(f"[bench] stages: {stages}")
 
-    # ---- roofline of the dominant kernel: the tile contraction (k_tile), timed
-    #      live with HIP events around each launch on the launch stream during
-    #      one extra step; algorithmic bytes per launch = bytes per cell of its
-    #      variant x cells it processed (active tiles)
+    # ---- roofline of the dominant kernel: every instrumented kernel is timed
+    #      live with HIP events around each of its launches on the launch
+    #      stream during one extra step, and counts the algorithmic bytes of
+    #      the units it processed (cells of active tiles, or listed nodes);
+    #      the kernel with the largest total time is the dominant one
     import ctypes
 
     ctx = _lib.ctx_for()
     _lib.call("pff_ctx_profile", ctx, 1)
     step()
-    n_l, t_ms, t_b = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
-    _lib.call("pff_ctx_profile_report", ctx, b"k_tile", ctypes.byref(n_l), ctypes.byref(t_ms), ctypes.byref(t_b))
+    kernels = {}
+    for name in ("k_flag_scatter2", "k_zprime_min_t", "k_tag_local", "k_tag_final", "k_tile"):
+        n_l, t_ms, t_b = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
+        _lib.call("pff_ctx_profile_report", ctx, name.encode(), ctypes.byref(n_l), ctypes.byref(t_ms),
+                  ctypes.byref(t_b))
+        if n_l.value > 0:
+            kernels[name] = (int(n_l.value), t_ms.value, t_b.value)
     _lib.call("pff_ctx_profile", ctx, 0)
-    launches = max(int(n_l.value), 1)
-    avg_us = 1e3 * t_ms.value / launches
-    bytes_launch = t_b.value / launches
-    achieved = t_b.value / (t_ms.value * 1e-3) / 1e9
+    dom = max(kernels, key=lambda k: kernels[k][1])
+    launches, t_tot, b_tot = kernels[dom]
+    avg_us = 1e3 * t_tot / launches
+    bytes_launch = b_tot / launches
+    achieved = b_tot / (t_tot * 1e-3) / 1e9
     traffic = None
-    tfile = os.path.join(HERE, "profiles", "k_tile_traffic.json")
+    tfile = os.path.join(HERE, "profiles", f"{dom}_traffic.json")
     if os.path.exists(tfile):
         traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "k_tile",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
                 "launches_per_step": launches, "avg_launch_us": round(avg_us, 2),
                 "bytes_per_launch": round(bytes_launch)}
     log(f"[bench] roofline {roofline}")
+    kernel_times = {k: {"launches": v[0], "ms_per_step": round(v[1], 4),
+                        "GBs": round(v[2] / (v[1] * 1e-3) / 1e9, 1) if v[1] > 0 else None}
+                    for k, v in sorted(kernels.items(), key=lambda kv: -kv[1][1])}
+    log(f"[bench] instrumented kernels: {kernel_times}")
     bytes_rcv = 12.0 * N
     rcv_roof = {"kernel": "k_receivers", "us": round(t_rcv * 1e3, 2),
                 "achieved_GBs": round(bytes_rcv / (t_rcv * 1e-3) / 1e9, 1)}
@@ -267,7 +278,7 @@ def main():
         "config": {"workload": f"depressions {n}x{n} periodic_EW dx=10 m, {pits} Gaussian pits, spectral beta=2.0 "
                                f"relief 1000 m seed 202; receivers + reroute_flow(carve) + accumulate_constant_Q(1e-5)",
                    "cells": N, "stages": stages, "lakeflow": lake, "parity": parity, "receivers_kernel": rcv_roof,
-                   "ragged_batch": batch, "graphflood": gflood},
+                   "instrumented_kernels": kernel_times, "ragged_batch": batch, "graphflood": gflood},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

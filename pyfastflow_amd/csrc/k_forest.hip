@@ -45,6 +45,11 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, FastDiv f) {
 // ---------------------------------------------------------------------------
 constexpr int kItems = 16;
 constexpr int kChunk = kItems * kBT;  // 4096
+// the scatter halves of the list rounds: 4 nodes per thread, so that the
+// lists of later rounds (often ~1e5 nodes) still spread over ~100 blocks,
+// with one append per 1024 nodes
+constexpr int kRItems = 4;
+constexpr int kRChunk = kRItems * kBT;
 
 // label of the cells that tiled labels with cycle marking finds on or
 // draining into a receiver cycle
@@ -101,11 +106,11 @@ __global__ __launch_bounds__(kBT) void k_count_scatter2(const int32_t* __restric
                                                         const int32_t* __restrict__ nxt, int32_t* __restrict__ out,
                                                         int64_t* next_count) {
   const int64_t n = *count;
-  for (int64_t base = (int64_t)blockIdx.x * kChunk; base < n; base += (int64_t)gridDim.x * kChunk) {
-    int32_t v[kItems];
-    bool take[kItems];
+  for (int64_t base = (int64_t)blockIdx.x * kRChunk; base < n; base += (int64_t)gridDim.x * kRChunk) {
+    int32_t v[kRItems];
+    bool take[kRItems];
 #pragma unroll
-    for (int j = 0; j < kItems; ++j) {
+    for (int j = 0; j < kRItems; ++j) {
       const int64_t w = base + j * kBT + threadIdx.x;
       take[j] = false;
       v[j] = 0;
@@ -118,7 +123,7 @@ __global__ __launch_bounds__(kBT) void k_count_scatter2(const int32_t* __restric
         take[j] = q >= 0;
       }
     }
-    block_append<kItems>(v, take, out, next_count);
+    block_append<kRItems>(v, take, out, next_count);
   }
 }
 
@@ -130,13 +135,16 @@ __global__ __launch_bounds__(kBT) void k_flag_scatter2(const int32_t* __restrict
                                                        uint32_t* S, int32_t* P, const int32_t* __restrict__ tgt,
                                                        const uint32_t* __restrict__ val,
                                                        const int32_t* __restrict__ nxt, int32_t* __restrict__ out,
-                                                       int64_t* next_count) {
+                                                       int64_t* next_count, unsigned long long* prof) {
   const int64_t n = *count;
-  for (int64_t base = (int64_t)blockIdx.x * kChunk; base < n; base += (int64_t)gridDim.x * kChunk) {
-    int32_t v[kItems];
-    bool take[kItems];
+  // profiling: algorithmic bytes per listed node (list, tgt, val, nxt, S[t]
+  // in; P, list out)
+  if (prof && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(prof, 28ull * (unsigned long long)n);
+  for (int64_t base = (int64_t)blockIdx.x * kRChunk; base < n; base += (int64_t)gridDim.x * kRChunk) {
+    int32_t v[kRItems];
+    bool take[kRItems];
 #pragma unroll
-    for (int j = 0; j < kItems; ++j) {
+    for (int j = 0; j < kRItems; ++j) {
       const int64_t w = base + j * kBT + threadIdx.x;
       take[j] = false;
       v[j] = 0;
@@ -152,7 +160,7 @@ __global__ __launch_bounds__(kBT) void k_flag_scatter2(const int32_t* __restrict
         take[j] = q >= 0;
       }
     }
-    block_append<kItems>(v, take, out, next_count);
+    block_append<kRItems>(v, take, out, next_count);
   }
 }
 
@@ -210,11 +218,11 @@ __global__ __launch_bounds__(kBT) void k_label_scatter2(const int32_t* __restric
                                                         int64_t* next_count, int32_t* __restrict__ level_next,
                                                         int32_t* __restrict__ fin, int32_t round, int64_t* self_loop) {
   const int64_t n = *count;
-  for (int64_t base = (int64_t)blockIdx.x * kChunk; base < n; base += (int64_t)gridDim.x * kChunk) {
-    int32_t v[kItems];
-    bool take[kItems];
+  for (int64_t base = (int64_t)blockIdx.x * kRChunk; base < n; base += (int64_t)gridDim.x * kRChunk) {
+    int32_t v[kRItems];
+    bool take[kRItems];
 #pragma unroll
-    for (int j = 0; j < kItems; ++j) {
+    for (int j = 0; j < kRItems; ++j) {
       const int64_t w = base + j * kBT + threadIdx.x;
       take[j] = false;
       v[j] = 0;
@@ -234,7 +242,7 @@ __global__ __launch_bounds__(kBT) void k_label_scatter2(const int32_t* __restric
         take[j] = !done;
       }
     }
-    block_append<kItems>(v, take, out, next_count);
+    block_append<kRItems>(v, take, out, next_count);
   }
 }
 
@@ -253,9 +261,12 @@ static int count_rounds(pff_ctx* ctx, Lists& L, uint32_t* S, int32_t* P, int32_t
     const int a = L.cur, b = 1 - L.cur;
     hipLaunchKernelGGL(k_count_gather2, dim3(list_grid()), dim3(kBT), 0, ctx->stream, L.l[a], L.c[a], S, P, tgt, val,
                        nxt, L.c[b], first_empty, k);
-    if (flags)
+    if (flags) {
+      unsigned long long* prof = prof_begin(ctx, "k_flag_scatter2", 1.0);
       hipLaunchKernelGGL(k_flag_scatter2, dim3(list_grid()), dim3(kBT), 0, ctx->stream, L.l[a], L.c[a], S, P, tgt,
-                         val, nxt, L.l[b], L.c[b]);
+                         val, nxt, L.l[b], L.c[b], prof);
+      prof_end(ctx);
+    }
     else
       hipLaunchKernelGGL(k_count_scatter2, dim3(list_grid()), dim3(kBT), 0, ctx->stream, L.l[a], L.c[a], S, P, tgt,
                          val, nxt, L.l[b], L.c[b]);
@@ -1049,7 +1060,6 @@ __global__ __launch_bounds__(kWB) void k_tag_local(TileArgs a, uint32_t* __restr
   const GridD& g = a.g;
   const int x0 = blockIdx.x * TS, y0 = blockIdx.y * TS;
   const int tw = min(TS, g.nx - x0), th = min(TS, g.ny - y0);
-  if (a.prof_cells && threadIdx.x == 0) atomicAdd(a.prof_cells, (unsigned long long)(tw * th));
   int rv[CPW];
   tile_load_rcv(a, x0, y0, tw, th, rv);
   const uint32_t w = threadIdx.x < TW ? wbits[(size_t)tile * TW + threadIdx.x] : 0u;
@@ -1129,6 +1139,10 @@ __global__ __launch_bounds__(kWB) void k_tag_local(TileArgs a, uint32_t* __restr
     if ((tmask >> j) & 1u) a.tag[cell_of(j)] = 1;
   seg_append_mask<CPW, kWB>(tmask, cell_of, sl.seg[0] + (size_t)tile * TC, &sl.cnt[0][tile]);
   seg_append<1, kWB>(xs, xt, sl.seg[1] + (size_t)tile * TC, &sl.cnt[1][tile]);
+  // profiling: algorithmic bytes of the tile (rcv in; ring slots, tags and
+  // list entries out)
+  if (a.prof_cells && threadIdx.x == 0)
+    atomicAdd(a.prof_cells, (unsigned long long)(4 * tw * th + 12 * nr + 5 * sl.cnt[0][tile] + 4 * sl.cnt[1][tile]));
 }
 
 // final pass on the tiles that receive a tag: walk from the ring cells with
@@ -1454,7 +1468,7 @@ int saddle_tags_walk(pff_ctx* ctx, const GridD& g, const int32_t* rcv, const int
     sl.append[1] = 0;
   }
   // rcv in; tags and the ring's E and inflow out (the ring is ~6% of a tile)
-  a.prof_cells = prof_begin(ctx, "k_tag_local", 4.0);
+  a.prof_cells = prof_begin(ctx, "k_tag_local", 1.0);  // the kernel adds its bytes
   hipLaunchKernelGGL(k_tag_local, tg, dim3(kWB), 0, s, a, wbits, sl);
   prof_end(ctx);
   a.prof_cells = nullptr;

@@ -581,7 +581,8 @@ template <int MODE>
 __global__ __launch_bounds__(kBT) void k_zprime_min_t(GridD g, int tiles_x, const uint32_t* __restrict__ tact,
                                                       const int32_t* __restrict__ bid, const float* __restrict__ z,
                                                       uint8_t* __restrict__ isb, float* __restrict__ zp,
-                                                      int64_t* saddle, int32_t* blist, int64_t* nblist) {
+                                                      int64_t* saddle, int32_t* blist, int64_t* nblist,
+                                                      unsigned long long* prof) {
   // one block per 64x64 tile, one wave per row: one table flush and one list
   // append per tile
   __shared__ SaddleTable t;
@@ -618,6 +619,17 @@ __global__ __launch_bounds__(kBT) void k_zprime_min_t(GridD g, int tiles_x, cons
     saddle_offer(t, keyb, res, saddle);
   }
   if (blist) block_append_ordered<R>(lv, lt, blist, nblist);
+  if (prof) {
+    // profiling: algorithmic bytes of the tile (z, bid in; z', border flag
+    // out; 4 per listed border cell)
+    int mine = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) mine += lt[r] ? 1 : 0;
+    int total;
+    (void)block_exclusive_scan<kBT>(mine, &total);
+    const int tw = min(TSL, g.nx - bx_ * TSL), th = min(TSL, g.ny - by_ * TSL);
+    if (threadIdx.x == 0 && threadIdx.y == 0) atomicAdd(prof, (unsigned long long)(13 * tw * th + 4 * total));
+  }
   saddle_table_flush(t, saddle);
 }
 
@@ -859,8 +871,10 @@ int saddlesort_active(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t*
   const dim3 lgz(2048);
   if (full) {
     const dim3 tt((g.nx + TSL - 1) / TSL, (g.ny + TSL - 1) / TSL);
+    unsigned long long* prof = prof_begin(ctx, "k_zprime_min_t", 1.0);
     PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_zprime_min_t<M>, tt, cb, 0, s, g, tiles_x, tact, bid, z, isb, zp,
-                                                 saddle, BL[out], NB[out]));
+                                                 saddle, BL[out], NB[out], prof));
+    prof_end(ctx);
   } else {
     if (ctx->host_words[W_NBL0 + in] > (int64_t)1 << 21)
       PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL((k_zprime_list<M, 16>), lgz, bl, 0, s, g, (const int32_t*)BL[in],
