@@ -64,11 +64,16 @@ int pff_ctx_destroy(pff_ctx* ctx);
 int pff_ctx_set_stream(pff_ctx* ctx, void* hip_stream);
 int pff_ctx_synchronize(pff_ctx* ctx);
 /* Opt-in per-kernel timing, no reference counterpart: while enabled, the
-   instrumented launches (the tile contraction, "k_tile") are bracketed by
-   HIP events on the context stream and count the cells they process.
-   Enabling clears the records; report() syncs and sums the launches of one
-   kernel: count, time (ms) and algorithmic bytes. */
+   instrumented launches ("k_tile", "k_tag_local", "k_tag_final",
+   "k_flag_scatter2", "k_zprime_min_t") record HIP events at the kernel's
+   start and end (hipExtLaunchKernelGGL) and count the algorithmic bytes of
+   what they process.  Enabling clears the records; report() syncs and sums
+   the launches of one kernel: count, time (ms) and algorithmic bytes.
+   launch_overhead() times reps launches of an empty kernel recorded the same
+   way and returns the median (us): what such an event pair adds to a
+   launch beyond the kernel's own run. */
 int pff_ctx_profile(pff_ctx* ctx, int32_t enable);
+int pff_ctx_launch_overhead(pff_ctx* ctx, int32_t reps, double* median_us);
 int pff_ctx_profile_report(pff_ctx* ctx, const char* kernel, int64_t* launches, double* total_ms,
                            double* total_bytes);
 

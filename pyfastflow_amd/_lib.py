@@ -86,6 +86,7 @@ SIGNATURES = {
     "pff_ctx_set_stream": [c_p, c_p],
     "pff_ctx_synchronize": [c_p],
     "pff_ctx_profile": [c_p, c_int],
+    "pff_ctx_launch_overhead": [c_p, ctypes.c_int32, ctypes.POINTER(c_d)],
     "pff_ctx_profile_report": [c_p, ctypes.c_char_p, ctypes.POINTER(c_i64), ctypes.POINTER(c_d),
                                ctypes.POINTER(c_d)],
     "pff_compute_receivers": [c_p, G, c_p, c_p, c_p],

@@ -82,6 +82,7 @@ int read_words(pff_ctx* ctx) {
 }
 
 unsigned long long* prof_begin(pff_ctx* ctx, const char* name, double bytes_per_cell) {
+  ctx->prof_pending = false;
   if (!ctx->prof_on || (int)ctx->prof.size() >= pff_ctx::kProfSlots) return nullptr;
   pff_ctx::ProfRec r;
   r.name = name;
@@ -89,14 +90,18 @@ unsigned long long* prof_begin(pff_ctx* ctx, const char* name, double bytes_per_
   r.slot = (int)ctx->prof.size();
   (void)hipEventCreate(&r.start);
   (void)hipEventCreate(&r.stop);
-  (void)hipEventRecord(r.start, ctx->stream);
   ctx->prof.push_back(r);
+  ctx->prof_pending = true;
   return ctx->prof_cells + r.slot;
 }
 
-void prof_end(pff_ctx* ctx) {
-  if (!ctx->prof_on || ctx->prof.empty()) return;
-  (void)hipEventRecord(ctx->prof.back().stop, ctx->stream);
+void prof_events(pff_ctx* ctx, hipEvent_t* start, hipEvent_t* stop) {
+  *start = nullptr;
+  *stop = nullptr;
+  if (!ctx->prof_pending) return;
+  ctx->prof_pending = false;
+  *start = ctx->prof.back().start;
+  *stop = ctx->prof.back().stop;
 }
 
 static void prof_clear(pff_ctx* ctx) {
@@ -261,9 +266,8 @@ int reroute_core(pff_ctx* ctx, const GridD& d, int32_t* bid, int32_t* rec, int32
       if (st) st->n_iter++;
       // carve (:525-526): rec equals rec_ here; rec is edited, then copied to rec_
       int32_t cyc = 0;
-      // the first trip's labels came from the tiled pass over rec_ (local ends stored)
-      PFF_TRY(carve_tiled(ctx, d, rec, rec_, bid, basin_saddlenode, outlet, rec_, tact, pits, npits, it == 0,
-                          label_rounds, &cyc));
+      PFF_TRY(carve_tiled(ctx, d, rec, rec_, bid, basin_saddlenode, outlet, rec_, tact, pits, npits, label_rounds,
+                          &cyc));
       if (!cyc) continue;
       // not reachable (the labels proved rec_ a forest); fall back regardless
       if (bs) {
@@ -351,6 +355,29 @@ int pff_ctx_profile(pff_ctx* ctx, int32_t enable) {
   }
   if (enable) PFF_TRY(check_hip(hipMemsetAsync(ctx->prof_cells, 0, pff_ctx::kProfSlots * 8, ctx->stream), "memset"));
   ctx->prof_on = enable != 0;
+  return 0;
+}
+
+__global__ void k_empty() {}
+
+int pff_ctx_launch_overhead(pff_ctx* ctx, int32_t reps, double* median_us) {
+  PFF_TRY(begin(ctx));
+  if (!median_us || reps < 1 || reps > 1000) return fail(PFF_EINVAL, "launch_overhead: bad argument");
+  std::vector<hipEvent_t> ev(2 * (size_t)reps, nullptr);
+  for (auto& e : ev) PFF_TRY(check_hip(hipEventCreate(&e), "hipEventCreate"));
+  for (int r = 0; r < reps; ++r)
+    hipExtLaunchKernelGGL(k_empty, dim3(1), dim3(64), 0, ctx->stream, ev[2 * r], ev[2 * r + 1], 0);
+  int rc = check_hip(hipStreamSynchronize(ctx->stream), "sync");
+  std::vector<double> t(reps, 0.0);
+  for (int r = 0; r < reps && rc == 0; ++r) {
+    float ms = 0.0f;
+    rc = check_hip(hipEventElapsedTime(&ms, ev[2 * r], ev[2 * r + 1]), "hipEventElapsedTime");
+    t[r] = 1e3 * ms;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  PFF_TRY(rc);
+  std::sort(t.begin(), t.end());
+  *median_us = t[reps / 2];
   return 0;
 }
 

@@ -263,9 +263,8 @@ static int count_rounds(pff_ctx* ctx, Lists& L, uint32_t* S, int32_t* P, int32_t
                        nxt, L.c[b], first_empty, k);
     if (flags) {
       unsigned long long* prof = prof_begin(ctx, "k_flag_scatter2", 1.0);
-      hipLaunchKernelGGL(k_flag_scatter2, dim3(list_grid()), dim3(kBT), 0, ctx->stream, L.l[a], L.c[a], S, P, tgt,
-                         val, nxt, L.l[b], L.c[b], prof);
-      prof_end(ctx);
+      PFF_PROF_LAUNCH(ctx, k_flag_scatter2, dim3(list_grid()), dim3(kBT), 0, ctx->stream, L.l[a], L.c[a], S, P, tgt,
+                      val, nxt, L.l[b], L.c[b], prof);
     }
     else
       hipLaunchKernelGGL(k_count_scatter2, dim3(list_grid()), dim3(kBT), 0, ctx->stream, L.l[a], L.c[a], S, P, tgt,
@@ -492,13 +491,6 @@ struct TileArgs {
   int ring_term;  // terminal slot (ntiles * 256): its pointer is -1
   int32_t* tlist;
   int64_t* n_tlist;
-  // saddle tags across lakeflow trips (nullable): tiles whose receivers the
-  // previous trip's carve changed; the local pass clears its tile's flag.
-  // A tile that is not flagged (or every tile, with ends_valid: the labels
-  // pass of the same trip wrote E) keeps the local ends in E, so a tile
-  // without a weight skips its rounds and reads them from there.
-  uint32_t* tdirty;
-  int ends_valid;
   // labels on receivers that may hold cycles: a cell on or draining into a
   // cycle gets E = -1 (in-tile cycle) or the label kCycleLabel (through the
   // reduced graph) instead of failing the pass
@@ -549,16 +541,6 @@ __global__ __launch_bounds__(kTB) void k_tile(TileArgs a, SegLists sl) {
   uint32_t rootmask = 0;
   int wany = 0;
   int farlink = 0;  // a receiver that is not a D4 neighbour (labels pass only)
-  __shared__ uint32_t dirty_s;
-  if constexpr (PH == 1 && W == W_SADDLE) {
-    if (threadIdx.x == 0) {
-      dirty_s = 1u;
-      if (a.tdirty) {
-        dirty_s = a.ends_valid ? 0u : a.tdirty[blockIdx.y * a.tiles_x + blockIdx.x];
-        a.tdirty[blockIdx.y * a.tiles_x + blockIdx.x] = 0u;
-      }
-    }
-  }
 #pragma unroll
   for (int j = 0; j < CPT; ++j) {
     const int li = threadIdx.x + j * kTB;
@@ -612,15 +594,9 @@ __global__ __launch_bounds__(kTB) void k_tile(TileArgs a, SegLists sl) {
   // no weight there only needs its local ends, jumped on E alone until they
   // stop changing (uniform per block)
   bool eonly = false;
-  // a weightless tile whose receivers did not change since its local ends
-  // were stored takes them from E and runs no rounds (uniform per block)
-  bool stored = false;
-  if constexpr (PH == 1 && W == W_SADDLE) {
-    eonly = !sums;
-    stored = eonly && dirty_s == 0u;
-  }
-  bool cyc = !stored;
-  for (int k = 0; k < (stored ? 0 : kTileRounds); ++k) {
+  if constexpr (PH == 1 && W == W_SADDLE) eonly = !sums;
+  bool cyc = true;
+  for (int k = 0; k < kTileRounds; ++k) {
     uint16_t pm[CPT], pp[CPT], ee[CPT];
     uint32_t sv[CPT];
     int any = 0;
@@ -677,9 +653,7 @@ __global__ __launch_bounds__(kTB) void k_tile(TileArgs a, SegLists sl) {
       const int c = cell[j];
       if (c < 0) continue;
       bool end;
-      if (stored) {
-        end = a.E[c] == c;
-      } else {
+      {
         if constexpr (W == W_ROOTS) {
           // the in-tile pointer of a cell whose path ends in the tile has
           // reached NONE after kTileRounds rounds; one that has not is on or
@@ -793,19 +767,6 @@ __global__ __launch_bounds__(kBT) void k_exit_link(TileArgs a, const int32_t* __
     // an exit keeps its initial label only if its chain never reaches a root
     if (labels) a.bid2[x] = a.mark_cycles ? kCycleLabel : 0;
     list_out[w] = x;
-  }
-}
-
-// tiles whose receivers this trip's carve changes: those of the tagged cells
-__global__ __launch_bounds__(kBT) void k_mark_dirty(int32_t N, int32_t nx, int tiles_x,
-                                                    const int32_t* __restrict__ list, const int64_t* nlist,
-                                                    uint32_t* tdirty) {
-  const int64_t n = *nlist;
-  for (int64_t w = (int64_t)blockIdx.x * kBT + threadIdx.x; w < n; w += (int64_t)gridDim.x * kBT) {
-    const int c = list[w];
-    if (c < 0 || c >= N) continue;
-    const int row = c / nx;
-    tdirty[(row / TS) * tiles_x + (c - row * nx) / TS] = 1u;
   }
 }
 
@@ -1232,7 +1193,7 @@ __global__ __launch_bounds__(kBT) void k_ring_inflow(TileArgs a, const int32_t* 
 
 struct TileBufs {
   int32_t *E, *P2, *bid2, *exits, *L0, *L1, *t0, *t1, *t2, *pits;
-  uint32_t *S, *inflow, *tact, *tin, *tdirty;
+  uint32_t *S, *inflow, *tact, *tin;
 };
 
 static int tile_bufs(pff_ctx* ctx, const GridD& g, TileBufs& b) {
@@ -1253,7 +1214,6 @@ static int tile_bufs(pff_ctx* ctx, const GridD& g, TileBufs& b) {
   const size_t nt = (size_t)((g.nx + TS - 1) / TS) * ((g.ny + TS - 1) / TS);
   PFF_TRY(scratch(ctx, S_TACT, nt * 4 * 4, &p)); b.tact = (uint32_t*)p;
   b.tin = b.tact + 2 * nt;
-  b.tdirty = b.tact + 3 * nt;
   return 0;
 }
 
@@ -1319,18 +1279,13 @@ void seg_join(pff_ctx* ctx, const GridD& g, const SegLists& sl, int nlists) {
 // graph that is not a forest (outputs are then not meaningful).
 int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, const int32_t* bid,
                    const int32_t* snode, float q0, float* Q, uint32_t* cnt, uint8_t* tag, const uint32_t* tact,
-                   int32_t* cycle, int rounds, int32_t* tag_list, int ends_valid) {
+                   int32_t* cycle, int rounds, int32_t* tag_list) {
   TileBufs b;
   PFF_TRY(tile_bufs(ctx, g, b));
   TileArgs a = tile_args(g, b, ctx, rcv);
   if (tag_list) {
     a.tlist = tag_list;
     a.n_tlist = ctx->dev_words + W_NTAG;
-  }
-  const bool stored_ends = wmode == W_SADDLE && tag_list && ends_valid >= 0;
-  if (stored_ends) {
-    a.tdirty = b.tdirty;
-    a.ends_valid = ends_valid;
   }
   a.bid = bid;
   a.snode = snode;
@@ -1347,9 +1302,8 @@ int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, 
   a.prof_cells = prof_begin(ctx, "k_tile", wmode == W_UNIT ? 16.0 : 24.0);
   SegLists sl;
   PFF_TRY(seg_lists(ctx, g, sl, b.exits, a.n_exits, nullptr, nullptr));
-  if (wmode == W_UNIT) hipLaunchKernelGGL((k_tile<0, 1, W_UNIT>), tg, dim3(kTB), 0, s, a, sl);
-  else hipLaunchKernelGGL((k_tile<0, 1, W_SADDLE>), tg, dim3(kTB), 0, s, a, sl);
-  prof_end(ctx);
+  if (wmode == W_UNIT) PFF_PROF_LAUNCH(ctx, (k_tile<0, 1, W_UNIT>), tg, dim3(kTB), 0, s, a, sl);
+  else PFF_PROF_LAUNCH(ctx, (k_tile<0, 1, W_SADDLE>), tg, dim3(kTB), 0, s, a, sl);
   a.prof_cells = nullptr;
   seg_join(ctx, g, sl, 1);
   // reduced graph over the exits: P2 (roots = -1), S holds the in-tile sums
@@ -1393,15 +1347,9 @@ int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, 
   hipLaunchKernelGGL(k_exit_inflow, dim3(list_grid()), dim3(kBT), 0, s, a, b.exits, a.n_exits, b.S, b.inflow);
   // rcv, inflow in; Q (or count) out / rcv, bid, snode, inflow in; tag out
   a.prof_cells = prof_begin(ctx, "k_tile", wmode == W_UNIT ? 12.0 : 17.0);
-  if (wmode == W_UNIT) hipLaunchKernelGGL((k_tile<0, 3, W_UNIT>), tg, dim3(kTB), 0, s, a, sl);
-  else hipLaunchKernelGGL((k_tile<0, 3, W_SADDLE>), tg, dim3(kTB), 0, s, a, sl);
-  prof_end(ctx);
+  if (wmode == W_UNIT) PFF_PROF_LAUNCH(ctx, (k_tile<0, 3, W_UNIT>), tg, dim3(kTB), 0, s, a, sl);
+  else PFF_PROF_LAUNCH(ctx, (k_tile<0, 3, W_SADDLE>), tg, dim3(kTB), 0, s, a, sl);
   a.prof_cells = nullptr;
-  // the next trip's local pass recomputes the local ends where this carve
-  // changes receivers
-  if (stored_ends)
-    hipLaunchKernelGGL(k_mark_dirty, dim3(list_grid()), dim3(kBT), 0, s, g.N, g.nx, a.tiles_x, (const int32_t*)tag_list,
-                       (const int64_t*)a.n_tlist, b.tdirty);
   return check_hip(hipGetLastError(), "tiled_upstream");
 }
 
@@ -1469,8 +1417,7 @@ int saddle_tags_walk(pff_ctx* ctx, const GridD& g, const int32_t* rcv, const int
   }
   // rcv in; tags and the ring's E and inflow out (the ring is ~6% of a tile)
   a.prof_cells = prof_begin(ctx, "k_tag_local", 1.0);  // the kernel adds its bytes
-  hipLaunchKernelGGL(k_tag_local, tg, dim3(kWB), 0, s, a, wbits, sl);
-  prof_end(ctx);
+  PFF_PROF_LAUNCH(ctx, k_tag_local, tg, dim3(kWB), 0, s, a, wbits, sl);
   a.prof_cells = nullptr;
   hipLaunchKernelGGL(k_seg_scan, dim3(2), dim3(1024), 0, s, sl, nt);
   hipLaunchKernelGGL(k_seg_copy, dim3(nt, 2), dim3(kBT), 0, s, sl);
@@ -1507,8 +1454,7 @@ int saddle_tags_walk(pff_ctx* ctx, const GridD& g, const int32_t* rcv, const int
                      (const int64_t*)a.n_exits, b.tin);
   // rcv and the tile's tags in; new tags out
   a.prof_cells = prof_begin(ctx, "k_tag_final", 5.0);
-  hipLaunchKernelGGL(k_tag_final, tg, dim3(kWB), 0, s, a, sl);
-  prof_end(ctx);
+  PFF_PROF_LAUNCH(ctx, k_tag_final, tg, dim3(kWB), 0, s, a, sl);
   a.prof_cells = nullptr;
   hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(1024), 0, s, sl, nt);
   hipLaunchKernelGGL(k_seg_copy, dim3(nt, 1), dim3(kBT), 0, s, sl);
@@ -1537,8 +1483,7 @@ int tiled_labels(pff_ctx* ctx, const GridD& g, const int32_t* rec, const uint32_
   a.prof_cells = prof_begin(ctx, "k_tile", 8.0);  // rcv in, E out
   SegLists sl;
   PFF_TRY(seg_lists(ctx, g, sl, b.exits, a.n_exits, b.pits, a.n_pits));
-  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL((k_tile<M, 1, W_ROOTS>), tg, dim3(kTB), 0, s, a, sl));
-  prof_end(ctx);
+  PFF_MODE_DISPATCH(g.mode, PFF_PROF_LAUNCH(ctx, (k_tile<M, 1, W_ROOTS>), tg, dim3(kTB), 0, s, a, sl));
   a.prof_cells = nullptr;
   seg_join(ctx, g, sl, 2);
   Lists L;

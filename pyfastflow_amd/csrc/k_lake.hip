@@ -872,9 +872,8 @@ int saddlesort_active(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t*
   if (full) {
     const dim3 tt((g.nx + TSL - 1) / TSL, (g.ny + TSL - 1) / TSL);
     unsigned long long* prof = prof_begin(ctx, "k_zprime_min_t", 1.0);
-    PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_zprime_min_t<M>, tt, cb, 0, s, g, tiles_x, tact, bid, z, isb, zp,
-                                                 saddle, BL[out], NB[out], prof));
-    prof_end(ctx);
+    PFF_MODE_DISPATCH(g.mode, PFF_PROF_LAUNCH(ctx, k_zprime_min_t<M>, tt, cb, 0, s, g, tiles_x, tact, bid, z, isb, zp,
+                                              saddle, BL[out], NB[out], prof));
   } else {
     if (ctx->host_words[W_NBL0 + in] > (int64_t)1 << 21)
       PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL((k_zprime_list<M, 16>), lgz, bl, 0, s, g, (const int32_t*)BL[in],
@@ -937,15 +936,16 @@ __global__ __launch_bounds__(kBT) void k_carve_sync_list(const int32_t* __restri
   }
 }
 
-// Carve for one trip with the tiled contraction: the cells the reference
+// Carve for one trip on a forest of D4 receivers: the cells the reference
 // tags (every cell on the path from a saddle node to its pit) are exactly the
-// cells with a saddle node upstream, an upstream sum of saddle indicators.
-// rec equals rec_in on entry; on return rec holds the carved receivers and
-// rec_work (if given) the same.  *cycle (host) = 1 if rec_in is not a forest
-// (nothing is written then).
+// cells with a saddle node upstream, found by the tag walks (saddle_tags_walk)
+// from the saddle-node bits that this trip's saddlesort_active set.  rec
+// equals rec_in on entry; on return rec holds the carved receivers and
+// rec_work (if given) the same; the tag bytes are cleared again.  *cycle is
+// always 0 (rec_in was proved a forest by this trip's labels).
 int carve_tiled(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t* rec_in, const int32_t* bid,
                 const int32_t* snode, const int64_t* outlet, int32_t* rec_work, const uint32_t* tact,
-                const int32_t* pits, const int64_t* npits, bool first, int rounds, int32_t* cycle) {
+                const int32_t* pits, const int64_t* npits, int rounds, int32_t* cycle) {
   void* tl;
   uint32_t* wbits;
   uint8_t* tag;
@@ -955,7 +955,6 @@ int carve_tiled(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t* rec_i
   // The tag walks also list the tagged cells (the counter was zeroed by
   // this trip's saddlesort_active, which also set the saddle-node bits).
   const int64_t* ntag = ctx->dev_words + W_NTAG;
-  (void)first;
   PFF_TRY(saddle_tags_walk(ctx, g, rec_in, bid, wbits, tag, tact, rounds, (int32_t*)tl));
   *cycle = 0;
   const dim3 lg(2048), bl(kBT);
@@ -992,7 +991,7 @@ int carve_split(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t* rec_i
   PFF_TRY(check_hip(hipMemsetAsync(ntag, 0, 8, s), "memset"));
   // paths that reach a root (a forest once the marked cells are roots)
   PFF_TRY(tiled_upstream(ctx, g, rcvp, 2, bid, snode, 0.0f, nullptr, nullptr, (uint8_t*)tag, nullptr, nullptr, 0,
-                         (int32_t*)tl, -1));
+                         (int32_t*)tl));
   const dim3 lg(2048), bl(kBT);
   hipLaunchKernelGGL(k_carve_reverse_list, lg, bl, 0, s, g.N, (const int32_t*)tl, (const int64_t*)ntag,
                      (const uint8_t*)tag, rcvp, rec);

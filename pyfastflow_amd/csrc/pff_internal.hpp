@@ -1,6 +1,7 @@
 // pff_internal.hpp -- host-side internals of libpff_hip.so: the context, error
 // reporting and the launchers each kernel file exports to the C ABI layer.
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -52,6 +53,7 @@ struct pff_ctx {
   } lake_log;
   hipEvent_t sync_ev = nullptr;  // spin-wait event of wait_stream
   bool prof_on = false;
+  bool prof_pending = false;  // prof_begin made a record that the next launch takes
   std::vector<ProfRec> prof;
   unsigned long long* prof_cells = nullptr;  // device, kProfSlots counters
   static constexpr int kProfSlots = 4096;
@@ -64,9 +66,16 @@ int fail(int code, const std::string& msg);
 int check_hip(hipError_t e, const char* what);
 int check_grid(const pff_grid* g, GridD* out);
 // profiling: returns the device counter for the launch's processed cells
-// (null when profiling is off); call prof_end after the launch
+// (null when profiling is off); launch the kernel with PFF_PROF_LAUNCH,
+// whose events mark the kernel's own start and end
 unsigned long long* prof_begin(pff_ctx* ctx, const char* name, double bytes_per_cell);
-void prof_end(pff_ctx* ctx);
+void prof_events(pff_ctx* ctx, hipEvent_t* start, hipEvent_t* stop);
+#define PFF_PROF_LAUNCH(ctx, kernel, grid, block, shmem, stream, ...)                      \
+  do {                                                                                    \
+    hipEvent_t e0_, e1_;                                                                  \
+    prof_events(ctx, &e0_, &e1_);                                                         \
+    hipExtLaunchKernelGGL(kernel, grid, block, shmem, stream, e0_, e1_, 0, __VA_ARGS__); \
+  } while (0)
 // wait for the context stream by polling an event (a short spin instead of
 // the runtime's blocking wait; every lakeflow trip waits once or twice)
 int wait_stream(pff_ctx* ctx);
@@ -138,7 +147,7 @@ int launch_saddlesort(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t*
 // tiled contraction (forests; *cycle = 1 when the receivers are not a forest)
 int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, const int32_t* bid,
                    const int32_t* snode, float q0, float* Q, uint32_t* cnt, uint8_t* tag, const uint32_t* tact,
-                   int32_t* cycle, int rounds = 0, int32_t* tag_list = nullptr, int ends_valid = -1);
+                   int32_t* cycle, int rounds = 0, int32_t* tag_list = nullptr);
 // mark_cycles: the receivers may hold cycles; the cells on or draining into
 // one get the label -1 in tiled_labels_write (no host read; *ndep, *cycle and
 // *rounds_used are not written)
@@ -188,7 +197,7 @@ int saddle_tags_walk(pff_ctx* ctx, const GridD& g, const int32_t* rcv, const int
 int tiles_count(const GridD& g);
 int carve_tiled(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t* rec_in, const int32_t* bid,
                 const int32_t* snode, const int64_t* outlet, int32_t* rec_work, const uint32_t* tact,
-                const int32_t* pits, const int64_t* npits, bool first, int rounds, int32_t* cycle);
+                const int32_t* pits, const int64_t* npits, int rounds, int32_t* cycle);
 int saddlesort_active(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t* isb, float* zp, int64_t* saddle,
                       int32_t* snode, int64_t* outlet, const float* z, const uint32_t* tact, const int32_t* pits,
                       const int64_t* npits, int* border_cur);
