@@ -76,9 +76,23 @@ int wait_stream(pff_ctx* ctx) {
 }
 
 int read_words(pff_ctx* ctx) {
+  PFF_TRY(post_words(ctx));
+  return wait_words(ctx);
+}
+
+int post_words(pff_ctx* ctx) {
   PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words, ctx->dev_words, W_NWORDS * sizeof(int64_t),
                                    hipMemcpyDeviceToHost, ctx->stream), "d2h"));
-  return wait_stream(ctx);
+  if (!ctx->words_ev) PFF_TRY(check_hip(hipEventCreateWithFlags(&ctx->words_ev, hipEventDisableTiming), "event"));
+  return check_hip(hipEventRecord(ctx->words_ev, ctx->stream), "event record");
+}
+
+int wait_words(pff_ctx* ctx) {
+  for (;;) {
+    const hipError_t e = hipEventQuery(ctx->words_ev);
+    if (e == hipSuccess) return 0;
+    if (e != hipErrorNotReady) return check_hip(e, "event query");
+  }
 }
 
 unsigned long long* prof_begin(pff_ctx* ctx, const char* name, double bytes_per_cell) {
@@ -334,6 +348,7 @@ int pff_ctx_destroy(pff_ctx* ctx) {
   if (ctx->dev_words) (void)hipFree(ctx->dev_words);
   if (ctx->lake_log.dev_count) (void)hipFree(ctx->lake_log.dev_count);
   if (ctx->sync_ev) (void)hipEventDestroy(ctx->sync_ev);
+  if (ctx->words_ev) (void)hipEventDestroy(ctx->words_ev);
   if (ctx->host_words) (void)hipHostFree(ctx->host_words);
   delete ctx;
   return 0;

@@ -1429,36 +1429,46 @@ int saddle_tags_walk(pff_ctx* ctx, const GridD& g, const int32_t* rcv, const int
   int64_t* fe = ctx->dev_words + W_EMPTY2;
   hipLaunchKernelGGL(k_ring_link, dim3(list_grid()), dim3(kBT), 0, s, a, (const int32_t*)b.exits,
                      (const int64_t*)a.n_exits, ringP2, b.L0, L.c[0], fe, b.tin, nt);
+  a.tin = b.tin;
+  // flagged exits mark the ring cells they drain into, the final pass tags
+  // below those, the new tags join the list.  All three only add tags: run
+  // again after more rounds, they add the tags the earlier flags missed.
+  auto finish = [&]() -> int {
+    hipLaunchKernelGGL(k_ring_inflow, dim3(list_grid()), dim3(kBT), 0, s, a, (const int32_t*)b.exits,
+                       (const int64_t*)a.n_exits, b.tin);
+    // rcv and the tile's tags in; new tags out
+    a.prof_cells = prof_begin(ctx, "k_tag_final", 5.0);
+    PFF_PROF_LAUNCH(ctx, k_tag_final, tg, dim3(kWB), 0, s, a, sl);
+    a.prof_cells = nullptr;
+    hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(1024), 0, s, sl, nt);
+    hipLaunchKernelGGL(k_seg_copy, dim3(nt, 1), dim3(kBT), 0, s, sl);
+    return check_hip(hipGetLastError(), "saddle_tags_walk");
+  };
   if (rounds > 0) {
     PFF_TRY(count_rounds(ctx, L, a.ringS, ringP2, b.t0, (uint32_t*)b.t1, b.t2, rounds, true));
-  } else {
-    const int cap = ceil_log2i(g.N) + 1;
-    int done_rounds = 0;
-    int64_t live = 1;
-    int step = std::max(2, std::min(ctx->last_tag_rounds + 1, cap));
-    while (done_rounds < cap) {
-      const int r = std::min(step, cap - done_rounds);
-      PFF_TRY(count_rounds(ctx, L, a.ringS, ringP2, b.t0, (uint32_t*)b.t1, b.t2, r, true, done_rounds, fe));
-      done_rounds += r;
-      step = 3;
-      PFF_TRY(read_words(ctx));
-      live = ctx->host_words[L.cur == 0 ? W_COUNT0 : W_COUNT1];
-      if (live == 0 || ctx->host_words[W_CYCLE] != 0) break;
-    }
-    const int64_t f = ctx->host_words[W_EMPTY2];
-    ctx->last_tag_rounds = (live == 0 && f < done_rounds) ? std::max<int>((int)f, 1) : done_rounds;
-    if (live != 0 || ctx->host_words[W_CYCLE] != 0) return fail(PFF_ENOTCONV, "saddle tags: receivers are not a forest");
+    return finish();
   }
-  a.tin = b.tin;
-  hipLaunchKernelGGL(k_ring_inflow, dim3(list_grid()), dim3(kBT), 0, s, a, (const int32_t*)b.exits,
-                     (const int64_t*)a.n_exits, b.tin);
-  // rcv and the tile's tags in; new tags out
-  a.prof_cells = prof_begin(ctx, "k_tag_final", 5.0);
-  PFF_PROF_LAUNCH(ctx, k_tag_final, tg, dim3(kWB), 0, s, a, sl);
-  a.prof_cells = nullptr;
-  hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(1024), 0, s, sl, nt);
-  hipLaunchKernelGGL(k_seg_copy, dim3(nt, 1), dim3(kBT), 0, s, sl);
-  return check_hip(hipGetLastError(), "saddle_tags_walk");
+  const int cap = ceil_log2i(g.N) + 1;
+  int done_rounds = 0;
+  int64_t live = 1;
+  int step = std::max(2, std::min(ctx->last_tag_rounds + 1, cap));
+  while (done_rounds < cap) {
+    const int r = std::min(step, cap - done_rounds);
+    PFF_TRY(count_rounds(ctx, L, a.ringS, ringP2, b.t0, (uint32_t*)b.t1, b.t2, r, true, done_rounds, fe));
+    done_rounds += r;
+    step = 3;
+    // the finishing passes run on the device while the host waits for the
+    // list count of the rounds
+    PFF_TRY(post_words(ctx));
+    PFF_TRY(finish());
+    PFF_TRY(wait_words(ctx));
+    live = ctx->host_words[L.cur == 0 ? W_COUNT0 : W_COUNT1];
+    if (live == 0 || ctx->host_words[W_CYCLE] != 0) break;
+  }
+  const int64_t f = ctx->host_words[W_EMPTY2];
+  ctx->last_tag_rounds = (live == 0 && f < done_rounds) ? std::max<int>((int)f, 1) : done_rounds;
+  if (live != 0 || ctx->host_words[W_CYCLE] != 0) return fail(PFF_ENOTCONV, "saddle tags: receivers are not a forest");
+  return 0;
 }
 
 // Labels bid = init(root) with the tiled contraction, plus the depression
@@ -1679,7 +1689,7 @@ __global__ __launch_bounds__(kBT) void k_relabel_tiles(GridD g, int tiles_x, con
                                                        const int32_t* __restrict__ M, int32_t* __restrict__ bid,
                                                        uint32_t* __restrict__ tact_out, int64_t* npit,
                                                        const int64_t* n_next) {
-  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) *npit = *n_next;
+  if (npit && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) *npit = *n_next;
   int bx, by;
   xcd_block(bx, by);
   const int t = by * tiles_x + bx;
@@ -1737,7 +1747,15 @@ int relabel_from_outlets(pff_ctx* ctx, const GridD& g, int32_t* bid, const int64
   hipLaunchKernelGGL(k_basin_chase, dim3(lg), dim3(kBT), 0, s, (const int32_t*)b.pits, (const int64_t*)npit, g.N,
                      M[0], 64, chg);
   PFF_TRY(check_hip(hipGetLastError(), "basin_chase"));
-  PFF_TRY(read_words(ctx));
+  // The labels are mapped with the chased map while the host reads whether
+  // the chase finished.  If it did not, the rounds below finish the map and a
+  // second pass maps the labels again: each label is a basin on its own
+  // chain, so the final map sends it to the chain's end.  The pit count
+  // (read by the rounds) is replaced only afterwards.
+  PFF_TRY(post_words(ctx));
+  hipLaunchKernelGGL(k_relabel_tiles, tile_grid(g), dim3(64, 4), 0, s, g, (g.nx + TS - 1) / TS, tact_in,
+                     (const int32_t*)M[0], bid, tact_out, (int64_t*)nullptr, (const int64_t*)n_next);
+  PFF_TRY(wait_words(ctx));
   const bool chased = ctx->host_words[W_MCHG] == 0;
   if (!chased) PFF_TRY(check_hip(hipMemsetAsync(chg, 0, 8, s), "memset"));
   // a chain of basins is at most n_prev long: ceil(log2 n_prev) + 1 rounds
@@ -1776,8 +1794,10 @@ int relabel_from_outlets(pff_ctx* ctx, const GridD& g, int32_t* bid, const int64
     step = std::min(4, cap + 1 - done);
   }
   const int64_t n2 = ctx->host_words[W_NPIT2];
-  hipLaunchKernelGGL(k_relabel_tiles, tile_grid(g), dim3(64, 4), 0, s, g, (g.nx + TS - 1) / TS, tact_in,
-                     (const int32_t*)M[cur], bid, tact_out, npit, (const int64_t*)n_next);
+  if (!chased)
+    hipLaunchKernelGGL(k_relabel_tiles, tile_grid(g), dim3(64, 4), 0, s, g, (g.nx + TS - 1) / TS, tact_in,
+                       (const int32_t*)M[cur], bid, tact_out, (int64_t*)nullptr, (const int64_t*)n_next);
+  PFF_TRY(check_hip(hipMemcpyAsync(npit, n_next, 8, hipMemcpyDeviceToDevice, s), "copy"));
   PFF_TRY(check_hip(hipGetLastError(), "relabel_tiles"));
   // the uncarved basins become the pits list: swap the two list buffers
   std::swap(ctx->buf[S_TPIT], ctx->buf[S_TPIT2]);

@@ -51,7 +51,8 @@ struct pff_ctx {
     int64_t* dev_count = nullptr;    // device running count (8 bytes, owned by the context)
     bool armed = false;
   } lake_log;
-  hipEvent_t sync_ev = nullptr;  // spin-wait event of wait_stream
+  hipEvent_t sync_ev = nullptr;   // spin-wait event of wait_stream
+  hipEvent_t words_ev = nullptr;  // recorded after the device words' copy (post_words)
   bool prof_on = false;
   bool prof_pending = false;  // prof_begin made a record that the next launch takes
   std::vector<ProfRec> prof;
@@ -81,6 +82,10 @@ void prof_events(pff_ctx* ctx, hipEvent_t* start, hipEvent_t* stop);
 int wait_stream(pff_ctx* ctx);
 // copy the device words to the pinned host words and wait
 int read_words(pff_ctx* ctx);
+// the same in two halves: post the copy (the stream can take more work),
+// then wait for the copy alone
+int post_words(pff_ctx* ctx);
+int wait_words(pff_ctx* ctx);
 // device scratch slot of at least `bytes`
 int scratch(pff_ctx* ctx, int slot, size_t bytes, void** out, bool zero_new = false);
 
