@@ -525,16 +525,24 @@ __device__ __forceinline__ void saddle_offer(SaddleTable& t, int keyb, int64_t r
   }
 }
 
+// every stored minimum is read before the first atomic: a read issued after
+// an atomic waits for that atomic's round trip as well
 __device__ __forceinline__ void saddle_table_flush(SaddleTable& t, int64_t* saddle) {
   __syncthreads();
   const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-  for (int q = tid; q < kSaddleSlots; q += kBT) {
-    const int b = t.b[q];
-    if (b >= 0) {
-      const long long v = t.k[q];
-      if (v < (long long)saddle[b]) atomicMin((long long*)&saddle[b], v);
-    }
+  constexpr int Q = kSaddleSlots / kBT;
+  int b[Q];
+  long long v[Q], cur[Q];
+#pragma unroll
+  for (int k = 0; k < Q; ++k) {
+    b[k] = t.b[tid + k * kBT];
+    v[k] = t.k[tid + k * kBT];
   }
+#pragma unroll
+  for (int k = 0; k < Q; ++k) cur[k] = b[k] >= 0 ? (long long)saddle[b[k]] : 0;
+#pragma unroll
+  for (int k = 0; k < Q; ++k)
+    if (b[k] >= 0 && v[k] < cur[k]) atomicMin((long long*)&saddle[b[k]], v[k]);
 }
 
 // z', border flag and the offer of cell i (not an outlet cell); returns true
@@ -548,15 +556,23 @@ __device__ __forceinline__ bool zprime_cell(const GridD& g, int i, int row, int 
   bool border = false;
   float zn = 1e9f;
   int nbj[4];
+  // the neighbours' elevations are loaded with their labels (one round trip
+  // instead of a second one that waits for the label comparison)
+  int jn[4];
+  float zj[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const int jn = neighbour<MODE>(g, i, row, col, k);
+    jn[k] = neighbour<MODE>(g, i, row, col, k);
+    zj[k] = jn[k] >= 0 ? z[jn[k]] : 0.f;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
     nbj[k] = -1;
-    if (jn < 0) continue;
-    const int bj = bid[jn];
+    if (jn[k] < 0) continue;
+    const int bj = bid[jn[k]];
     if (bj != bi) {
       border = true;
-      zn = fminf(zn, z[jn]);
+      zn = fminf(zn, zj[k]);
       nbj[k] = bj;
     }
   }
@@ -677,16 +693,20 @@ template <int MODE>
 __device__ __forceinline__ void saddle_node_cell(const GridD& g, int i, int row, int col, int bi,
                                                  const int32_t* __restrict__ bid, const float* __restrict__ zp,
                                                  const int64_t* __restrict__ saddle, int32_t* snode) {
+  // the neighbours' labels are loaded before the saddle key arrives (one
+  // round trip fewer than loading them after the comparison)
+  int bj[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int jn = neighbour<MODE>(g, i, row, col, k);
+    bj[k] = jn >= 0 ? bid[jn] : -1;
+  }
   const int64_t key = saddle[bi];
   if (zp[i] != key_float(key)) return;
   const int tb = key_index(key);
   bool here = false;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int jn = neighbour<MODE>(g, i, row, col, k);
-    if (jn < 0) continue;
-    if (bid[jn] == tb) here = true;
-  }
+  for (int k = 0; k < 4; ++k) here |= bj[k] >= 0 && bj[k] == tb;
   if (here) atomicMax(&snode[bi], i);
 }
 
