@@ -981,8 +981,10 @@ __device__ __forceinline__ void tile_build_pointers(const TileArgs& a, int x0, i
 __device__ __forceinline__ int walk_tag(int li, const uint16_t* P, uint32_t* T, int64_t* cycle) {
   int c = li;
   for (int s = 0; s < TC; ++s) {
-    if (bit_test_set(T, c)) return -1;
+    // the pointer read is issued before the tag atomic returns: both LDS
+    // round trips of a step overlap
     const uint16_t n = P[c];
+    if (bit_test_set(T, c)) return -1;
     if (n == NONE) return c;
     c = n;
   }
