@@ -52,7 +52,7 @@ def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the routing path runs on a single MI355X here "
-              "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+              "(multi-GPU sharding is a design only, not run)", file=sys.stderr)
         sys.exit(2)
 
     import torch
