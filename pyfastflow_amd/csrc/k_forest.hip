@@ -1569,16 +1569,18 @@ int tiled_pits(pff_ctx* ctx, const GridD& g, int32_t** pits, int64_t** npits) {
 // root) or 0.  The pits of the next trip are the uncarved basins.
 
 // M[b] = bid[outlet node of b], or b when b has no outlet (then b is kept in
-// the next pits list); M[0] = 0
+// the next pits list); M[0] = 0; *np_saved = *np (the rounds after the
+// relabel read the old pit count there once the relabel has replaced *np)
 __global__ __launch_bounds__(kBT) void k_basin_next(const int32_t* __restrict__ pits, const int64_t* np,
                                                     const int64_t* __restrict__ outlet,
                                                     const int32_t* __restrict__ bid, int32_t N, int32_t* __restrict__ M0,
                                                     int32_t* __restrict__ M1, int32_t* __restrict__ next_pits,
-                                                    int64_t* n_next) {
+                                                    int64_t* n_next, int64_t* np_saved) {
   const int64_t n = *np;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     M0[0] = 0;
     M1[0] = 0;
+    *np_saved = n;
   }
   // the loop bound is uniform per block: block_append needs every thread
   for (int64_t base = (int64_t)blockIdx.x * kBT; base < n; base += (int64_t)gridDim.x * kBT) {
@@ -1738,12 +1740,13 @@ int relabel_from_outlets(pff_ctx* ctx, const GridD& g, int32_t* bid, const int64
   int64_t* npit = ctx->dev_words + W_NPIT;
   int64_t* n_next = ctx->dev_words + W_NPIT2;
   int64_t* chg = ctx->dev_words + W_MCHG;
+  const int64_t* nprev_d = ctx->dev_words + W_NPREV;  // the old pit count, kept by k_basin_next
   *cycle = 0;
   static_assert(W_MCHG == W_NPIT2 + 1, "next pit count and change round are adjacent words");
   PFF_TRY(check_hip(hipMemsetAsync(n_next, 0, 16, s), "memset"));
   const unsigned lg = grid_for(std::max<int64_t>(n_prev, 1), kBT, 4096);
   hipLaunchKernelGGL(k_basin_next, dim3(lg), dim3(kBT), 0, s, (const int32_t*)b.pits, (const int64_t*)npit, outlet,
-                     (const int32_t*)bid, g.N, M[0], M[1], next_pits, n_next);
+                     (const int32_t*)bid, g.N, M[0], M[1], next_pits, n_next, ctx->dev_words + W_NPREV);
   // chase the chains to their ends in one pass; the synchronous rounds below
   // finish what a cycle or a very long chain leaves unfinished
   hipLaunchKernelGGL(k_basin_chase, dim3(lg), dim3(kBT), 0, s, (const int32_t*)b.pits, (const int64_t*)npit, g.N,
@@ -1752,11 +1755,11 @@ int relabel_from_outlets(pff_ctx* ctx, const GridD& g, int32_t* bid, const int64
   // The labels are mapped with the chased map while the host reads whether
   // the chase finished.  If it did not, the rounds below finish the map and a
   // second pass maps the labels again: each label is a basin on its own
-  // chain, so the final map sends it to the chain's end.  The pit count
-  // (read by the rounds) is replaced only afterwards.
+  // chain, so the final map sends it to the chain's end.  This pass also
+  // replaces the pit count; the rounds read the old one from nprev_d.
   PFF_TRY(post_words(ctx));
   hipLaunchKernelGGL(k_relabel_tiles, tile_grid(g), dim3(64, 4), 0, s, g, (g.nx + TS - 1) / TS, tact_in,
-                     (const int32_t*)M[0], bid, tact_out, (int64_t*)nullptr, (const int64_t*)n_next);
+                     (const int32_t*)M[0], bid, tact_out, npit, (const int64_t*)n_next);
   PFF_TRY(wait_words(ctx));
   const bool chased = ctx->host_words[W_MCHG] == 0;
   if (!chased) PFF_TRY(check_hip(hipMemsetAsync(chg, 0, 8, s), "memset"));
@@ -1767,7 +1770,7 @@ int relabel_from_outlets(pff_ctx* ctx, const GridD& g, int32_t* bid, const int64
   int step = std::max(2, std::min(cap + 1, ctx->last_jump_rounds));
   while (!chased) {
     for (int k = 0; k < step; ++k) {
-      hipLaunchKernelGGL(k_basin_jump, dim3(lg), dim3(kBT), 0, s, (const int32_t*)b.pits, (const int64_t*)npit, g.N,
+      hipLaunchKernelGGL(k_basin_jump, dim3(lg), dim3(kBT), 0, s, (const int32_t*)b.pits, nprev_d, g.N,
                          (const int32_t*)M[cur], M[cur ^ 1], done + k + 1, chg);
       cur ^= 1;
     }
@@ -1779,7 +1782,7 @@ int relabel_from_outlets(pff_ctx* ctx, const GridD& g, int32_t* bid, const int64
     if (ctx->host_words[W_MCHG] < done) {
       ctx->last_jump_rounds = std::max<int>((int)ctx->host_words[W_MCHG] + 1, 1);
       PFF_TRY(check_hip(hipMemsetAsync(chg, 0, 8, s), "memset"));
-      hipLaunchKernelGGL(k_basin_check_ends, dim3(lg), dim3(kBT), 0, s, (const int32_t*)b.pits, (const int64_t*)npit,
+      hipLaunchKernelGGL(k_basin_check_ends, dim3(lg), dim3(kBT), 0, s, (const int32_t*)b.pits, nprev_d,
                          g.N, (const int32_t*)M[cur], outlet, chg);
       PFF_TRY(check_hip(hipGetLastError(), "basin_check_ends"));
       PFF_TRY(read_words(ctx));
@@ -1799,7 +1802,6 @@ int relabel_from_outlets(pff_ctx* ctx, const GridD& g, int32_t* bid, const int64
   if (!chased)
     hipLaunchKernelGGL(k_relabel_tiles, tile_grid(g), dim3(64, 4), 0, s, g, (g.nx + TS - 1) / TS, tact_in,
                        (const int32_t*)M[cur], bid, tact_out, (int64_t*)nullptr, (const int64_t*)n_next);
-  PFF_TRY(check_hip(hipMemcpyAsync(npit, n_next, 8, hipMemcpyDeviceToDevice, s), "copy"));
   PFF_TRY(check_hip(hipGetLastError(), "relabel_tiles"));
   // the uncarved basins become the pits list: swap the two list buffers
   std::swap(ctx->buf[S_TPIT], ctx->buf[S_TPIT2]);
