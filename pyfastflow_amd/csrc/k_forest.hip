@@ -1383,8 +1383,8 @@ int saddle_tags_walk(pff_ctx* ctx, const GridD& g, const int32_t* rcv, const int
   a.tag = tag;
   a.tact = tact;
   hipStream_t s = ctx->stream;
-  static_assert(W_NEXIT == W_CYCLE + 1, "cycle flag and exit count are adjacent words");
-  PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_CYCLE, 0, 16, s), "memset"));
+  // the cycle flag and the exit count were zeroed by this trip's saddle
+  // initialisation (saddlesort_active, the only caller's predecessor)
   const dim3 tg = tile_grid(g);
   const int nt = tiles_count(g);
   const int nslots = nt * kRing;

@@ -464,13 +464,16 @@ static dim3 cell_grid(const GridD& g) { return dim3((g.nx + 63) / 64, (g.ny + kC
 // every basin entry (lakeflow.py:164-167 initialises all of them): in the
 // first trip the pits touch almost every cache line of the three arrays, so
 // one streaming pass of whole lines is cheaper than the scattered per-pit
-// stores; also zeroes the list counters of this trip (clear0, clear1)
+// stores; also zeroes the words of this trip (clear0, clear1, and the two
+// words at clear2)
 __global__ __launch_bounds__(kBT) void k_saddle_init_all(int32_t n, int64_t* __restrict__ saddle,
                                                          int64_t* __restrict__ outlet, int32_t* __restrict__ snode,
-                                                         int64_t* clear0, int64_t* clear1) {
+                                                         int64_t* clear0, int64_t* clear1, int64_t* clear2) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     *clear0 = 0;
     *clear1 = 0;
+    clear2[0] = 0;
+    clear2[1] = 0;
   }
   for (int64_t b = (int64_t)blockIdx.x * kBT + threadIdx.x; b < n; b += (int64_t)gridDim.x * kBT) {
     saddle[b] = INVALID_KEY;
@@ -479,13 +482,18 @@ __global__ __launch_bounds__(kBT) void k_saddle_init_all(int32_t n, int64_t* __r
   }
 }
 
-// also zeroes the list counters of this trip (clear0, clear1; nullable)
+// also zeroes the words of this trip (clear0, clear1, and the two words at
+// clear2; nullable)
 __global__ __launch_bounds__(kBT) void k_saddle_init_list(const int32_t* __restrict__ pits, const int64_t* np,
                                                           int64_t* saddle, int64_t* outlet, int32_t* snode,
-                                                          int64_t* clear0, int64_t* clear1) {
+                                                          int64_t* clear0, int64_t* clear1, int64_t* clear2) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     if (clear0) *clear0 = 0;
     if (clear1) *clear1 = 0;
+    if (clear2) {
+      clear2[0] = 0;
+      clear2[1] = 0;
+    }
   }
   const int64_t n = *np;
   for (int64_t w = (int64_t)blockIdx.x * kBT + threadIdx.x; w < n; w += (int64_t)gridDim.x * kBT) {
@@ -903,12 +911,16 @@ int saddlesort_active(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t*
   hipStream_t s = ctx->stream;
   const int tiles_x = (g.nx + TSL - 1) / TSL;
   const dim3 lg(1024), bl(kBT), cb(64, 4);
+  // also zeroes the cycle flag and the exit count that this trip's carve
+  // tags (saddle_tags_walk, the next use of them) count into
+  static_assert(W_NEXIT == W_CYCLE + 1, "cycle flag and exit count are adjacent words");
+  int64_t* cyc_words = ctx->dev_words + W_CYCLE;
   if (in < 0)  // first trip of the call: every tile is active
     hipLaunchKernelGGL(k_saddle_init_all, dim3(8192), bl, 0, s, g.N, saddle, outlet, snode, NB[out],
-                       ctx->dev_words + W_NTAG);
+                       ctx->dev_words + W_NTAG, cyc_words);
   else
     hipLaunchKernelGGL(k_saddle_init_list, lg, bl, 0, s, pits, npits, saddle, outlet, snode, NB[out],
-                       ctx->dev_words + W_NTAG);
+                       ctx->dev_words + W_NTAG, cyc_words);
   const bool full = in < 0 || ctx->host_words[W_NBL0 + in] * 2 > (int64_t)g.N;
   const dim3 lgz(2048);
   if (full) {

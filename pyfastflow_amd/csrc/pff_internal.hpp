@@ -196,7 +196,8 @@ int tag_state(pff_ctx* ctx, const GridD& g, uint32_t** wbits, uint8_t** tag, boo
 // node upstream, each listed once in tag_list (count W_NTAG, zeroed by the
 // caller).  Per tile, the paths from the saddle nodes and from the tile's
 // ring cells are walked in LDS; only their ends enter the reduced graph.
-// rounds > 0: known bound of the reduced rounds (no host read).
+// rounds > 0: known bound of the reduced rounds (no host read).  Expects the
+// W_CYCLE and W_NEXIT words zeroed (saddlesort_active's saddle init does it).
 int saddle_tags_walk(pff_ctx* ctx, const GridD& g, const int32_t* rcv, const int32_t* bid, uint32_t* wbits,
                      uint8_t* tag, const uint32_t* tact, int rounds, int32_t* tag_list);
 int tiles_count(const GridD& g);
