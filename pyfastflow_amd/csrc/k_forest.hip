@@ -1688,12 +1688,16 @@ __global__ __launch_bounds__(kBT) void k_basin_check_ends(const int32_t* __restr
 
 // bid <- M[bid] on the tiles of tact_in; tact_out[t] = 1 for the tiles that
 // still hold a closed basin, 0 for the others (one block per 64x64 tile, so
-// every flag is written and none needs clearing first)
+// every flag is written and none needs clearing first).  With npit:
+// *npit = *n_next, and *n_next is zeroed for the next relabel.
 __global__ __launch_bounds__(kBT) void k_relabel_tiles(GridD g, int tiles_x, const uint32_t* __restrict__ tact_in,
                                                        const int32_t* __restrict__ M, int32_t* __restrict__ bid,
                                                        uint32_t* __restrict__ tact_out, int64_t* npit,
-                                                       const int64_t* n_next) {
-  if (npit && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) *npit = *n_next;
+                                                       int64_t* n_next) {
+  if (npit && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
+    *npit = *n_next;
+    *n_next = 0;
+  }
   int bx, by;
   xcd_block(bx, by);
   const int t = by * tiles_x + bx;
@@ -1743,7 +1747,8 @@ int relabel_from_outlets(pff_ctx* ctx, const GridD& g, int32_t* bid, const int64
   const int64_t* nprev_d = ctx->dev_words + W_NPREV;  // the old pit count, kept by k_basin_next
   *cycle = 0;
   static_assert(W_MCHG == W_NPIT2 + 1, "next pit count and change round are adjacent words");
-  PFF_TRY(check_hip(hipMemsetAsync(n_next, 0, 16, s), "memset"));
+  if (!ctx->relabel_words_zero) PFF_TRY(check_hip(hipMemsetAsync(n_next, 0, 16, s), "memset"));
+  ctx->relabel_words_zero = false;
   const unsigned lg = grid_for(std::max<int64_t>(n_prev, 1), kBT, 4096);
   hipLaunchKernelGGL(k_basin_next, dim3(lg), dim3(kBT), 0, s, (const int32_t*)b.pits, (const int64_t*)npit, outlet,
                      (const int32_t*)bid, g.N, M[0], M[1], next_pits, n_next, ctx->dev_words + W_NPREV);
@@ -1759,9 +1764,11 @@ int relabel_from_outlets(pff_ctx* ctx, const GridD& g, int32_t* bid, const int64
   // replaces the pit count; the rounds read the old one from nprev_d.
   PFF_TRY(post_words(ctx));
   hipLaunchKernelGGL(k_relabel_tiles, tile_grid(g), dim3(64, 4), 0, s, g, (g.nx + TS - 1) / TS, tact_in,
-                     (const int32_t*)M[0], bid, tact_out, npit, (const int64_t*)n_next);
+                     (const int32_t*)M[0], bid, tact_out, npit, n_next);
   PFF_TRY(wait_words(ctx));
   const bool chased = ctx->host_words[W_MCHG] == 0;
+  // read before the rounds' read-backs: the mapping pass zeroes the device word
+  const int64_t n2 = ctx->host_words[W_NPIT2];
   if (!chased) PFF_TRY(check_hip(hipMemsetAsync(chg, 0, 8, s), "memset"));
   // a chain of basins is at most n_prev long: ceil(log2 n_prev) + 1 rounds
   // reach every end; a label still changing then means a cycle
@@ -1798,10 +1805,11 @@ int relabel_from_outlets(pff_ctx* ctx, const GridD& g, int32_t* bid, const int64
     }
     step = std::min(4, cap + 1 - done);
   }
-  const int64_t n2 = ctx->host_words[W_NPIT2];
   if (!chased)
     hipLaunchKernelGGL(k_relabel_tiles, tile_grid(g), dim3(64, 4), 0, s, g, (g.nx + TS - 1) / TS, tact_in,
-                       (const int32_t*)M[cur], bid, tact_out, (int64_t*)nullptr, (const int64_t*)n_next);
+                       (const int32_t*)M[cur], bid, tact_out, (int64_t*)nullptr, n_next);
+  else
+    ctx->relabel_words_zero = true;  // count zeroed by the mapping pass, no change recorded
   PFF_TRY(check_hip(hipGetLastError(), "relabel_tiles"));
   // the uncarved basins become the pits list: swap the two list buffers
   std::swap(ctx->buf[S_TPIT], ctx->buf[S_TPIT2]);

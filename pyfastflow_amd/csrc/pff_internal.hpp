@@ -32,6 +32,9 @@ struct pff_ctx {
   // carves (each carve clears what it set); a carve that stopped half-way
   // leaves this set and the next one clears both in full
   bool tag_state_dirty = true;
+  // the relabel's next-pit count and change words are known to be zero (the
+  // last relabel's first mapping pass zeroed the count and its chase finished)
+  bool relabel_words_zero = false;
   // opt-in per-kernel timing (pff_ctx_profile): HIP events around each
   // instrumented launch on the launch stream, plus the cells it processed
   struct ProfRec {
