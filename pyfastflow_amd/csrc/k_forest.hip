@@ -1488,9 +1488,11 @@ int tiled_labels(pff_ctx* ctx, const GridD& g, const int32_t* rec, const uint32_
   a.tact = tact_in;
   a.mark_cycles = mark_cycles ? 1 : 0;
   hipStream_t s = ctx->stream;
-  PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_NEXIT, 0, 8, s), "memset"));
-  PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_CYCLE, 0, 8, s), "memset"));
-  PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_NPIT, 0, 8, s), "memset"));
+  // cycle flag, exit count and pit count in one fill (the word between them
+  // is unused)
+  static_assert(W_NEXIT == W_CYCLE + 1 && W_NDEP == W_CYCLE + 2 && W_NPIT == W_CYCLE + 3,
+                "cycle flag, exit count and pit count lie in four adjacent words");
+  PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_CYCLE, 0, 32, s), "memset"));
   const dim3 tg = tile_grid(g);
   a.prof_cells = prof_begin(ctx, "k_tile", 8.0);  // rcv in, E out
   SegLists sl;
