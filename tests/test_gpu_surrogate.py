@@ -100,7 +100,7 @@ def test_pool_upsample_concat_head(pf):
     hw = torch.randn(64, generator=g).cuda().requires_grad_(True)
     hb = torch.randn(1, generator=g).cuda().requires_grad_(True)
     y = ops.head1x1(c, hw, hb)
-    yr = (_ref64(c) * _ref64(hw)).sum(-1) + float(hb)
+    yr = (_ref64(c) * _ref64(hw)).sum(-1) + float(hb.detach())
     _close(y, yr, rel=1e-5, absf=1e-5, what="head")
     gy = torch.randn(y.shape, generator=g)
     y.backward(gy.cuda())
