@@ -161,7 +161,8 @@ def test_unet_gradients_against_cpu(pf):
     o64 = cpu_forward(p64, planes.to(torch.float64))
     l64 = ((o64 - target.to(torch.float64)) ** 2).mean()
     l64.backward()
-    assert abs(float(loss) - float(l64)) <= 0.02 * abs(float(l64)) + 1e-3
+    lv, rv = float(loss.detach()), float(l64.detach())
+    assert abs(lv - rv) <= 0.02 * abs(rv) + 1e-3
     for k, (pg, pr) in enumerate(zip(order, p64)):
         a = pg.grad.detach().cpu().to(torch.float64).ravel()
         r = pr.grad.ravel()
