@@ -1,9 +1,11 @@
 """Build oracle/libpff_oracle.so from pff_oracle.c (gcc, OpenMP).
 
 Test infrastructure: the oracle is the CPU restatement the HIP library is
-checked against, and the CPU baseline timed by bench.py.  Built portable
-(no -march=native: the library travels to other x86 hosts) and without
-floating-point contraction, so its f32 arithmetic is the plain IEEE one.
+checked against, and the CPU baseline timed by bench.py.  Built for
+x86-64-v3 (AVX2: every host this build runs on has it; -march=native would
+tie the library, which travels between hosts, to the build machine) and
+without floating-point contraction, so its f32 arithmetic is the plain IEEE
+one.
 """
 
 import os
@@ -17,7 +19,7 @@ LIB = os.path.join(HERE, "libpff_oracle.so")
 def build(verbose: bool = False) -> str:
     src = os.path.join(HERE, "pff_oracle.c")
     tmp = LIB + ".tmp"
-    cmd = ["gcc", "-O3", "-std=c11", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off", "-fno-fast-math",
+    cmd = ["gcc", "-O3", "-march=x86-64-v3", "-std=c11", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off", "-fno-fast-math",
            "-Wall", "-Wextra", "-Wno-unused-parameter", "-o", tmp, src, "-lm"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

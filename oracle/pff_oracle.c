@@ -254,6 +254,35 @@ int64_t ora_receivers_div_ambiguity(const ora_grid* g, const float* z) {
   return cnt;
 }
 
+/* Cells whose stored gradient (receivers.py:43,55-58) would differ in its bits
+   if the division by DX were a multiplication by f32(1/DX) (what a fast-math
+   compile may emit); receivers that differ are counted by
+   ora_receivers_div_ambiguity. */
+int64_t ora_gradient_div_ambiguity(const ora_grid* g, const float* z) {
+  const int64_t N = NN(g);
+  const float dx = g->dx;
+  const float inv = 1.0f / dx;
+  int64_t cnt = 0;
+#pragma omp parallel for reduction(+ : cnt) schedule(static)
+  for (int64_t ii = 0; ii < N; ++ii) {
+    const int32_t i = (int32_t)ii;
+    float s1 = 0.0f, s2 = 0.0f;
+    for (int k = 0; k < 4; ++k) {
+      const int32_t j = nb(g, i, k);
+      if (j == -1) continue;
+      const float d = z[i] - z[j];
+      const float t1 = d / dx, t2 = d * inv;
+      if (t1 > s1) s1 = t1;
+      if (t2 > s2) s2 = t2;
+    }
+    uint32_t b1, b2;
+    memcpy(&b1, &s1, 4);
+    memcpy(&b2, &s2, 4);
+    if (b1 != b2) cnt++;
+  }
+  return cnt;
+}
+
 /* ------------------------------------------------------------------------- */
 /* Accumulation: downstream_propag.py + flowfields.py:254-292                  */
 /* ------------------------------------------------------------------------- */

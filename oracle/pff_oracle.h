@@ -52,6 +52,7 @@ void ora_compute_receivers(const ora_grid* g, const float* z, int32_t* rcv, floa
 /* number of cells whose receiver choice would differ if (z_i - z_j)/dx were
    computed as (z_i - z_j) * fl(1/dx) */
 int64_t ora_receivers_div_ambiguity(const ora_grid* g, const float* z);
+int64_t ora_gradient_div_ambiguity(const ora_grid* g, const float* z);
 
 /* accumulation: reference schedule (f32) and exact forms */
 void ora_rcv2donor(const ora_grid* g, const int32_t* rcv, int32_t* dnr, int32_t* ndnr);

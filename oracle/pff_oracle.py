@@ -63,6 +63,7 @@ def lib():
         _lib.ora_pack_key.argtypes = [ctypes.c_float, c_int]
         _lib.ora_depression_counter.restype = c_i64
         _lib.ora_receivers_div_ambiguity.restype = c_i64
+        _lib.ora_gradient_div_ambiguity.restype = c_i64
         _lib.ora_num_threads.restype = c_int
         _lib.ora_sentinel_hits.restype = c_i64
         _lib.ora_sentinel_hits.argtypes = [c_int]
@@ -147,6 +148,11 @@ def compute_receivers(g: Grid, z):
 def receivers_div_ambiguity(g: Grid, z) -> int:
     z = _f32(z)
     return int(lib().ora_receivers_div_ambiguity(g.ref, _p(z)))
+
+
+def gradient_div_ambiguity(g: Grid, z) -> int:
+    z = _f32(z)
+    return int(lib().ora_gradient_div_ambiguity(g.ref, _p(z)))
 
 
 def rcv2donor(g: Grid, rcv):

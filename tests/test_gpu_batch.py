@@ -75,6 +75,36 @@ def test_batch_vs_oracle_per_tile(pf, ora):
     assert res.status["capped_tiles"] == capped
 
 
+def test_ragged_256_tiles_vs_oracle(pf, ora):
+    """The batched-routing workload itself: 256 tiles with sides log-uniform
+    in [128, 1024] (seed 404), routed in one launch sequence; every tile's
+    receivers equal the oracle's reroute_flow(carve) bit for bit, and its Q
+    equals f32(q0) times the oracle's exact upstream count (error stated per
+    tile: the largest relative error of all tiles against the FP64 sums)."""
+    tiles = [t["z"] for t in synth.ragged_batch(256)]
+    ncell = sum(t.size for t in tiles)
+    print(f"case batch ragged-256: {ncell} cells, largest side {max(t.shape[0] for t in tiles)}", flush=True)
+    res = pf.flow.route_batch(tiles, dx=DX, lakeflow=True, value=1e-5)
+    torch.cuda.synchronize()
+    assert not res.status["per_tile"]
+    assert res.status["cells"] == ncell
+    q0 = np.float32(1e-5 * DX * DX)
+    worst = (0.0, -1)
+    for k, z in enumerate(tiles):
+        g, rcv = _oracle(ora, z)
+        np.testing.assert_array_equal(res.receivers[k].cpu().numpy(), rcv, err_msg=f"tile {k} {z.shape}")
+        cnt = ora.upstream_count(g, rcv)
+        Q = res.Q[k].cpu().numpy()
+        np.testing.assert_array_equal(Q, (q0 * cnt.astype(np.float32)).astype(np.float32),
+                                      err_msg=f"tile {k} {z.shape}")
+        rel = float((np.abs(Q.astype(np.float64) - cnt * float(q0)) / (cnt * float(q0))).max())
+        worst = max(worst, (rel, k))
+        if k % 32 == 31:
+            print(f"  {k + 1} tiles checked", flush=True)
+    print(f"  Q max relative error vs FP64: {worst[0]:.3e} (tile {worst[1]})", flush=True)
+    assert worst[0] <= 1e-5
+
+
 def test_batch_vs_single_grid_runs(pf):
     tiles = [t["z"] for t in synth.ragged_batch(n_tiles=24, seed=404, lo=64, hi=512)]
     print(f"case batch ragged {len(tiles)} tiles, {sum(t.size for t in tiles)} cells", flush=True)
