@@ -184,6 +184,9 @@ typedef struct {
   double first_above_frac; /* the same at the first check */
   double mean_change;      /* mean |h(t) - h(t - check_every)| over non-NaN cells (/N), last check, m */
   double first_mean_change;/* the same at the first check */
+  double q50, q90, q99, q999; /* quantiles of |h(t) - h(t - check_every)| over non-NaN cells, last
+                                 check: upper edge of the 1/8-decade histogram bin holding each, m */
+  int64_t nan_iteration;   /* iteration count at the first check that saw a NaN h (0: none) */
 } pff_gf_status;
 
 /* gf_fields.py:111-141  Flooder.run_graphflood(N): n_iter iterations of

@@ -877,6 +877,33 @@ void ora_diffuse_Q_constant_prec_f64(const ora_grid* g, const ora_hydro* p, cons
 /* graphflood_core_cte_mannings: gf_hydrodynamics.py:78-129.  h receives tdh
    twice (:121 and :128) and zh once (:129); the negative clamp at :122-123
    only changes dh. */
+/* h^(5/3) rounded to f32 (gf_hydrodynamics.py:115 computes ti.pow(h, 5/3)
+ * in f32; Taichi's powf is not pinned here).  Built from IEEE-exact double
+ * operations only, so that the HIP kernel's copy of this function gives the
+ * same bits: the cube root y of x by Newton's method y <- y - (y^3 - x)/(3y^2)
+ * from the bit-level seed bits(x)/3 + 0x2A9F7893782DA1CE (a division of the
+ * exponent by three, within 4% of the root), six steps (the iterate settles
+ * to the last bit by the fourth), then x*y*y rounded once to f32.  Negative
+ * and NaN depths give NaN, as powf does for a non-integer exponent.  Equal to
+ * f32(pow((double)h, 5/3)) on every value tested (tests/test_oracle_fill_gf.py). */
+float ora_pow53(float h) {
+  if (h != h || h < 0.0f) return NAN;
+  if (h == 0.0f) return 0.0f;
+  if (isinf(h)) return INFINITY;
+  const double x = (double)h;
+  uint64_t b;
+  memcpy(&b, &x, 8);
+  b = b / 3u + 0x2A9F7893782DA1CEull;
+  double y;
+  memcpy(&y, &b, 8);
+  for (int k = 0; k < 6; ++k) y = y - (y * y * y - x) / (3.0 * y * y);
+  return (float)(x * y * y);
+}
+
+void ora_pow53_array(int64_t n, const float* in, float* out) {
+  for (int64_t i = 0; i < n; ++i) out[i] = ora_pow53(in[i]);
+}
+
 void ora_graphflood_core_cte_mannings(const ora_grid* g, const ora_hydro* p, float* h, float* zh,
                                       const int32_t* rec, const float* Q) {
   const int64_t N = NN(g);
@@ -885,7 +912,6 @@ void ora_graphflood_core_cte_mannings(const ora_grid* g, const ora_hydro* p, flo
   const float man = (float)p->manning;
   const float dt = (float)p->dt;
   const float dx2 = (float)((double)g->dx * (double)g->dx);
-  const float e53 = (float)(5.0 / 3.0);
   float* dh = (float*)malloc((size_t)N * 4);
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < N; ++i) {
@@ -894,7 +920,7 @@ void ora_graphflood_core_cte_mannings(const ora_grid* g, const ora_hydro* p, flo
       const float s = (zh[i] - zh[rec[i]]) / dx;
       tS = s > 1e-4f ? s : 1e-4f;
     }
-    const float Qo = dx * powf(h[i], e53) / man * sqrtf(tS);
+    const float Qo = dx * ora_pow53(h[i]) / man * sqrtf(tS);
     float tdh = (Q[i] - Qo) / dx2 * dt;
     if (p->update == 0) {
       h[i] += tdh;

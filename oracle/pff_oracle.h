@@ -107,6 +107,9 @@ void ora_topofill(const ora_grid* g, float* z, const int32_t* rec, double eps);
 typedef struct { double prec, manning, edge_slope, dt; int32_t update; } ora_hydro;
 void ora_diffuse_Q_constant_prec(const ora_grid* g, const ora_hydro* p, const float* zh, float* Q);
 void ora_diffuse_Q_constant_prec_f64(const ora_grid* g, const ora_hydro* p, const double* zh, double* Q);
+/* h^(5/3) rounded to f32 from exact double operations (the HIP kernel holds the same function) */
+float ora_pow53(float h);
+void ora_pow53_array(int64_t n, const float* in, float* out);
 void ora_graphflood_core_cte_mannings(const ora_grid* g, const ora_hydro* p, float* h, float* zh,
                                       const int32_t* rec, const float* Q);
 void ora_graphflood_core_cte_mannings_f64(const ora_grid* g, const ora_hydro* p, double* h, double* zh,

@@ -72,6 +72,7 @@ def lib():
         _lib.ora_fill_z_add_delta.argtypes = [c_p, c_p, c_p, c_p, c_d]
         _lib.ora_fill_z_add_delta_f64.argtypes = [c_p, c_p, c_p, c_p, c_d]
         _lib.ora_topofill.argtypes = [c_p, c_p, c_p, c_d]
+        _lib.ora_pow53_array.argtypes = [ctypes.c_int64, c_p, c_p]
     return _lib
 
 
@@ -301,6 +302,15 @@ def topofill(g: Grid, z, rec, eps=1e-4):
     rec = _i32(rec)
     lib().ora_topofill(g.ref, _p(z), _p(rec), c_d(eps))
     return z
+
+
+def pow53(h):
+    """h^(5/3) rounded to f32, as the Manning update computes it (exact
+    double operations; equal to f32(pow(double(h), 5/3)) on tested values)."""
+    h = _f32(h)
+    out = np.empty_like(h)
+    lib().ora_pow53_array(h.size, _p(h), _p(out))
+    return out
 
 
 def hydro(prec=10e-3 / 3600, manning=0.033, edge_slope=1e-2, dt=5e-3, update=0):

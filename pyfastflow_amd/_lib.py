@@ -7,6 +7,7 @@ every compute call raises.
 
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
 import threading
@@ -53,6 +54,11 @@ class PffGfStatus(ctypes.Structure):
         ("first_above_frac", c_d),
         ("mean_change", c_d),
         ("first_mean_change", c_d),
+        ("q50", c_d),
+        ("q90", c_d),
+        ("q99", c_d),
+        ("q999", c_d),
+        ("nan_iteration", c_i64),
     ]
 
 
@@ -140,6 +146,24 @@ _RESTYPE = {"pff_version": ctypes.c_char_p, "pff_last_error": ctypes.c_char_p}
 _lib = None
 _lock = threading.Lock()
 _ctx = {}
+
+
+def _destroy_contexts():
+    """Release every context (its events, pinned words and scratch) while the
+    HIP runtime is still up.  Registered after torch is imported, so Python
+    runs it before torch's own exit hooks and before the runtime unloads."""
+    lib = _lib
+    if lib is None:
+        return
+    while _ctx:
+        _, c = _ctx.popitem()
+        try:
+            lib.pff_ctx_destroy(c)
+        except Exception:  # noqa: BLE001 - nothing useful to do at exit
+            pass
+
+
+atexit.register(_destroy_contexts)
 
 
 def load():

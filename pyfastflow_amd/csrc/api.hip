@@ -349,6 +349,7 @@ int pff_ctx_destroy(pff_ctx* ctx) {
   if (ctx->lake_log.dev_count) (void)hipFree(ctx->lake_log.dev_count);
   if (ctx->sync_ev) (void)hipEventDestroy(ctx->sync_ev);
   if (ctx->words_ev) (void)hipEventDestroy(ctx->words_ev);
+  if (ctx->switch_ev) (void)hipEventDestroy(ctx->switch_ev);
   if (ctx->host_words) (void)hipHostFree(ctx->host_words);
   delete ctx;
   return 0;
@@ -356,7 +357,17 @@ int pff_ctx_destroy(pff_ctx* ctx) {
 
 int pff_ctx_set_stream(pff_ctx* ctx, void* hip_stream) {
   if (!ctx) return fail(PFF_EINVAL, "context is NULL");
-  ctx->stream = (hipStream_t)hip_stream;
+  const hipStream_t ns = (hipStream_t)hip_stream;
+  if (ns == ctx->stream) return 0;
+  // work already queued on the old stream may still use the context's
+  // scratch and device words: the new stream waits for it before anything
+  // the context enqueues there
+  PFF_TRY(begin(ctx));
+  if (!ctx->switch_ev)
+    PFF_TRY(check_hip(hipEventCreateWithFlags(&ctx->switch_ev, hipEventDisableTiming), "event"));
+  PFF_TRY(check_hip(hipEventRecord(ctx->switch_ev, ctx->stream), "event record"));
+  PFF_TRY(check_hip(hipStreamWaitEvent(ns, ctx->switch_ev, 0), "stream wait"));
+  ctx->stream = ns;
   return 0;
 }
 
@@ -682,12 +693,13 @@ struct GfFields {
   float *zp, *Qt, *dh, *hprev;
   int64_t *outlet, *saddle;
   uint8_t *isb, *tag, *tag_;
+  uint32_t* hist;  // residual histogram (kResidBins counters)
 };
 
 // one device block holding every field the iteration needs besides zh, h, Q, rec
 int gf_fields(pff_ctx* ctx, const GridD& d, GfFields& f) {
   const size_t N = (size_t)d.N;
-  const size_t bytes = N * (4 * 4 + 4 * 4 + 8 * 2) + N * 3 + 4096;
+  const size_t bytes = N * (4 * 4 + 4 * 4 + 8 * 2) + N * 3 + 8192;
   void* base;
   PFF_TRY(scratch(ctx, S_GF0, bytes, &base, true));
   char* p = (char*)base;
@@ -709,6 +721,7 @@ int gf_fields(pff_ctx* ctx, const GridD& d, GfFields& f) {
   f.isb = (uint8_t*)take(N);
   f.tag = (uint8_t*)take(N);
   f.tag_ = (uint8_t*)take(N);
+  f.hist = (uint32_t*)take(kResidBins * 4);
   return 0;
 }
 
@@ -729,17 +742,37 @@ int gf_iteration(pff_ctx* ctx, const pff_grid* g, const GridD& d, const pff_hydr
   return launch_manning(ctx, d, p, h, zh, f.dh, rec, Q);
 }
 
-int gf_check(pff_ctx* ctx, const GridD& d, const float* h, const float* hprev, double tol, double* res,
-             int64_t* nans, int64_t* above, double* mean) {
+// upper edge of the histogram bin holding the q-quantile of n values
+double resid_quantile(const uint32_t* hist, int64_t n, double q) {
+  if (n <= 0) return 0.0;
+  const double want = q * (double)n;
+  double cum = 0.0;
+  for (int k = 0; k < kResidBins; ++k) {
+    cum += (double)hist[k];
+    if (cum >= want) return k == 0 ? 0.0 : std::pow(10.0, (k - 80) / 8.0);
+  }
+  return std::pow(10.0, (kResidBins - 80) / 8.0);
+}
+
+int gf_check(pff_ctx* ctx, const GridD& d, const float* h, const float* hprev, double tol, uint32_t* hist_dev,
+             double* res, int64_t* nans, int64_t* above, double* mean, double* quant /* q50, q90, q99, q999 */) {
   uint32_t* mx = (uint32_t*)(ctx->dev_words + W_GFMAX);
   unsigned long long* nn = (unsigned long long*)(ctx->dev_words + W_GFNAN);
   unsigned long long* ab = (unsigned long long*)(ctx->dev_words + W_GFABOVE);
   double* sm = (double*)(ctx->dev_words + W_GFSUM);
   static_assert(W_GFNAN == W_GFMAX + 1 && W_GFSUM == W_GFABOVE + 1, "residual words are pairs");
-  PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_GFMAX, 0, 16, ctx->stream), "memset"));
-  PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_GFABOVE, 0, 16, ctx->stream), "memset"));
-  PFF_TRY(launch_maxdiff(ctx, d.N, h, hprev, mx, nn, (float)tol, ab, sm));
+  PFF_TRY(launch_fill_i64(ctx, 2, ctx->dev_words + W_GFMAX, 0));
+  PFF_TRY(launch_fill_i64(ctx, 2, ctx->dev_words + W_GFABOVE, 0));
+  PFF_TRY(launch_fill_u32(ctx, kResidBins, hist_dev, 0u));
+  PFF_TRY(launch_maxdiff(ctx, d.N, h, hprev, mx, nn, (float)tol, ab, sm, hist_dev));
   PFF_TRY(read_words(ctx));
+  uint32_t hist[kResidBins];
+  PFF_TRY(check_hip(hipMemcpy(hist, hist_dev, sizeof(hist), hipMemcpyDeviceToHost), "d2h"));
+  const int64_t finite = (int64_t)d.N - ctx->host_words[W_GFNAN];
+  quant[0] = resid_quantile(hist, finite, 0.5);
+  quant[1] = resid_quantile(hist, finite, 0.9);
+  quant[2] = resid_quantile(hist, finite, 0.99);
+  quant[3] = resid_quantile(hist, finite, 0.999);
   double sumv;
   std::memcpy(&sumv, &ctx->host_words[W_GFSUM], 8);
   *mean = sumv / (double)d.N;
@@ -801,11 +834,16 @@ int pff_graphflood_to_steady(pff_ctx* ctx, const pff_grid* g, const pff_hydro* p
     if (n < check_every) break;  // a partial block cannot be judged
     double res = 0;
     int64_t nans = 0, above = 0;
-    double mean = 0;
-    PFF_TRY(gf_check(ctx, d, h, f.hprev, tol, &res, &nans, &above, &mean));
+    double mean = 0, quant[4];
+    PFF_TRY(gf_check(ctx, d, h, f.hprev, tol, f.hist, &res, &nans, &above, &mean, quant));
     s.checks++;
     s.residual = res;
     s.nan_cells = nans;
+    s.q50 = quant[0];
+    s.q90 = quant[1];
+    s.q99 = quant[2];
+    s.q999 = quant[3];
+    if (nans > 0 && s.nan_iteration == 0) s.nan_iteration = s.iterations;
     s.above_frac = (double)above / (double)d.N;
     s.mean_change = mean;
     if (s.first_residual < 0) {

@@ -2,7 +2,31 @@
 // gather-form (atomic-free) discharge diffusion and the Manning depth update.
 #include "pff_internal.hpp"
 
+// No fused multiply-add contraction in this file: every f32 operation rounds
+// as written, as in the oracle's C restatement (built with contraction off),
+// so a GraphFlood iteration gives the same bits on both.
+#pragma clang fp contract(off)
+
 namespace pff {
+
+// h^(5/3) rounded to f32 (gf_hydrodynamics.py:115, ti.pow(h, 5/3)), from
+// IEEE-exact double operations only so that the oracle's ora_pow53 gives the
+// same bits: cube root y of x by Newton's method y <- y - (y^3 - x)/(3y^2)
+// from the seed bits(x)/3 + 0x2A9F7893782DA1CE (the exponent divided by
+// three, within 4% of the root; settled to the last bit by the fourth step),
+// six steps, then x*y*y rounded once.  Negative or NaN depth gives NaN, as
+// powf does for a non-integer exponent.
+__device__ __forceinline__ float pow53(float h) {
+  if (!(h >= 0.0f)) return __builtin_nanf("");
+  if (h == 0.0f) return 0.0f;
+  if (h == __builtin_inff()) return h;
+  const double x = (double)h;
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x) / 3ull + 0x2A9F7893782DA1CEull;
+  double y = __longlong_as_double((long long)b);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) y = y - (y * y * y - x) / (3.0 * y * y);
+  return (float)(x * y * y);
+}
 
 // ---------------------------------------------------------------------------
 // _topofill passes (fill_topo.py:16-45), synchronous form: every read sees the
@@ -165,7 +189,7 @@ int launch_diffuse(pff_ctx* ctx, const GridD& g, const pff_hydro* p, const float
 // kernel applies dh after every slope has been read.
 // ---------------------------------------------------------------------------
 struct ManningC {
-  float dx, edgesw, manning, dt, dx2, e53;
+  float dx, edgesw, manning, dt, dx2;
   int update;
 };
 
@@ -180,7 +204,7 @@ __global__ void k_manning1(int32_t N, ManningC c, float* __restrict__ h, const f
     tS = s > 1e-4f ? s : 1e-4f;
   }
   const float hi = h[i];
-  const float Qo = c.dx * powf(hi, c.e53) / c.manning * sqrtf(tS);
+  const float Qo = c.dx * pow53(hi) / c.manning * sqrtf(tS);
   float tdh = (Q[i] - Qo) / c.dx2 * c.dt;
   if (c.update == 0) {
     const float hn = hi + tdh;
@@ -210,7 +234,6 @@ int launch_manning(pff_ctx* ctx, const GridD& g, const pff_hydro* p, float* h, f
   c.manning = (float)p->manning;
   c.dt = (float)p->dt;
   c.dx2 = (float)((double)g.dx * (double)g.dx);
-  c.e53 = (float)(5.0 / 3.0);
   c.update = p->update;
   const dim3 gr(grid_for(g.N)), bl(kBlock);
   hipLaunchKernelGGL(k_manning1, gr, bl, 0, ctx->stream, g.N, c, h, zh, dh, rec, Q);
@@ -218,14 +241,24 @@ int launch_manning(pff_ctx* ctx, const GridD& g, const pff_hydro* p, float* h, f
   return check_hip(hipGetLastError(), "manning");
 }
 
-// max |a - b| over N cells (both finite) and the number of NaN cells of a;
-// out[0] holds the max as float bits (non-negative floats order as integers),
-// out[1] the NaN count
-// max |a - b| over non-NaN cells, NaN count, and (with out_above) the number
-// of cells with |a - b| > tol or NaN
+// bin of |a - b| in the residual histogram: 0 for an exact zero, then eight
+// bins per decade from 1e-10 (bin 1 holds everything below) to 1e5 (the last
+// bin holds everything above): bin k >= 1 ends at 10^((k - 80) / 8)
+__device__ __forceinline__ int resid_bin(float d) {
+  if (d == 0.0f) return 0;
+  const float l = floorf(8.0f * log10f(d)) + 81.0f;
+  return l < 1.0f ? 1 : (l > (float)(kResidBins - 1) ? kResidBins - 1 : (int)l);
+}
+
+// max |a - b| over non-NaN cells (as float bits: non-negative floats order as
+// integers), NaN count of a, (with out_above) the cells with |a - b| > tol or
+// NaN, the sum of |a - b|, and (with hist) its histogram (resid_bin)
 __global__ __launch_bounds__(256) void k_maxdiff(int32_t N, const float* __restrict__ a, const float* __restrict__ b,
                                                  uint32_t* out_max, unsigned long long* out_nan, float tol,
-                                                 unsigned long long* out_above, double* out_sum) {
+                                                 unsigned long long* out_above, double* out_sum, uint32_t* hist) {
+  __shared__ uint32_t lh[kResidBins];
+  for (int k = threadIdx.x; k < kResidBins; k += blockDim.x) lh[k] = 0u;
+  __syncthreads();
   float m = 0.0f;
   double sum = 0.0;
   unsigned nan = 0, above = 0;
@@ -239,7 +272,12 @@ __global__ __launch_bounds__(256) void k_maxdiff(int32_t N, const float* __restr
     m = d > m ? d : m;
     above += d > tol ? 1u : 0u;
     sum += (double)d;
+    if (hist) atomicAdd(&lh[resid_bin(d)], 1u);
   }
+  __syncthreads();
+  if (hist)
+    for (int k = threadIdx.x; k < kResidBins; k += blockDim.x)
+      if (lh[k]) atomicAdd(&hist[k], lh[k]);
   for (int o = 32; o > 0; o >>= 1) {
     m = fmaxf(m, __shfl_down(m, o));
     nan += __shfl_down(nan, o);
@@ -255,9 +293,10 @@ __global__ __launch_bounds__(256) void k_maxdiff(int32_t N, const float* __restr
 }
 
 int launch_maxdiff(pff_ctx* ctx, int32_t N, const float* a, const float* b, uint32_t* out_max,
-                   unsigned long long* out_nan, float tol, unsigned long long* out_above, double* out_sum) {
+                   unsigned long long* out_nan, float tol, unsigned long long* out_above, double* out_sum,
+                   uint32_t* hist) {
   hipLaunchKernelGGL(k_maxdiff, dim3(1024), dim3(256), 0, ctx->stream, N, a, b, out_max, out_nan, tol, out_above,
-                     out_sum);
+                     out_sum, hist);
   return check_hip(hipGetLastError(), "maxdiff");
 }
 

@@ -56,6 +56,10 @@ struct pff_ctx {
   } lake_log;
   hipEvent_t sync_ev = nullptr;   // spin-wait event of wait_stream
   hipEvent_t words_ev = nullptr;  // recorded after the device words' copy (post_words)
+  hipEvent_t switch_ev = nullptr; // orders a new stream after the old one (pff_ctx_set_stream)
+  // scratch is a fixed arena set up by the caller: scratch() never allocates
+  // and fails when a slot is too small (the no-allocation entry points)
+  bool no_alloc = false;
   bool prof_on = false;
   bool prof_pending = false;  // prof_begin made a record that the next launch takes
   std::vector<ProfRec> prof;
@@ -285,9 +289,12 @@ int launch_head_bwd(pff_ctx* ctx, int64_t npix, int C, const uint16_t* x, const 
 int launch_pack_input(pff_ctx* ctx, int64_t npix, int nplanes, int C, const float* planes, uint16_t* x);
 
 int launch_edge_maps(pff_ctx* ctx, const GridD& g, uint8_t* edges, uint8_t* out);
+// residual histogram of launch_maxdiff: bin 0 = exact zero, bins 1.. eight per
+// decade, bin k >= 1 ending at 10^((k - 80) / 8) (1e-10 .. 1e5, ends clamped)
+constexpr int kResidBins = 122;
 int launch_maxdiff(pff_ctx* ctx, int32_t N, const float* a, const float* b, uint32_t* out_max,
                    unsigned long long* out_nan, float tol = 0.f, unsigned long long* out_above = nullptr,
-                   double* out_sum = nullptr);
+                   double* out_sum = nullptr, uint32_t* hist = nullptr);
 int launch_pack(pff_ctx* ctx, int64_t n, int64_t* arr, const float* ft, const int32_t* it);
 int launch_unpack(pff_ctx* ctx, int64_t n, const int64_t* arr, float* ft, int32_t* it);
 int launch_swap(pff_ctx* ctx, int64_t n, uint32_t* a, uint32_t* b);

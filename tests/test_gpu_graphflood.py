@@ -140,6 +140,53 @@ def test_graphflood_iterations_vs_oracle(pf, ora, update):
         zh, h, Q = r.z.to_numpy().copy(), fl.h.to_numpy().copy(), r.Q.to_numpy().copy()
 
 
+def _err_stats(got, ref):
+    """max / p99 absolute error, argmax cell, and the cells outside 1e-5
+    relative / 1e-4 absolute, of got against ref (FP64)."""
+    e = np.abs(np.asarray(got, np.float64) - ref)
+    bad = (e > ATOL) & (e > RTOL * np.abs(ref))
+    return dict(max=float(e.max()), p99=float(np.quantile(e, 0.99)), argmax=int(np.argmax(e)),
+                out_of_tol=int(bad.sum()))
+
+
+@pytest.mark.parametrize("update", ["reference", "single"])
+def test_graphflood_60_iterations_end_to_end(pf, ora, update):
+    """60 iterations of the reference schedule from a cold start on a GraphFlood
+    DEM (beta 2.2, 300 m relief, 2 pits, open edges, 96 x 128) on the GPU and
+    in the f32 oracle, with no resync: receivers, zh, h and Q bit-exact at
+    every 10th iteration (same f32 operations in the same order, no FMA
+    contraction, h^(5/3) from exact double operations on both).
+
+    Both runs are also compared with the FP64 restatement (routing on the f32
+    rounding of its zh): the error per result (max, p99, argmax cell) is
+    printed, and it is the f32 arithmetic's own, equal for GPU and oracle.
+    zh and h are stored in f32 at the elevation's magnitude (ulp(300 m) =
+    3e-5 m) and every iteration rounds them, so this drift grows past 1e-4 m
+    within tens of iterations for the reference's f32 method itself."""
+    ny, nx = 96, 128
+    z = synth.dem(ny, nx, 2.2, 300.0, 303, n_pits=2, dx=10.0)
+    r = pf.flow.FlowRouter(nx, ny, 10.0, boundary_mode="normal")
+    r.set_z(z)
+    fl = pf.graphflood.Flooder(r, precipitation_rates=1e-5, manning=0.033, edge_slope=1e-2, update=update)
+    g = ora.Grid(nx, ny, 10.0, 0)
+    hy = ora.hydro(prec=1e-5, manning=0.033, edge_slope=1e-2, dt=5e-3, update=0 if update == "reference" else 1)
+    zh, h, Q = z.ravel().copy(), np.zeros(nx * ny, np.float32), np.zeros(nx * ny, np.float32)
+    zh64, h64, Q64 = zh.astype(np.float64), np.zeros(nx * ny), np.zeros(nx * ny)
+    for blk in range(6):
+        fl.run_graphflood(10)
+        for _ in range(10):
+            zh, h, Q, rec = ora.graphflood_iteration(g, hy, zh, h, Q)
+            zh64, h64, Q64, _ = ora.graphflood_iteration(g, hy, zh64, h64, Q64, f64=True)
+        it = 10 * (blk + 1)
+        print(f"case graphflood e2e {update} iteration {it}", flush=True)
+        np.testing.assert_array_equal(r.receivers.to_numpy(), rec)
+        np.testing.assert_array_equal(r.z.to_numpy().view(np.uint32), zh.view(np.uint32))
+        np.testing.assert_array_equal(fl.h.to_numpy().view(np.uint32), h.view(np.uint32))
+        np.testing.assert_array_equal(r.Q.to_numpy().view(np.uint32), Q.view(np.uint32))
+        for name, a, b in (("h", h, h64), ("zh", zh, zh64), ("Q", Q, Q64)):
+            print(f"  {name} vs FP64: {_err_stats(a, b)}", flush=True)
+
+
 def test_graphflood_survives_carve_cycles(pf):
     """On the 2048x2048 GraphFlood workload the lakeflow carve of the water
     surface leaves a receiver cycle within the first 20 iterations (only

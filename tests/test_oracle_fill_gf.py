@@ -126,3 +126,45 @@ def test_manning_slope_floor_and_receiver_slope(ora):
     h2, _ = ora.manning(g, hy, h, zh, rec, Q, f64=True)
     qo0 = dx * 1.0 / n * math.sqrt(1e-4)
     assert h2[0] == pytest.approx(1.0 + 2 * (-qo0 / dx**2 * dt), rel=1e-12)
+
+
+def test_pow53_equals_rounded_double_pow(ora):
+    """The Manning update's h^(5/3) (exact double operations, Newton cube
+    root) equals numpy's float64 power rounded to f32 on 400 000 depths over
+    the whole positive f32 range.  numpy's f32 power raises to f32(5/3) =
+    5/3 - 4e-8 (the reference's f32 literal), a relative change of 4e-8 |ln h|,
+    i.e. 0.67 |ln h| ulp (at most ~5 ulp for depths of 1e-3 to 200 m) plus
+    one ulp of rounding.  Zero, negative, infinite and
+    NaN depths follow powf."""
+    rng = np.random.default_rng(5)
+    v = np.concatenate([10 ** rng.uniform(-44, 38, 200_000), rng.uniform(0, 200, 200_000)]).astype(np.float32)
+    r = ora.pow53(v)
+    with np.errstate(over="ignore"):
+        ref = (v.astype(np.float64) ** (5.0 / 3.0)).astype(np.float32)
+        r32 = np.power(v, np.float32(5.0 / 3.0))
+    np.testing.assert_array_equal(r, ref)
+    fin = (v >= 1e-3) & (v <= 200.0)
+    ulp = np.abs(r[fin].view(np.int32).astype(np.int64) - r32[fin].view(np.int32).astype(np.int64))
+    assert np.all(ulp <= 1 + np.ceil(0.68 * np.abs(np.log(v[fin].astype(np.float64)))))
+    sp = ora.pow53(np.array([0.0, -0.0, -1.0, np.inf, np.nan], np.float32))
+    assert sp[0] == 0 and sp[1] == 0 and np.isnan(sp[2]) and sp[3] == np.inf and np.isnan(sp[4])
+
+
+@pytest.mark.parametrize("update", [0, 1])
+def test_manning_f32_vs_f64(ora, update):
+    """The f32 Manning restatement against the FP64 one from the same inputs:
+    h and zh within 1e-5 relative or 1e-4 m absolute."""
+    ny, nx = 48, 64
+    z = synth.dem(ny, nx, 2.2, 300.0, 77, n_pits=2, dx=10.0).ravel()
+    g = ora.Grid(nx, ny, 10.0, 0)
+    rcv, _ = ora.compute_receivers(g, z)
+    rng = np.random.default_rng(3)
+    h0 = rng.uniform(0, 5.0, nx * ny).astype(np.float32)
+    h0[:50] = 1e-4
+    Q0 = rng.uniform(0, 80.0, nx * ny).astype(np.float32)
+    hy = ora.hydro(manning=0.033, edge_slope=1e-2, dt=5e-3, update=update)
+    h32, zh32 = ora.manning(g, hy, h0, z, rcv, Q0)
+    h64, zh64 = ora.manning(g, hy, h0, z, rcv, Q0, f64=True)
+    for a, b in ((h32, h64), (zh32, zh64)):
+        e = np.abs(a.astype(np.float64) - b)
+        assert np.all((e <= 1e-4) | (e <= 1e-5 * np.abs(b)))
