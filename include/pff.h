@@ -96,6 +96,17 @@ int pff_fuse(pff_ctx* ctx, const pff_grid* g, float* A, const int32_t* src, cons
    nullable) is 1 unless the receivers hold a cycle. */
 int pff_accumulate_constant_Q(pff_ctx* ctx, const pff_grid* g, const int32_t* rcv, double value, int32_t area,
                               float* Q, int32_t* converged);
+/* No counterpart in the reference.  For callers that cannot allocate (code
+   under HIP graph capture, tight serving loops): pff_accumulate_constant_Q
+   with no context and no allocation.  Its scratch is one fixed device array
+   of 64 MiB inside the library (48 bytes per cell + 32 KiB per 64x64 tile,
+   e.g. a 1024x1024 grid); larger grids are rejected with PFF_EINVAL.  Work
+   is enqueued on hip_stream of the current device and nothing is read back:
+   *converged is a DEVICE int32, written 1 on a forest and 0 when the
+   receivers hold a cycle (Q is then not meaningful).  All calls on one
+   device share the array: calls that overlap on a device race. */
+int pff_accumulate_constant_Q_noalloc(void* hip_stream, const pff_grid* g, const int32_t* rcv, double value,
+                                      int32_t area, float* Q, int32_t* converged);
 /* Upstream cell counts (including self) as uint32; the integer core of
    accumulate_constant_Q. */
 int pff_upstream_count(pff_ctx* ctx, const pff_grid* g, const int32_t* rcv, uint32_t* count);

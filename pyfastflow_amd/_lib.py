@@ -100,6 +100,7 @@ SIGNATURES = {
     "pff_rake_compress_accum": [c_p, G, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_int],
     "pff_fuse": [c_p, G, c_p, c_p, c_p, c_int],
     "pff_accumulate_constant_Q": [c_p, G, c_p, c_d, c_int, c_p, ctypes.POINTER(c_int)],
+    "pff_accumulate_constant_Q_noalloc": [c_p, G, c_p, c_d, c_int, c_p, c_p],
     "pff_upstream_count": [c_p, G, c_p, c_p],
     "pff_depression_counter": [c_p, G, c_p, ctypes.POINTER(c_i64)],
     "pff_basin_id_init": [c_p, G, c_p],

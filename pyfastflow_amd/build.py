@@ -19,7 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpff_hip.so")
 OBJDIR = os.path.join(HERE, "_build")
 
-SOURCES = ["api.hip", "k_flow.hip", "k_forest.hip", "k_lake.hip", "k_gf.hip", "batch.hip", "k_conv.hip"]
+SOURCES = ["api.hip", "k_flow.hip", "k_forest.hip", "k_lake.hip", "k_gf.hip", "batch.hip", "k_conv.hip", "noalloc.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -44,6 +44,9 @@ int check_grid(const pff_grid* g, GridD* out) {
 int scratch(pff_ctx* ctx, int slot, size_t bytes, void** out, bool zero_new) {
   if (slot < 0 || slot >= S_NSLOTS) return fail(PFF_EINVAL, "scratch slot out of range");
   if (bytes == 0) bytes = 4;
+  if (ctx->cap[slot] < bytes && ctx->no_alloc)
+    return fail(PFF_EINVAL, "fixed scratch: slot " + std::to_string(slot) + " holds " +
+                                std::to_string(ctx->cap[slot]) + " bytes, " + std::to_string(bytes) + " needed");
   if (ctx->cap[slot] < bytes) {
     if (ctx->buf[slot]) {
       // the old buffer may still be read by queued kernels

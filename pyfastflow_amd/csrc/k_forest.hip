@@ -1193,6 +1193,12 @@ __global__ __launch_bounds__(kBT) void k_ring_inflow(TileArgs a, const int32_t* 
   }
 }
 
+// converged = 1 when the reduced rounds emptied their list and no tile saw
+// an in-tile cycle (the receivers are a forest), else 0
+__global__ void k_forest_status(const int64_t* live, const int64_t* cycle, int32_t* converged) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *converged = (*live == 0 && *cycle == 0) ? 1 : 0;
+}
+
 struct TileBufs {
   int32_t *E, *P2, *bid2, *exits, *L0, *L1, *t0, *t1, *t2, *pits;
   uint32_t *S, *inflow, *tact, *tin;
@@ -1217,6 +1223,29 @@ static int tile_bufs(pff_ctx* ctx, const GridD& g, TileBufs& b) {
   PFF_TRY(scratch(ctx, S_TACT, nt * 4 * 4, &p)); b.tact = (uint32_t*)p;
   b.tin = b.tact + 2 * nt;
   return 0;
+}
+
+size_t tiled_upstream_slots(const GridD& g, int* slots, size_t* bytes, int* nslots) {
+  const size_t n4 = (size_t)g.N * 4;
+  const size_t nt = (size_t)tiles_count(g);
+  const int s_n[] = {S_TE, S_TP2, S_TBID, S_TEXIT, S_WL0, S_WL1, S_I1, S_I2, S_I3, S_TS, S_TIN, S_TPIT};
+  int k = 0;
+  for (int sl : s_n) {
+    slots[k] = sl;
+    bytes[k++] = n4;
+  }
+  slots[k] = S_TACT;
+  bytes[k++] = nt * 4 * 4;
+  slots[k] = S_SEGD0;
+  bytes[k++] = nt * TC * 4;
+  slots[k] = S_SEGD1;
+  bytes[k++] = nt * TC * 4;
+  slots[k] = S_SEGCO;
+  bytes[k++] = nt * 4 * 4;
+  *nslots = k;
+  size_t total = 0;
+  for (int j = 0; j < k; ++j) total += (bytes[j] + 255) & ~(size_t)255;
+  return total;
 }
 
 static TileArgs tile_args(const GridD& g, const TileBufs& b, pff_ctx* ctx, const int32_t* rcv) {
@@ -1281,7 +1310,7 @@ void seg_join(pff_ctx* ctx, const GridD& g, const SegLists& sl, int nlists) {
 // graph that is not a forest (outputs are then not meaningful).
 int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, const int32_t* bid,
                    const int32_t* snode, float q0, float* Q, uint32_t* cnt, uint8_t* tag, const uint32_t* tact,
-                   int32_t* cycle, int rounds, int32_t* tag_list) {
+                   int32_t* cycle, int rounds, int32_t* tag_list, int32_t* converged_dev) {
   TileBufs b;
   PFF_TRY(tile_bufs(ctx, g, b));
   TileArgs a = tile_args(g, b, ctx, rcv);
@@ -1298,7 +1327,7 @@ int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, 
   a.tact = tact;
   hipStream_t s = ctx->stream;
   static_assert(W_NEXIT == W_CYCLE + 1, "cycle flag and exit count are adjacent words");
-  PFF_TRY(check_hip(hipMemsetAsync(ctx->dev_words + W_CYCLE, 0, 16, s), "memset"));
+  PFF_TRY(launch_fill_i64(ctx, 2, ctx->dev_words + W_CYCLE, 0));  // a kernel, not a runtime fill: capturable
   const dim3 tg = tile_grid(g);
   // algorithmic bytes per cell: rcv in; E, S, inflow out (+ bid, snode for saddle weights)
   a.prof_cells = prof_begin(ctx, "k_tile", wmode == W_UNIT ? 16.0 : 24.0);
@@ -1320,8 +1349,13 @@ int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, 
   hipLaunchKernelGGL(k_exit_link, dim3(list_grid()), dim3(kBT), 0, s, a, b.exits, a.n_exits, 0, b.L0, L.c[0], fe,
                      flags ? b.tin : nullptr, tiles_count(g));
   if (rounds > 0 && !cycle) {
-    // the caller knows the depth bound (the labels of the same graph)
+    // the caller knows the depth bound (the labels of the same graph), or
+    // runs the full ceil(log2 N)+1 rounds (rounds on an empty list return at
+    // once) and learns on the device whether the list emptied
     PFF_TRY(count_rounds(ctx, L, b.S, b.P2, b.t0, (uint32_t*)b.t1, b.t2, rounds, flags));
+    if (converged_dev)
+      hipLaunchKernelGGL(k_forest_status, dim3(1), dim3(64), 0, s, L.c[L.cur], ctx->dev_words + W_CYCLE,
+                         converged_dev);
   } else {
     // start from the round count the previous call of this kind needed,
     // then confirm on the host that the list has emptied

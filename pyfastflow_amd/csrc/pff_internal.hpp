@@ -159,7 +159,11 @@ int launch_saddlesort(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t*
 // tiled contraction (forests; *cycle = 1 when the receivers are not a forest)
 int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, const int32_t* bid,
                    const int32_t* snode, float q0, float* Q, uint32_t* cnt, uint8_t* tag, const uint32_t* tact,
-                   int32_t* cycle, int rounds = 0, int32_t* tag_list = nullptr);
+                   int32_t* cycle, int rounds = 0, int32_t* tag_list = nullptr,
+                   int32_t* converged_dev = nullptr);
+// scratch bytes of an unweighted tiled_upstream call, and the slots it takes
+// (slot, bytes) in the order a fixed arena lays them out
+size_t tiled_upstream_slots(const GridD& g, int* slots, size_t* bytes, int* nslots);
 // mark_cycles: the receivers may hold cycles; the cells on or draining into
 // one get the label -1 in tiled_labels_write (no host read; *ndep, *cycle and
 // *rounds_used are not written)
