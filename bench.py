@@ -42,10 +42,184 @@ def parse():
                    help="skip the ragged-batch line (256 tiles, sides 128-1024, routed in one launch sequence)")
     p.add_argument("--no-graphflood", action="store_true",
                    help="skip the GraphFlood line (2048^2, iterations of the reference schedule)")
+    p.add_argument("--steady-budget", type=int, default=10000,
+                   help="iterations of the 2048^2 GraphFlood run towards max |dh| < 1e-6 m (checked every 100)")
+    p.add_argument("--no-warm", action="store_true",
+                   help="skip the surrogate warm start against the cold start (2048^2, two solves)")
     p.add_argument("--pmc-calibrate", action="store_true",
                    help="after the steps, run a 1 GiB read + 1 GiB write elementwise pass 3 times "
                         "(scale for rocprofv3 FETCH/WRITE_SIZE)")
     return p.parse_args()
+
+
+def kernel_report(_lib, ctx, names):
+    """Per-kernel launches, total ms and algorithmic GB/s of the instrumented
+    launches since pff_ctx_profile(ctx, 1), minus the event pair's own cost."""
+    import ctypes
+
+    ovh = ctypes.c_double()
+    _lib.call("pff_ctx_launch_overhead", ctx, 50, ctypes.byref(ovh))
+    out = {}
+    for name in names:
+        n_l, t_ms, t_b = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
+        _lib.call("pff_ctx_profile_report", ctx, name.encode(), ctypes.byref(n_l), ctypes.byref(t_ms),
+                  ctypes.byref(t_b))
+        if n_l.value > 0:
+            t = max(t_ms.value - n_l.value * ovh.value * 1e-3, 1e-6)
+            gbs = t_b.value / (t * 1e-3) / 1e9
+            out[name] = {"launches": int(n_l.value), "avg_us": round(1e3 * t / n_l.value, 2),
+                         "bytes_per_launch": round(t_b.value / n_l.value), "achieved_GBs": round(gbs, 1),
+                         "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
+
+
+def fp64_drift(synth, iters=60, n=128):
+    """The f32 GraphFlood method against its FP64 restatement on a small
+    C3-style grid, cold start: error per result (max, p99, argmax) after
+    `iters` iterations of the reference schedule.  Run on the CPU by the
+    oracle; the GPU gives the same bits as the f32 oracle (checked by the
+    caller), so this is also the GPU's error."""
+    from oracle import pff_oracle as O
+
+    z = synth.dem(n, n, 2.2, 300.0, 303, n_pits=2, dx=10.0).ravel()
+    g = O.Grid(n, n, 10.0, 0)
+    hy = O.hydro(prec=1e-5, manning=0.033, edge_slope=1e-2, dt=5e-3, update=0)
+    zh, h, Q = z.copy(), np.zeros(z.size, np.float32), np.zeros(z.size, np.float32)
+    zh64, h64, Q64 = z.astype(np.float64), np.zeros(z.size), np.zeros(z.size)
+    for _ in range(iters):
+        zh, h, Q, _ = O.graphflood_iteration(g, hy, zh, h, Q)
+        zh64, h64, Q64, _ = O.graphflood_iteration(g, hy, zh64, h64, Q64, f64=True)
+    out = {}
+    for name, a, b in (("h", h, h64), ("zh", zh, zh64), ("Q", Q, Q64)):
+        e = np.abs(a.astype(np.float64) - b)
+        bad = (e > 1e-4) & (e > 1e-5 * np.abs(b))
+        out[name] = {"max_abs": float(e.max()), "p99_abs": float(np.quantile(e, 0.99)), "argmax_cell": int(np.argmax(e)),
+                     "cells_outside_1e-5rel_1e-4abs": int(bad.sum())}
+    return z, (zh, h, Q), out
+
+
+def gpu_drift(pf, synth, n_small, it_small):
+    z, (zh32, h32, Q32), drift = fp64_drift(synth, it_small, n_small)
+    rs = pf.flow.FlowRouter(n_small, n_small, 10.0, boundary_mode="normal")
+    rs.set_z(z)
+    fs = pf.graphflood.Flooder(rs, precipitation_rates=1e-5, manning=0.033, edge_slope=1e-2)
+    fs.run_graphflood(it_small)
+    drift["gpu_bit_exact_with_f32_oracle"] = bool(
+        np.array_equal(fs.h.to_numpy().view(np.uint32), h32.view(np.uint32))
+        and np.array_equal(rs.z.to_numpy().view(np.uint32), zh32.view(np.uint32))
+        and np.array_equal(rs.Q.to_numpy().view(np.uint32), Q32.view(np.uint32)))
+    drift["workload"] = (f"{n_small}x{n_small} beta 2.2, relief 300 m, 2 pits, seed 303, {it_small} iterations from "
+                         "h = Q = 0, reference update; f32 method vs its FP64 restatement")
+    return drift
+
+
+def graphflood_block(pf, _lib, synth, s, args):
+    import torch
+
+    cg = synth.graphflood_plain()
+    N = cg["nx"] * cg["ny"]
+    rg = pf.flow.FlowRouter(cg["nx"], cg["ny"], cg["dx"], boundary_mode="normal")
+    rg.set_z(cg["z"])
+    fl = pf.graphflood.Flooder(rg, precipitation_rates=cg["precipitation"], manning=cg["manning"], edge_slope=1e-2)
+    fl.run_graphflood(5)
+    torch.cuda.synchronize()
+    n_it = 200
+    eg = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    eg[0].record(s)
+    fl.run_graphflood(n_it)
+    eg[1].record(s)
+    torch.cuda.synchronize()
+    gms = eg[0].elapsed_time(eg[1]) / n_it
+    h = fl.get_h()
+    # the fused hydro kernel and the fill passes over 20 more iterations
+    ctx = _lib.ctx_for()
+    _lib.call("pff_ctx_profile", ctx, 1)
+    fl.run_graphflood(20)
+    kern = kernel_report(_lib, ctx, ("k_hydro", "k_topofill"))
+    _lib.call("pff_ctx_profile", ctx, 0)
+    del rg, fl
+    out = {"workload": "2048x2048 normal dx=10 m, beta=2.2, relief 300 m, 500 pits, seed 303; P=1e-5 m/s, "
+                       "n=0.033, edge_slope=1e-2, dt=5e-3 s, fill eps=1e-3, 5 diffusions; reference update",
+           "iterations_timed": n_it, "ms_per_iteration": round(gms, 4),
+           "Mcells_per_s": round(N / (gms * 1e-3) / 1e6, 1),
+           "h_mean_m": float(np.nanmean(h)), "h_nan_cells": int(np.isnan(h).sum()),
+           "kernels": {"k_hydro (fill apply + 5 diffusions + Manning, 32 B/cell)": kern.get("k_hydro"),
+                       "k_topofill (one fill pass, 28 B/cell)": kern.get("k_topofill")}}
+    # FP64 drift of the f32 method (oracle, CPU) and the GPU's bits against it
+    n_small, it_small = 128, 60
+    if args.no_cpu:
+        out["fp64_drift"] = None
+    else:
+        out["fp64_drift"] = gpu_drift(pf, synth, n_small, it_small)
+    # towards steady state: cold start (h = 0, Q = 0), reference update,
+    # max |h(t) - h(t-100)| < 1e-6 m checked every 100 iterations
+    rg = pf.flow.FlowRouter(cg["nx"], cg["ny"], cg["dx"], boundary_mode="normal")
+    rg.set_z(cg["z"])
+    fl = pf.graphflood.Flooder(rg, precipitation_rates=cg["precipitation"], manning=cg["manning"], edge_slope=1e-2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    its, median_it, nan_it, stop = 0, None, None, "budget"
+    while its < args.steady_budget:
+        st = fl.run_to_steady(tol=1e-6, check_every=100, max_iter=100, frac=0.0)
+        its += int(st.iterations)
+        if median_it is None and st.q50 < 1e-6:
+            median_it = its
+        if st.converged:
+            stop = "converged"
+            break
+        if st.nan_cells:
+            stop, nan_it = "NaN depth", its
+            break
+    torch.cuda.synchronize()
+    out["steady_state"] = {
+        "criterion": "max |h(t) - h(t-100)| < 1e-6 m, checked every 100 iterations (cold start h = 0, Q = 0)",
+        "reached": stop == "converged", "stop": stop, "iterations": its,
+        "seconds": round(time.perf_counter() - t0, 2),
+        "at_stop_m": {"max": st.residual, "mean": st.mean_change, "q50": st.q50, "q90": st.q90, "q99": st.q99,
+                      "frac_above_1e-6": st.above_frac},
+        "nan_iteration": nan_it,
+        "median_below_1e-6_at_iteration": median_it}
+    del rg, fl
+    torch.cuda.empty_cache()
+    return out
+
+
+def warm_start_block(pf, synth, args):
+    """Surrogate warm start against the cold start on a held-out 2048^2
+    config-2-style DEM (beta 2.0, relief 1000 m, 500 pits, seed 9091),
+    P = 1e-5 m/s, n = 0.033, single depth update; the reference update's
+    first NaN on the same DEM is measured beside it."""
+    import torch
+
+    from pyfastflow_amd.surrogate import UNet
+    from pyfastflow_amd.surrogate.warmstart import load_weights, solve
+
+    n = 2048
+    z = synth.dem(n, n, 2.0, 1000.0, 9091, n_pits=500, dx=10.0)
+    model = load_weights(UNet().cuda()).eval()
+    out = solve(z, 1e-5, 0.033, model, tol=1e-6, frac=0.5, max_iter=40000, update="single", log=log)
+    res = {"workload": "2048x2048 held-out DEM: beta 2.0, relief 1000 m, 500 pits, seed 9091; P=1e-5 m/s, n=0.033, "
+                       "single depth update; cold = h 0, warm = network prediction (inference timed in)",
+           "criterion": "at most 50% of the cells change by more than 1e-6 m over 100 iterations",
+           "cold": out["cold"], "warm": out["warm"], "used": out["used"], "agreement": out.get("agreement")}
+    if out["cold"]["converged"] and out["warm"]["converged"]:
+        res["warm_over_cold_seconds"] = round(out["warm"]["seconds"] / out["cold"]["seconds"], 3)
+        res["warm_over_cold_iterations"] = round(out["warm"]["iterations"] / out["cold"]["iterations"], 3)
+    # the reference's update on the same DEM, cold: first NaN depth
+    r = pf.flow.FlowRouter(n, n, 10.0, boundary_mode="normal")
+    r.set_z(z)
+    fl = pf.graphflood.Flooder(r, precipitation_rates=1e-5, manning=0.033, edge_slope=1e-2, update="reference")
+    its, nan_it = 0, None
+    while its < 2000:
+        st = fl.run_to_steady(tol=1e-6, check_every=100, max_iter=100, frac=0.0)
+        its += int(st.iterations)
+        if st.nan_cells:
+            nan_it = its
+            break
+    res["reference_update_first_nan_check"] = nan_it
+    del r, fl, model
+    torch.cuda.empty_cache()
+    return res
 
 
 def main():
@@ -211,31 +385,16 @@ def main():
 
     # ---- GraphFlood: 2048^2 open-edge DEM, cold start, run_graphflood
     #      iterations (receivers -> reroute_flow -> fill -> 5 diffusions ->
-    #      Manning) with the reference's depth update
+    #      Manning) with the reference's depth update; then the same DEM
+    #      iterated towards steady state, and the warm-start comparison
     gflood = None
+    warm = None
     if not args.no_graphflood:
-        cg = synth.graphflood_plain()
-        rg = pf.flow.FlowRouter(cg["nx"], cg["ny"], cg["dx"], boundary_mode="normal")
-        rg.set_z(cg["z"])
-        fl = pf.graphflood.Flooder(rg, precipitation_rates=cg["precipitation"], manning=cg["manning"],
-                                   edge_slope=1e-2)
-        fl.run_graphflood(5)
-        torch.cuda.synchronize()
-        n_it = 200
-        eg = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-        eg[0].record(s)
-        fl.run_graphflood(n_it)
-        eg[1].record(s)
-        torch.cuda.synchronize()
-        gms = eg[0].elapsed_time(eg[1]) / n_it
-        h = fl.get_h()
-        gflood = {"workload": "2048x2048 normal dx=10 m, beta=2.2, relief 300 m, 500 pits, seed 303; P=1e-5 m/s, "
-                              "n=0.033, edge_slope=1e-2, dt=5e-3 s, fill eps=1e-3, 5 diffusions; reference update",
-                  "iterations_timed": n_it, "ms_per_iteration": round(gms, 4),
-                  "Mcells_per_s": round(cg["nx"] * cg["ny"] / (gms * 1e-3) / 1e6, 1),
-                  "h_mean_m": float(np.nanmean(h)), "h_nan_cells": int(np.isnan(h).sum())}
+        gflood = graphflood_block(pf, _lib, synth, s, args)
         log(f"[bench] graphflood {gflood}")
-        del rg, fl
+        if not args.no_warm:
+            warm = warm_start_block(pf, synth, args)
+            log(f"[bench] warm start {warm}")
 
     # ---- CPU baseline (oracle, the reference algorithm restated in C) and
     #      full-size parity of the GPU result against it ----------------------
@@ -334,7 +493,8 @@ def main():
         "config": {"workload": f"depressions {n}x{n} periodic_EW dx=10 m, {pits} Gaussian pits, spectral beta=2.0 "
                                f"relief 1000 m seed 202; receivers + reroute_flow(carve) + accumulate_constant_Q(1e-5)",
                    "cells": N, "stages": stages, "lakeflow": lake, "parity": parity, "receivers_kernel": rcv_roof,
-                   "instrumented_kernels": kernel_times, "ragged_batch": batch, "graphflood": gflood},
+                   "instrumented_kernels": kernel_times, "ragged_batch": batch, "graphflood": gflood,
+                   "warm_start": warm},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -65,7 +65,8 @@ int pff_ctx_set_stream(pff_ctx* ctx, void* hip_stream);
 int pff_ctx_synchronize(pff_ctx* ctx);
 /* Opt-in per-kernel timing, no reference counterpart: while enabled, the
    instrumented launches ("k_tile", "k_tag_local", "k_tag_final",
-   "k_flag_scatter2", "k_zprime_min_t") record HIP events at the kernel's
+   "k_flag_scatter2", "k_zprime_min_t", "k_topofill", "k_hydro") record HIP
+   events at the kernel's
    start and end (hipExtLaunchKernelGGL) and count the algorithmic bytes of
    what they process.  Enabling clears the records; report() syncs and sums
    the launches of one kernel: count, time (ms) and algorithmic bytes.
@@ -80,6 +81,15 @@ int pff_ctx_profile_report(pff_ctx* ctx, const char* kernel, int64_t* launches, 
 /* ---- flow: receivers -------------------------------------------------- */
 /* receivers.py:15-58  compute_receivers(z, receivers, gradient) */
 int pff_compute_receivers(pff_ctx* ctx, const pff_grid* g, const float* z, int32_t* rcv, float* gradient);
+/* receivers.py:46-47  the same with RAND_RCV (FlowRouter(stochastic_receivers=True)):
+   each valid link's slope is multiplied by a uniform draw in [0, 1) before
+   the comparison, and gradient holds the weighted slope.  The draws are a
+   pure function of (seed, cell, direction): splitmix64 of
+   seed + (4*i + k) * 0x9E3779B97F4A7C15, top 24 bits / 2^24.  Taichi's
+   ti.random() is another generator, so only the distribution matches the
+   reference, not the draws. */
+int pff_compute_receivers_stochastic(pff_ctx* ctx, const pff_grid* g, const float* z, int32_t* rcv, float* gradient,
+                                     uint64_t seed);
 
 /* ---- flow: accumulation ------------------------------------------------ */
 /* downstream_propag.py:77-96  rcv2donor(rcv, dnr, ndnr); ndnr must be zeroed

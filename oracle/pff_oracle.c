@@ -231,6 +231,44 @@ void ora_compute_receivers(const ora_grid* g, const float* z, int32_t* rcv, floa
   }
 }
 
+/* Uniform draw in [0, 1) for cell i, direction k of one stochastic call:
+ * splitmix64 (Steele, Lea and Flood 2014) of seed + (4i+k) times the golden
+ * ratio constant, top 24 bits scaled by 2^-24 (exact in f32).  Stands in for
+ * ti.random() of receivers.py:46-47, whose generator is Taichi's own: the
+ * draws cannot match the reference's, only their distribution. */
+float ora_uniform01(uint64_t seed, int64_t i, int k) {
+  uint64_t x = seed + (uint64_t)(4 * i + k) * 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  x ^= x >> 31;
+  return (float)(x >> 40) * 0x1p-24f;
+}
+
+/* receivers.py:15-58 with RAND_RCV (:46-47): each valid link's slope is
+ * multiplied by a uniform draw before the strict > comparison; the stored
+ * gradient is the weighted slope, as in the reference. */
+void ora_compute_receivers_stochastic(const ora_grid* g, const float* z, int32_t* rcv, float* grad, uint64_t seed) {
+  const int64_t N = NN(g);
+  const float dx = g->dx;
+#pragma omp parallel for schedule(static)
+  for (int64_t ii = 0; ii < N; ++ii) {
+    const int32_t i = (int32_t)ii;
+    int32_t r = i;
+    float sr = 0.0f;
+    for (int k = 0; k < 4; ++k) {
+      const int32_t j = nb(g, i, k);
+      if (j == -1) continue; /* -1 * random() <= 0 never beats sr >= 0 */
+      const float t = ((z[i] - z[j]) / dx) * ora_uniform01(seed, i, k);
+      if (t > sr) {
+        sr = t;
+        r = j;
+      }
+    }
+    rcv[i] = r;
+    grad[i] = sr;
+  }
+}
+
 int64_t ora_receivers_div_ambiguity(const ora_grid* g, const float* z) {
   const int64_t N = NN(g);
   const float dx = g->dx;

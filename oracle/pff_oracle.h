@@ -48,6 +48,9 @@ void    ora_pack_array(int64_t n, const float* f, const int32_t* idx, int64_t* o
 void    ora_unpack_array(int64_t n, const int64_t* key, float* f, int32_t* idx);
 
 /* receivers */
+/* stochastic receivers (RAND_RCV): seeded uniform draws, see pff_oracle.c */
+float ora_uniform01(uint64_t seed, int64_t i, int k);
+void ora_compute_receivers_stochastic(const ora_grid* g, const float* z, int32_t* rcv, float* grad, uint64_t seed);
 void ora_compute_receivers(const ora_grid* g, const float* z, int32_t* rcv, float* grad);
 /* number of cells whose receiver choice would differ if (z_i - z_j)/dx were
    computed as (z_i - z_j) * fl(1/dx) */

@@ -73,6 +73,9 @@ def lib():
         _lib.ora_fill_z_add_delta_f64.argtypes = [c_p, c_p, c_p, c_p, c_d]
         _lib.ora_topofill.argtypes = [c_p, c_p, c_p, c_d]
         _lib.ora_pow53_array.argtypes = [ctypes.c_int64, c_p, c_p]
+        _lib.ora_compute_receivers_stochastic.argtypes = [c_p, c_p, c_p, c_p, ctypes.c_uint64]
+        _lib.ora_uniform01.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int]
+        _lib.ora_uniform01.restype = ctypes.c_float
     return _lib
 
 
@@ -144,6 +147,20 @@ def compute_receivers(g: Grid, z):
     grad = np.empty(g.N, np.float32)
     lib().ora_compute_receivers(g.ref, _p(z), _p(rcv), _p(grad))
     return rcv, grad
+
+
+def compute_receivers_stochastic(g: Grid, z, seed: int):
+    """receivers.py:15-58 with RAND_RCV: slopes weighted by seeded uniform
+    draws (splitmix64 of seed + (4i+k) * golden ratio, 24 bits)."""
+    z = _f32(z)
+    rcv = np.empty(g.N, np.int32)
+    grad = np.empty(g.N, np.float32)
+    lib().ora_compute_receivers_stochastic(g.ref, _p(z), _p(rcv), _p(grad), ctypes.c_uint64(seed))
+    return rcv, grad
+
+
+def uniform01(seed: int, i: int, k: int) -> float:
+    return float(lib().ora_uniform01(ctypes.c_uint64(seed), ctypes.c_int64(i), ctypes.c_int(k)))
 
 
 def receivers_div_ambiguity(g: Grid, z) -> int:

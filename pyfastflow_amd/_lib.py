@@ -96,6 +96,7 @@ SIGNATURES = {
     "pff_ctx_profile_report": [c_p, ctypes.c_char_p, ctypes.POINTER(c_i64), ctypes.POINTER(c_d),
                                ctypes.POINTER(c_d)],
     "pff_compute_receivers": [c_p, G, c_p, c_p, c_p],
+    "pff_compute_receivers_stochastic": [c_p, G, c_p, c_p, c_p, ctypes.c_uint64],
     "pff_rcv2donor": [c_p, G, c_p, c_p, c_p],
     "pff_rake_compress_accum": [c_p, G, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_int],
     "pff_fuse": [c_p, G, c_p, c_p, c_p, c_int],

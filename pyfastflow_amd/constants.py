@@ -21,6 +21,22 @@ DX = 1.0
 NX = 512
 NY = 512
 RAND_RCV = False
+# stochastic receivers draw from a seeded counter-based generator (not
+# Taichi's ti.random): call k of compute_receivers uses next_rand_seed()
+RAND_SEED = 0x5EED
+_rand_calls = 0
+
+
+def next_rand_seed() -> int:
+    """Seed of the next stochastic receivers call: splitmix64 of
+    RAND_SEED + call count, so every call draws fresh numbers and a run is
+    reproducible from RAND_SEED."""
+    global _rand_calls
+    x = (RAND_SEED + _rand_calls * 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+    _rand_calls += 1
+    x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return x ^ (x >> 31)
 # 0 normal, 1 periodic_EW, 2 periodic_NS, 3 custom (codes 0/1/3/7/9)
 BOUND_MODE = 0
 

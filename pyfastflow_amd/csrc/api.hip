@@ -491,6 +491,15 @@ int pff_upstream_count(pff_ctx* ctx, const pff_grid* g, const int32_t* rcv, uint
   return 0;
 }
 
+int pff_compute_receivers_stochastic(pff_ctx* ctx, const pff_grid* g, const float* z, int32_t* rcv, float* gradient,
+                                     uint64_t seed) {
+  PFF_TRY(begin(ctx));
+  GridD d;
+  PFF_TRY(check_grid(g, &d));
+  if (!z || !rcv || !gradient) return fail(PFF_EINVAL, "compute_receivers_stochastic: NULL array");
+  return launch_receivers_stochastic(ctx, d, z, rcv, gradient, seed);
+}
+
 int pff_accumulate_constant_Q(pff_ctx* ctx, const pff_grid* g, const int32_t* rcv, double value, int32_t area,
                               float* Q, int32_t* converged) {
   PFF_TRY(begin(ctx));
@@ -728,21 +737,41 @@ int gf_fields(pff_ctx* ctx, const GridD& d, GfFields& f) {
   return 0;
 }
 
-// one Flooder iteration (gf_fields.py:126-141)
-int gf_iteration(pff_ctx* ctx, const pff_grid* g, const GridD& d, const pff_hydro* p, float* zh, float* h, float* Q,
-                 int32_t* rec, int32_t n_diffuse, double eps, GfFields& f) {
+// one Flooder iteration (gf_fields.py:126-141).  Q ping-pongs between the
+// caller's array and f.Qt: *qcur holds it on entry and on return.
+int gf_iteration(pff_ctx* ctx, const pff_grid* g, const GridD& d, const pff_hydro* p, float* zh, float* h,
+                 float** qcur, float* Q, int32_t* rec, int32_t n_diffuse, double eps, GfFields& f) {
   PFF_TRY(launch_receivers(ctx, d, zh, rec, f.zp));  // the gradient output is scratch here
   PFF_TRY(pff_reroute_flow(ctx, g, f.bid, rec, f.rec_, f.rec__, zh, f.zp, f.isb, f.outlet, f.saddle, f.snode, f.tag,
                            f.tag_, 1, nullptr, nullptr));
+  float* qin = *qcur;
+  float* qout = qin == Q ? f.Qt : Q;
+  if (d.mode == 0 && n_diffuse >= 1 && n_diffuse <= 5) {
+    // fill_z_add_delta's passes (fill_topo.py:116-122), then its apply, the
+    // diffusion and Manning fused in one kernel
+    const size_t n4 = (size_t)d.N * 4;
+    PFF_TRY(check_hip(hipMemcpyAsync(f.zp, zh, n4, hipMemcpyDeviceToDevice, ctx->stream), "copy"));
+    PFF_TRY(check_hip(hipMemcpyAsync(f.rec_, rec, n4, hipMemcpyDeviceToDevice, ctx->stream), "copy"));
+    PFF_TRY(fill_passes(ctx, d, zh, f.zp, f.rec_, f.rec__, eps, -1));
+    PFF_TRY(launch_hydro_fused(ctx, d, p, n_diffuse, f.zp, zh, h, qin, qout, rec));
+    *qcur = qout;
+    return 0;
+  }
   PFF_TRY(pff_fill_z_add_delta(ctx, g, zh, h, f.zp, rec, f.rec_, f.rec__, eps));
-  float* a = Q;
-  float* b = f.Qt;
+  float* a = qin;
+  float* b = qout;
   for (int k = 0; k < n_diffuse; ++k) {
     PFF_TRY(launch_diffuse(ctx, d, p, zh, a, b));
     std::swap(a, b);
   }
-  if (a != Q) PFF_TRY(check_hip(hipMemcpyAsync(Q, a, (size_t)d.N * 4, hipMemcpyDeviceToDevice, ctx->stream), "copy"));
-  return launch_manning(ctx, d, p, h, zh, f.dh, rec, Q);
+  *qcur = a;
+  return launch_manning(ctx, d, p, h, zh, f.dh, rec, a);
+}
+
+// the caller's Q array receives the iteration's Q if it is in f.Qt
+int gf_q_home(pff_ctx* ctx, const GridD& d, float* qcur, float* Q) {
+  if (qcur == Q) return 0;
+  return check_hip(hipMemcpyAsync(Q, qcur, (size_t)d.N * 4, hipMemcpyDeviceToDevice, ctx->stream), "copy");
 }
 
 // upper edge of the histogram bin holding the q-quantile of n values
@@ -801,7 +830,9 @@ int pff_graphflood_run(pff_ctx* ctx, const pff_grid* g, const pff_hydro* p, floa
   if (n_iter < 0 || n_diffuse < 0) return fail(PFF_EINVAL, "graphflood_run: negative count");
   GfFields f;
   PFF_TRY(gf_fields(ctx, d, f));
-  for (int it = 0; it < n_iter; ++it) PFF_TRY(gf_iteration(ctx, g, d, p, zh, h, Q, rec, n_diffuse, fill_eps, f));
+  float* qc = Q;
+  for (int it = 0; it < n_iter; ++it) PFF_TRY(gf_iteration(ctx, g, d, p, zh, h, &qc, Q, rec, n_diffuse, fill_eps, f));
+  PFF_TRY(gf_q_home(ctx, d, qc, Q));
   if (st) {
     std::memset(st, 0, sizeof(*st));
     st->iterations = n_iter;
@@ -830,9 +861,11 @@ int pff_graphflood_to_steady(pff_ctx* ctx, const pff_grid* g, const pff_hydro* p
   s.first_mean_change = -1.0;
   const size_t n4 = (size_t)d.N * 4;
   PFF_TRY(check_hip(hipMemcpyAsync(f.hprev, h, n4, hipMemcpyDeviceToDevice, ctx->stream), "copy"));
+  float* qc = Q;
   while (s.iterations < max_iter) {
     const int64_t n = std::min<int64_t>(check_every, max_iter - s.iterations);
-    for (int64_t k = 0; k < n; ++k) PFF_TRY(gf_iteration(ctx, g, d, p, zh, h, Q, rec, n_diffuse, fill_eps, f));
+    for (int64_t k = 0; k < n; ++k)
+      PFF_TRY(gf_iteration(ctx, g, d, p, zh, h, &qc, Q, rec, n_diffuse, fill_eps, f));
     s.iterations += n;
     if (n < check_every) break;  // a partial block cannot be judged
     double res = 0;
@@ -863,6 +896,7 @@ int pff_graphflood_to_steady(pff_ctx* ctx, const pff_grid* g, const pff_hydro* p
     }
     PFF_TRY(check_hip(hipMemcpyAsync(f.hprev, h, n4, hipMemcpyDeviceToDevice, ctx->stream), "copy"));
   }
+  PFF_TRY(gf_q_home(ctx, d, qc, Q));
   if (st) *st = s;
   return 0;
 }

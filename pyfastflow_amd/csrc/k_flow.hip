@@ -12,9 +12,21 @@ namespace pff {
 // (strict >), invalid links never win, and the slope is the IEEE f32 quotient
 // (z_i - z_j) / dx, so receivers match the reference bit for bit.
 // ---------------------------------------------------------------------------
-template <int MODE>
+// Uniform draw in [0, 1) for cell i, direction k of one stochastic call
+// (RAND_RCV, receivers.py:46-47): splitmix64 (Steele, Lea and Flood 2014) of
+// seed + (4i+k) times the golden-ratio constant, top 24 bits times 2^-24
+// (exact in f32).  The oracle's ora_uniform01 is the same function.
+__device__ __forceinline__ float uniform01(uint64_t seed, int i, int k) {
+  uint64_t x = seed + (uint64_t)(4 * (int64_t)i + k) * 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  x ^= x >> 31;
+  return (float)(x >> 40) * 0x1p-24f;
+}
+
+template <int MODE, bool STOCH = false>
 __global__ __launch_bounds__(256) void k_receivers(GridD g, const float* __restrict__ z, int32_t* __restrict__ rcv,
-                                                   float* __restrict__ grad) {
+                                                   float* __restrict__ grad, uint64_t seed = 0) {
   int bx, by;
   xcd_block(bx, by);
   const int col = bx * 64 + threadIdx.x;
@@ -28,7 +40,8 @@ __global__ __launch_bounds__(256) void k_receivers(GridD g, const float* __restr
   for (int k = 0; k < 4; ++k) {
     const int j = neighbour<MODE>(g, i, row, col, k);
     if (j >= 0) {
-      const float t = (zi - z[j]) / g.dx;
+      float t = (zi - z[j]) / g.dx;
+      if constexpr (STOCH) t = t * uniform01(seed, i, k);  // the weighted slope is also the stored gradient
       if (t > sr) {
         sr = t;
         r = j;
@@ -42,8 +55,17 @@ __global__ __launch_bounds__(256) void k_receivers(GridD g, const float* __restr
 int launch_receivers(pff_ctx* ctx, const GridD& g, const float* z, int32_t* rcv, float* grad) {
   dim3 block(64, 4);
   dim3 grid((g.nx + 63) / 64, (g.ny + 3) / 4);
-  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_receivers<M>, grid, block, 0, ctx->stream, g, z, rcv, grad));
+  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_receivers<M>, grid, block, 0, ctx->stream, g, z, rcv, grad, 0ull));
   return check_hip(hipGetLastError(), "receivers");
+}
+
+int launch_receivers_stochastic(pff_ctx* ctx, const GridD& g, const float* z, int32_t* rcv, float* grad,
+                                uint64_t seed) {
+  dim3 block(64, 4);
+  dim3 grid((g.nx + 63) / 64, (g.ny + 3) / 4);
+  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL((k_receivers<M, true>), grid, block, 0, ctx->stream, g, z, rcv, grad,
+                                               seed));
+  return check_hip(hipGetLastError(), "receivers (stochastic)");
 }
 
 // ---------------------------------------------------------------------------

@@ -138,6 +138,8 @@ inline unsigned stride_grid() { return 2048; }
 
 // ---- launchers (defined in k_*.hip) -----------------------------------------
 int launch_receivers(pff_ctx* ctx, const GridD& g, const float* z, int32_t* rcv, float* grad);
+int launch_receivers_stochastic(pff_ctx* ctx, const GridD& g, const float* z, int32_t* rcv, float* grad,
+                                uint64_t seed);
 
 int launch_rcv2donor(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int32_t* dnr, int32_t* ndnr);
 int launch_rake_compress(pff_ctx* ctx, const GridD& g, int32_t* dnr, int32_t* ndnr, float* p, int32_t* src,
@@ -250,6 +252,11 @@ int launch_apply_fill(pff_ctx* ctx, const GridD& g, float* zh, float* h, const f
 int launch_diffuse(pff_ctx* ctx, const GridD& g, const pff_hydro* p, const float* zh, const float* Q, float* temp);
 int launch_manning(pff_ctx* ctx, const GridD& g, const pff_hydro* p, float* h, float* zh, float* dh,
                    const int32_t* rec, const float* Q);
+// fill apply + n_diffuse diffusion passes + Manning in one kernel ('normal'
+// grids, 1..5 passes): zf = filled surface, zh = surface before the fill
+// (overwritten with the new surface), Qin -> Qout (distinct buffers)
+int launch_hydro_fused(pff_ctx* ctx, const GridD& g, const pff_hydro* p, int n_diffuse, const float* zf, float* zh,
+                       float* h, const float* Qin, float* Qout, const int32_t* rec);
 
 // ---- drivers shared by the entry points (api.hip) ----------------------------
 struct BatchState;

@@ -32,9 +32,7 @@ class FlowRouter:
     """
 
     def __init__(self, nx, ny, dx, boundary_mode="normal", boundaries=None, lakeflow=True,
-                 stochastic_receivers=False, device=None):
-        if stochastic_receivers:
-            raise NotImplementedError("stochastic receivers are not implemented")
+                 stochastic_receivers=False, device=None, seed=None):
         self.nx = int(nx)
         self.ny = int(ny)
         self.dx = float(dx)
@@ -87,7 +85,9 @@ class FlowRouter:
         self.last_lake_status = None
         # globals for the module-level functions (flowfields.py:163-179)
         cte.NX, cte.NY, cte.DX = self.nx, self.ny, self.dx
-        cte.RAND_RCV = False
+        cte.RAND_RCV = bool(stochastic_receivers)
+        if seed is not None:
+            cte.RAND_SEED = int(seed)
         cte.BOUND_MODE = self.mode
         cte.boundaries = self._boundaries_dev
         cte.INITIALISED = True
@@ -117,7 +117,14 @@ class FlowRouter:
 
     # ---- pipeline ------------------------------------------------------------
     def compute_receivers(self):
-        """D4 steepest-descent receivers and gradient (flowfields.py:216-226)."""
+        """D4 steepest-descent receivers and gradient (flowfields.py:216-226);
+        with stochastic_receivers, slopes weighted by seeded uniform draws
+        (receivers.py:46-47, see flow.compute_receivers), a fresh seed per
+        call from constants.next_rand_seed()."""
+        if self.stochastic_receivers:
+            _lib.call("pff_compute_receivers_stochastic", self._ctx(), self._grid(), _lib.ptr(self.z),
+                      _lib.ptr(self.receivers), _lib.ptr(self.gradient), ctypes.c_uint64(cte.next_rand_seed()))
+            return
         _lib.call("pff_compute_receivers", self._ctx(), self._grid(), _lib.ptr(self.z), _lib.ptr(self.receivers),
                   _lib.ptr(self.gradient))
 

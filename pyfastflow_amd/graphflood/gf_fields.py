@@ -74,8 +74,13 @@ class Flooder:
 
     def run_graphflood(self, N=10, n_diffuse=5):
         """N GraphFlood iterations (gf_fields.py:111-141), looped inside the
-        HIP library."""
+        HIP library (on a router with stochastic receivers, through the
+        module-level calls, so that each iteration draws its receivers)."""
         r = self.router
+        if r.stochastic_receivers:
+            for _ in range(int(N)):
+                self.step_modules(n_diffuse)
+            return None
         st = _lib.PffGfStatus()
         _lib.call("pff_graphflood_run", r._ctx(), r._grid(), self._hydro(), _lib.ptr(r.z), _lib.ptr(self.h),
                   _lib.ptr(r.Q), _lib.ptr(r.receivers), int(N), int(n_diffuse), 1e-3, ctypes.byref(st))

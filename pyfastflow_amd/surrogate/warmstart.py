@@ -181,6 +181,7 @@ def solve(z: np.ndarray, rain: float, manning: float, model: UNet | None, tol: f
     both."""
     ny, nx = z.shape
     out = {}
+    finals = {}
     for mode in (["cold", "warm"] if model is not None else ["cold"]):
         r = FlowRouter(nx, ny, 10.0, boundary_mode="normal")
         r.set_z(z)
@@ -209,9 +210,19 @@ def solve(z: np.ndarray, rain: float, manning: float, model: UNet | None, tol: f
                      "residual": float(st.residual), "first_residual": first,
                      "above_frac": float(st.above_frac), "first_above_frac": first_frac,
                      "mean_change": float(st.mean_change), "first_mean_change": first_mean,
+                     "q50": float(st.q50), "q90": float(st.q90), "q99": float(st.q99),
                      "converged": bool(st.converged), "nan_cells": int(st.nan_cells)}
         log(f"[surrogate] {mode}: {out[mode]}")
+        finals[mode] = fl.h.data.clone()
         del fl, r
+    if "warm" in finals:
+        # where the two end states agree: |h_warm - h_cold| per cell
+        d = (finals["warm"] - finals["cold"]).abs().float()
+        q = torch.quantile(d[::7][:1 << 24], torch.tensor([0.5, 0.9, 0.99], device=d.device)).tolist()
+        out["agreement"] = {"max_m": float(d.max()), "median_m": q[0], "p90_m": q[1], "p99_m": q[2],
+                            "frac_within_1e-3_m": float((d <= 1e-3).float().mean()),
+                            "frac_within_1e-2_m": float((d <= 1e-2).float().mean())}
+        log(f"[surrogate] warm vs cold end states: {out['agreement']}")
     # the initial residual: mean change of h over the first check block
     key = "first_mean_change"
     if "warm" in out and out["warm"][key] > out["cold"][key]:

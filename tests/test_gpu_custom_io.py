@@ -1,0 +1,196 @@
+"""GPU parity of the remaining call surface against the oracle:
+
+- custom boundary codes (mode 3, constants.py:39-83) through the whole
+  routing path (receivers -> reroute_flow -> accumulate_constant_Q) and
+  through GraphFlood, including a pit on the last cell whose basin id is N
+  (lakeflow.py:61, quirk Q20);
+- stochastic receivers (RAND_RCV, receivers.py:46-47) with seeded draws;
+- the (ny, nx) row-major f32 copies of get_Q / get_Z / get_h
+  (flowfields.py:298-319, gf_fields.py:152-159).
+
+Integer outputs bit-exact; Q equal to f32(q0) * exact upstream count; the
+GraphFlood state bit-exact with the f32 oracle.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from pyfastflow_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _codes(ny, nx, kind):
+    b = np.ones((ny, nx), np.uint8)
+    if kind == "mixed":
+        # outlets north and south, periodic east-west, a no-data block, inlets
+        b[0, :] = 3
+        b[-1, :] = 3
+        b[:, 0] = 9
+        b[:, -1] = 9
+        b[ny // 3:ny // 3 + 5, nx // 4:nx // 4 + 6] = 0
+        b[ny // 2, 3] = 7
+        b[2 * ny // 3, nx // 2] = 7
+    elif kind == "north_outlet":
+        # only the north row drains out: every pit is closed, and the last
+        # cell (south-east corner) is an interior cell
+        b[0, :] = 3
+        b[ny // 2:ny // 2 + 4, 7:12] = 0
+    return b
+
+
+def _pit_last_cell(z):
+    """Make the last cell a local minimum (a closed pit with basin id N)."""
+    z = z.copy()
+    z[-1, -1] = min(z[-2, -1], z[-1, -2]) - 5.0
+    return z
+
+
+@pytest.mark.parametrize("kind,ny,nx,seed,pits", [("mixed", 96, 80, 41, 6), ("north_outlet", 64, 72, 42, 4),
+                                                  ("mixed", 200, 264, 43, 25)])
+def test_custom_routing_pipeline(pf, ora, kind, ny, nx, seed, pits):
+    print(f"case custom routing {kind} {ny}x{nx}", flush=True)
+    z = synth.dem(ny, nx, 2.0, 200.0, seed, n_pits=pits, dx=10.0)
+    if kind == "north_outlet":
+        z = _pit_last_cell(z)
+    b = _codes(ny, nx, kind)
+    r = pf.flow.FlowRouter(nx, ny, 10.0, boundary_mode="custom", boundaries=b)
+    r.set_z(z)
+    r.compute_receivers()
+    g = ora.Grid(nx, ny, 10.0, "custom", b)
+    rcv, _ = ora.compute_receivers(g, z)
+    np.testing.assert_array_equal(r.receivers.to_numpy(), rcv)
+    if kind == "north_outlet":
+        assert rcv[-1] == ny * nx - 1  # the last cell is a pit: its basin id is N
+    bf = torch.empty(nx * ny, dtype=torch.int32, device="cuda")
+    r.reroute_flow(carve=True, bid_first=bf)
+    lk = ora.reroute_flow(g, z, rcv, carve=True)
+    np.testing.assert_array_equal(bf.cpu().numpy(), lk["bid_first"])
+    if kind == "north_outlet":
+        assert lk["bid_first"][-1] == ny * nx
+    np.testing.assert_array_equal(r.receivers.to_numpy(), lk["rec"])
+    assert r.accumulate_constant_Q(1e-5, area=True)
+    cnt = ora.upstream_count(g, lk["rec"])
+    q0 = np.float32(1e-5 * 10.0 * 10.0)
+    np.testing.assert_array_equal(r.Q.to_numpy(), q0 * cnt.astype(np.float32))
+
+
+@pytest.mark.parametrize("kind", ["north_outlet", "outlets_no_wrap"])
+def test_custom_graphflood_bit_exact(pf, ora, kind):
+    """20 GraphFlood iterations in custom mode from a cold start, bit-exact
+    with the f32 oracle (codes 0/1/3/7: every cell's sources are gathered in
+    ascending index order, the oracle's scatter order)."""
+    ny, nx = 64, 80
+    z = synth.dem(ny, nx, 2.2, 150.0, 44, n_pits=3, dx=10.0)
+    if kind == "north_outlet":
+        b = _codes(ny, nx, "north_outlet")
+        z = _pit_last_cell(z)
+    else:
+        b = _codes(ny, nx, "mixed")
+        b[:, 0] = 3
+        b[:, -1] = 3
+    print(f"case custom graphflood {kind}", flush=True)
+    r = pf.flow.FlowRouter(nx, ny, 10.0, boundary_mode="custom", boundaries=b)
+    r.set_z(z)
+    fl = pf.graphflood.Flooder(r, precipitation_rates=1e-5, manning=0.033, edge_slope=1e-2)
+    g = ora.Grid(nx, ny, 10.0, "custom", b)
+    hy = ora.hydro(prec=1e-5, manning=0.033, edge_slope=1e-2, dt=5e-3, update=0)
+    zh, h, Q = z.ravel().copy(), np.zeros(nx * ny, np.float32), np.zeros(nx * ny, np.float32)
+    for blk in range(2):
+        fl.run_graphflood(10)
+        for _ in range(10):
+            zh, h, Q, rec = ora.graphflood_iteration(g, hy, zh, h, Q)
+        np.testing.assert_array_equal(r.receivers.to_numpy(), rec)
+        np.testing.assert_array_equal(r.z.to_numpy().view(np.uint32), zh.view(np.uint32))
+        np.testing.assert_array_equal(fl.h.to_numpy().view(np.uint32), h.view(np.uint32))
+        np.testing.assert_array_equal(r.Q.to_numpy().view(np.uint32), Q.view(np.uint32))
+
+
+@pytest.mark.parametrize("mode,seed", [("normal", 1), ("periodic_EW", 99), ("periodic_NS", 2**63 + 5)])
+def test_stochastic_receivers_bit_exact(pf, ora, mode, seed):
+    ny, nx = 120, 136
+    z = synth.dem(ny, nx, 2.0, 100.0, 45, n_pits=4, dx=10.0, periodic_ew=(mode == "periodic_EW"))
+    pf.flow.FlowRouter(nx, ny, 10.0, boundary_mode=mode, stochastic_receivers=True)
+    tz = torch.from_numpy(z.ravel().astype(np.float32)).cuda()
+    rcv = torch.empty(nx * ny, dtype=torch.int32, device="cuda")
+    grad = torch.empty(nx * ny, dtype=torch.float32, device="cuda")
+    pf.flow.compute_receivers(tz, rcv, grad, seed=seed)
+    g = ora.Grid(nx, ny, 10.0, mode)
+    orc, ogr = ora.compute_receivers_stochastic(g, z, seed)
+    np.testing.assert_array_equal(rcv.cpu().numpy(), orc)
+    np.testing.assert_array_equal(grad.cpu().numpy().view(np.uint32), ogr.view(np.uint32))
+
+
+def test_stochastic_router_draws_and_reproduces(pf, ora):
+    """A stochastic router draws fresh receivers on every call; the same
+    RAND_SEED replays the same sequence; the draws follow the oracle's
+    seeds (constants.next_rand_seed); lakeflow and GraphFlood run on them."""
+    from pyfastflow_amd import constants as cte
+
+    ny, nx = 64, 64
+    z = synth.dem(ny, nx, 2.0, 100.0, 46, n_pits=3, dx=10.0)
+
+    def run():
+        r = pf.flow.FlowRouter(nx, ny, 10.0, stochastic_receivers=True, seed=777)
+        cte._rand_calls = 0
+        r.set_z(z)
+        out = []
+        for _ in range(3):
+            r.compute_receivers()
+            out.append(r.receivers.to_numpy().copy())
+        return r, out
+
+    r, a = run()
+    _, b = run()
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+    assert (a[0] != a[1]).any() and (a[1] != a[2]).any()
+    cte.RAND_SEED, cte._rand_calls = 777, 0
+    g = ora.Grid(nx, ny, 10.0, 0)
+    for k in range(3):
+        orc, _ = ora.compute_receivers_stochastic(g, z, cte.next_rand_seed())
+        np.testing.assert_array_equal(a[k], orc)
+    r.reroute_flow(carve=True)
+    assert r.accumulate_constant_Q(1.0, area=True)
+    fl = pf.graphflood.Flooder(r, precipitation_rates=1e-5)
+    fl.run_graphflood(3)
+    assert np.isfinite(fl.get_h()).all()
+
+
+def test_get_Q_Z_h_layout(pf, ora):
+    """get_Z, get_Q and get_h return (ny, nx) row-major f32 copies
+    (flowfields.py:298-319, gf_fields.py:152-159) on a non-square grid, equal
+    to the oracle's arrays reshaped; writing to them leaves the device fields
+    alone."""
+    ny, nx = 48, 72
+    z = synth.dem(ny, nx, 2.2, 100.0, 47, n_pits=2, dx=10.0)
+    r = pf.flow.FlowRouter(nx, ny, 10.0, boundary_mode="normal")
+    r.set_z(z)
+    Z = r.get_Z()
+    assert Z.shape == (ny, nx) and Z.dtype == np.float32 and Z.flags["C_CONTIGUOUS"]
+    np.testing.assert_array_equal(Z, z.astype(np.float32))
+    r.compute_receivers()
+    r.reroute_flow(carve=True)
+    assert r.accumulate_constant_Q(1.0, area=True)
+    g = ora.Grid(nx, ny, 10.0, 0)
+    rcv, _ = ora.compute_receivers(g, z)
+    lk = ora.reroute_flow(g, z, rcv, carve=True)
+    Qg = r.get_Q()
+    assert Qg.shape == (ny, nx) and Qg.dtype == np.float32
+    cnt = ora.upstream_count(g, lk["rec"]).reshape(ny, nx)
+    np.testing.assert_array_equal(Qg, np.float32(100.0) * cnt.astype(np.float32))
+    Qg[:] = -1.0
+    assert (r.get_Q() >= 100.0).all()
+    fl = pf.graphflood.Flooder(r, precipitation_rates=1e-5)
+    r.set_z(z)
+    r.Q.fill(0.0)
+    fl.run_graphflood(4)
+    hy = ora.hydro(prec=1e-5, manning=0.033, edge_slope=1e-2, dt=5e-3, update=0)
+    zh, h, Q = z.ravel().copy(), np.zeros(nx * ny, np.float32), np.zeros(nx * ny, np.float32)
+    for _ in range(4):
+        zh, h, Q, _ = ora.graphflood_iteration(g, hy, zh, h, Q)
+    H = fl.get_h()
+    assert H.shape == (ny, nx) and H.dtype == np.float32
+    np.testing.assert_array_equal(H, h.reshape(ny, nx))
+    np.testing.assert_array_equal(r.get_Z(), zh.reshape(ny, nx))

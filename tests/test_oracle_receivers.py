@@ -63,3 +63,59 @@ def test_division_ambiguity_is_counted(ora):
     different receivers; for the 512^2 plain workload (dx=1) there are none."""
     c = synth.plain_small(n=128)
     assert ora.receivers_div_ambiguity(ora.Grid(128, 128, 1.0, 0), c["z"]) == 0
+
+
+def test_stochastic_draws_are_uniform(ora):
+    """The seeded draws of RAND_RCV: mean 1/2, variance 1/12, and the
+    fraction below 1/4 is 1/4, over 40 000 draws (5 sigma bounds)."""
+    u = np.array([ora.uniform01(12345, i, k) for i in range(10_000) for k in range(4)])
+    assert (u >= 0).all() and (u < 1).all()
+    n = u.size
+    assert abs(u.mean() - 0.5) < 5 * np.sqrt(1 / 12 / n)
+    assert abs(u.var() - 1 / 12) < 5 * np.sqrt(1 / 180 / n)
+    assert abs((u < 0.25).mean() - 0.25) < 5 * np.sqrt(0.25 * 0.75 / n)
+
+
+def test_stochastic_receivers_two_slope_probability(ora):
+    """On the plane z = -(2 row + col) every interior cell has two lower
+    neighbours, bottom (slope 2/dx) and right (1/dx).  With the slopes
+    weighted by independent uniform draws, bottom wins with probability
+    P(2 u1 > u2) = 1 - 1/4 = 3/4.  The weighted slope is stored as gradient:
+    max(2 u1, u2)/dx, with mean 13/12 / dx and variance 35/144 / dx^2."""
+    ny, nx, dx = 300, 400, 2.0
+    r, c = np.mgrid[0:ny, 0:nx]
+    z = -(2.0 * r + 1.0 * c).astype(np.float32)
+    g = ora.Grid(nx, ny, dx, 0)
+    rcv, grad = ora.compute_receivers_stochastic(g, z, seed=7)
+    inner = (r < ny - 1) & (c < nx - 1)
+    i = np.arange(nx * ny).reshape(ny, nx)
+    down = rcv.reshape(ny, nx)[inner] == (i + nx)[inner]
+    right = rcv.reshape(ny, nx)[inner] == (i + 1)[inner]
+    assert (down | right).all()
+    p = down.mean()
+    n = down.size
+    assert abs(p - 0.75) < 5 * np.sqrt(0.75 * 0.25 / n)
+    gi = grad.reshape(ny, nx)[inner]
+    assert (gi > 0).all() and (gi < 2.0 / dx).all()
+    assert abs(gi.mean() * dx - 13 / 12) < 5 * np.sqrt(35 / 144 / n)
+
+
+def test_stochastic_receivers_reproducible_and_downhill(ora):
+    """Same seed, same receivers; another seed, other receivers; every
+    receiver is the cell itself or a strictly lower valid neighbour, and a
+    cell with no lower neighbour keeps itself."""
+    from pyfastflow_amd import synth
+
+    z = synth.dem(64, 80, 2.0, 100.0, 3, n_pits=2, dx=10.0)
+    g = ora.Grid(80, 64, 10.0, 1)
+    a, ga = ora.compute_receivers_stochastic(g, z, seed=1)
+    b, gb = ora.compute_receivers_stochastic(g, z, seed=1)
+    c, _ = ora.compute_receivers_stochastic(g, z, seed=2)
+    np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(ga, gb)
+    assert (a != c).any()
+    det, _ = ora.compute_receivers(g, z)
+    zf = z.ravel()
+    moved = a != np.arange(a.size)
+    assert (zf[a[moved]] < zf[moved]).all()
+    np.testing.assert_array_equal(a[det == np.arange(a.size)], np.flatnonzero(det == np.arange(a.size)))
