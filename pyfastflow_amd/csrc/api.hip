@@ -210,6 +210,8 @@ int reroute_core(pff_ctx* ctx, const GridD& d, int32_t* bid, int32_t* rec, int32
   // held a closed basin after trip t-1 (all tiles in the first trip)
   uint32_t *flag_a = nullptr, *flag_b = nullptr;
   PFF_TRY(tile_flags(ctx, d, &flag_a, &flag_b));
+  // new receivers: no cached ring walk is valid
+  PFF_TRY(ring_cache_reset(ctx, d));
   const uint32_t* tact = nullptr;
   int label_rounds = 0;
   int nouter = 1;

@@ -6,12 +6,17 @@ namespace pff {
 
 // ---------------------------------------------------------------------------
 // D4 steepest-descent receivers (receivers.py:15-58).
-// 2-D launch, one lane per cell, 64 consecutive columns per wavefront so each
-// row access is one coalesced 256 B line; the vertical neighbours are rows the
-// adjacent wavefronts load, served from L2.  Ties keep the earliest direction
-// (strict >), invalid links never win, and the slope is the IEEE f32 quotient
+// Each wavefront owns a strip of 64 columns and marches down kRcvRows rows:
+// a lane keeps the elevations of its column in the rows above, at and below
+// the current row in registers (one coalesced 256 B load per row and wave,
+// issued one row ahead), takes the left and right elevations from its
+// neighbour lanes with a shuffle, and loads from memory only the neighbours
+// outside the strip (the strip's edge lanes, periodic wraps).  Every z is
+// thus read once from HBM.  Ties keep the earliest direction (strict >),
+// invalid links never win, and the slope is the IEEE f32 quotient
 // (z_i - z_j) / dx, so receivers match the reference bit for bit.
 // ---------------------------------------------------------------------------
+constexpr int kRcvRows = 16;
 // Uniform draw in [0, 1) for cell i, direction k of one stochastic call
 // (RAND_RCV, receivers.py:46-47): splitmix64 (Steele, Lea and Flood 2014) of
 // seed + (4i+k) times the golden-ratio constant, top 24 bits times 2^-24
@@ -29,32 +34,64 @@ __global__ __launch_bounds__(256) void k_receivers(GridD g, const float* __restr
                                                    float* __restrict__ grad, uint64_t seed = 0) {
   int bx, by;
   xcd_block(bx, by);
-  const int col = bx * 64 + threadIdx.x;
-  const int row = by * 4 + threadIdx.y;
-  if (col >= g.nx || row >= g.ny) return;
-  const int i = row * g.nx + col;
-  const float zi = z[i];
-  int r = i;
-  float sr = 0.0f;
+  const int lane = threadIdx.x;
+  const int col = bx * 64 + lane;
+  const int r0 = (by * 4 + threadIdx.y) * kRcvRows;
+  if (r0 >= g.ny) return;  // uniform per wavefront
+  const int r1 = min(r0 + kRcvRows, g.ny);
+  const bool cv = col < g.nx;
+  float zu = (cv && r0 > 0) ? z[(r0 - 1) * g.nx + col] : 0.0f;  // row above
+  float zc = cv ? z[r0 * g.nx + col] : 0.0f;                     // current row
+  for (int row = r0; row < r1; ++row) {
+    const int i = row * g.nx + col;
+    const float zd = (cv && row + 1 < g.ny) ? z[i + g.nx] : 0.0f;  // row below
+    const float zl = __shfl_up(zc, 1);    // column - 1 (lanes >= 1)
+    const float zr = __shfl_down(zc, 1);  // column + 1 (lanes <= 62)
+    if (cv) {
+      int jn[4];
+      float zn[4];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int j = neighbour<MODE>(g, i, row, col, k);
-    if (j >= 0) {
-      float t = (zi - z[j]) / g.dx;
-      if constexpr (STOCH) t = t * uniform01(seed, i, k);  // the weighted slope is also the stored gradient
-      if (t > sr) {
-        sr = t;
-        r = j;
+      for (int k = 0; k < 4; ++k) {
+        const int j = neighbour<MODE>(g, i, row, col, k);
+        jn[k] = j;
+        // neighbours outside the registers are loaded, all before any use
+        const bool held = j == i - g.nx || j == i + g.nx || (j == i - 1 && lane > 0) ||
+                          (j == i + 1 && lane < 63 && col + 1 < g.nx);
+        zn[k] = (j >= 0 && !held) ? z[j] : 0.0f;
       }
+      int r = i;
+      float sr = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int j = jn[k];
+        if (j >= 0) {
+          const float zj = j == i - g.nx ? zu
+                         : j == i + g.nx ? zd
+                         : (j == i - 1 && lane > 0) ? zl
+                         : (j == i + 1 && lane < 63 && col + 1 < g.nx) ? zr : zn[k];
+          float t = (zc - zj) / g.dx;
+          if constexpr (STOCH) t = t * uniform01(seed, i, k);  // the weighted slope is also the stored gradient
+          if (t > sr) {
+            sr = t;
+            r = j;
+          }
+        }
+      }
+      rcv[i] = r;
+      grad[i] = sr;
     }
+    zu = zc;
+    zc = zd;
   }
-  rcv[i] = r;
-  grad[i] = sr;
+}
+
+static dim3 receivers_grid(const GridD& g) {
+  return dim3((g.nx + 63) / 64, (g.ny + 4 * kRcvRows - 1) / (4 * kRcvRows));
 }
 
 int launch_receivers(pff_ctx* ctx, const GridD& g, const float* z, int32_t* rcv, float* grad) {
   dim3 block(64, 4);
-  dim3 grid((g.nx + 63) / 64, (g.ny + 3) / 4);
+  dim3 grid = receivers_grid(g);
   PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_receivers<M>, grid, block, 0, ctx->stream, g, z, rcv, grad, 0ull));
   return check_hip(hipGetLastError(), "receivers");
 }
@@ -62,7 +99,7 @@ int launch_receivers(pff_ctx* ctx, const GridD& g, const float* z, int32_t* rcv,
 int launch_receivers_stochastic(pff_ctx* ctx, const GridD& g, const float* z, int32_t* rcv, float* grad,
                                 uint64_t seed) {
   dim3 block(64, 4);
-  dim3 grid((g.nx + 63) / 64, (g.ny + 3) / 4);
+  dim3 grid = receivers_grid(g);
   PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL((k_receivers<M, true>), grid, block, 0, ctx->stream, g, z, rcv, grad,
                                                seed));
   return check_hip(hipGetLastError(), "receivers (stochastic)");

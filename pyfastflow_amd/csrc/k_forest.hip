@@ -496,6 +496,12 @@ struct TileArgs {
   // reduced graph) instead of failing the pass
   int mark_cycles;
   unsigned long long* prof_cells;  // profiling: cells processed by this launch
+  // carve tag walks across the trips of one reroute_flow call: per ring slot
+  // the local end of its walk (12 bits) and whether that end is a root
+  // (bit 15), valid for the tiles whose tdirty word is 0 (their receivers
+  // have not changed since the walk)
+  uint16_t* rcache;
+  uint32_t* tdirty;
 };
 
 template <int W>
@@ -1003,6 +1009,45 @@ __device__ __forceinline__ int walk_end(int li, const uint16_t* P, int64_t* cycl
   return li;
 }
 
+// the same walk cut after `cap` steps: -1 if it has not reached the end
+__device__ __forceinline__ int walk_end_capped(int li, const uint16_t* P, int cap) {
+  int c = li;
+  for (int s = 0; s < cap; ++s) {
+    const uint16_t n = P[c];
+    if (n == NONE) return c;
+    c = n;
+  }
+  return -1;
+}
+
+// Local ends of a whole tile by pointer jumping in place in LDS: P[c] <-
+// P[P[c]] while that is not the end marker.  Concurrent updates only move a
+// pointer further along its own chain, so any interleaving converges; after
+// it, the end of c is c if P[c] is NONE, else P[c].  13 rounds cover chains
+// of 4096 cells; a pointer still moving then lies on a cycle.
+template <int NT, int PER>
+__device__ __forceinline__ void tile_jump_ends(uint16_t* P, int64_t* cycle) {
+  for (int r = 0; r < 14; ++r) {
+    bool moved = false;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int li = threadIdx.x + j * NT;
+      const uint16_t n = P[li];
+      if (n != NONE) {
+        const uint16_t nn = P[n];
+        if (nn != NONE) {
+          P[li] = nn;
+          moved = true;
+        }
+      }
+    }
+    if (!__syncthreads_or(moved)) return;
+  }
+  if (threadIdx.x == 0) atomicOr((unsigned long long*)cycle, 1ull);
+}
+
+constexpr int kWalkCap = 384;  // ring walks longer than this switch the tile to pointer jumping
+
 // local pass: tags below the tile's saddle nodes (wbits, cleared here), the
 // local end E of every ring cell, the reduced-graph nodes (the ends those
 // walks reach: listed exits with S = 1 if a saddle path reaches them, or
@@ -1023,9 +1068,37 @@ __global__ __launch_bounds__(kWB) void k_tag_local(TileArgs a, uint32_t* __restr
   const GridD& g = a.g;
   const int x0 = blockIdx.x * TS, y0 = blockIdx.y * TS;
   const int tw = min(TS, g.nx - x0), th = min(TS, g.ny - y0);
+  const uint32_t w = threadIdx.x < TW ? wbits[(size_t)tile * TW + threadIdx.x] : 0u;
+  const int base = tile * kRing;
+  // A tile whose receivers have not changed since its ring walks were cached
+  // and that holds no saddle node this trip adds no tag: only its ring-slot
+  // graph is needed, and the cached ends give it without loading the tile
+  // (the exit test reads the end's current label: labels only merge).
+  const bool clean = a.rcache && a.tdirty[tile] == 0u;  // uniform per block
+  if (clean && !__syncthreads_or(w != 0u)) {
+    const int nr = ring_count(tw, th);
+    int32_t xs[1] = {0};
+    bool xt[1] = {false};
+    if ((int)threadIdx.x < nr) {
+      const int k = threadIdx.x;
+      const uint32_t cv = a.rcache[base + k];
+      const int e = (int)(cv & 0x0FFFu);
+      const bool eroot = (cv >> 15) != 0u;
+      const bool ex = !eroot && a.bid[(y0 + (e >> 6)) * g.nx + x0 + (e & 63)] != 0;
+      a.ringE[base + k] = ex ? base + ring_index(e >> 6, e & 63, tw, th) : a.ring_term;
+      a.ringIn[base + k] = 0u;
+      a.ringS[base + k] = 0u;
+      xt[0] = ex && e == ring_cell(k, tw, th);  // the ring cell is its own end: an exit
+      xs[0] = base + k;
+    }
+    if (threadIdx.x == 0) sl.cnt[0][tile] = 0;
+    seg_append<1, kWB>(xs, xt, sl.seg[1] + (size_t)tile * TC, &sl.cnt[1][tile]);
+    if (a.prof_cells && threadIdx.x == 0)
+      atomicAdd(a.prof_cells, (unsigned long long)(16 * nr + 4 * sl.cnt[1][tile]));
+    return;
+  }
   int rv[CPW];
   tile_load_rcv(a, x0, y0, tw, th, rv);
-  const uint32_t w = threadIdx.x < TW ? wbits[(size_t)tile * TW + threadIdx.x] : 0u;
   if (threadIdx.x < TW) {
     T[threadIdx.x] = 0u;
     Lb[threadIdx.x] = 0u;
@@ -1051,14 +1124,30 @@ __global__ __launch_bounds__(kWB) void k_tag_local(TileArgs a, uint32_t* __restr
       atomicOr(&Lb[e >> 5], 1u << (e & 31));
     }
   }
-  // ring cells: their local ends
+  // ring cells: their local ends.  Walks are cut at kWalkCap steps; if one
+  // is longer (long paths through the tile, common once basins have been
+  // carved together) the tile finds every local end by pointer jumping,
+  // which takes log2 of the path length in LDS rounds.  The tags are done,
+  // so the pointers may be overwritten.
   __shared__ uint16_t rend[kRing];
   __shared__ uint32_t Xb[TW];
   const int nr = ring_count(tw, th);
+  bool unfinished = false;
   for (int k = threadIdx.x; k < nr; k += kWB) {
-    const int e = walk_end(ring_cell(k, tw, th), P, a.cycle);
-    rend[k] = (uint16_t)e;
-    atomicOr(&Lb[e >> 5], 1u << (e & 31));
+    const int e = walk_end_capped(ring_cell(k, tw, th), P, kWalkCap);
+    rend[k] = e < 0 ? NONE : (uint16_t)e;
+    if (e >= 0) atomicOr(&Lb[e >> 5], 1u << (e & 31));
+    else unfinished = true;
+  }
+  if (__syncthreads_or(unfinished)) {
+    tile_jump_ends<kWB, CPW>(P, a.cycle);
+    for (int k = threadIdx.x; k < nr; k += kWB) {
+      if (rend[k] != NONE) continue;
+      const int c = ring_cell(k, tw, th);
+      const int e = P[c] == NONE ? c : P[c];
+      rend[k] = (uint16_t)e;
+      atomicOr(&Lb[e >> 5], 1u << (e & 31));
+    }
   }
   __syncthreads();
   // reached ends: roots and ends outside closed basins are terminals, the
@@ -1067,7 +1156,6 @@ __global__ __launch_bounds__(kWB) void k_tag_local(TileArgs a, uint32_t* __restr
     const int li = threadIdx.x + j * kWB;
     return (y0 + (li >> 6)) * g.nx + x0 + (li & 63);
   };
-  const int base = tile * kRing;
   int bv[CPW];
 #pragma unroll
   for (int j = 0; j < CPW; ++j) {
@@ -1092,6 +1180,7 @@ __global__ __launch_bounds__(kWB) void k_tag_local(TileArgs a, uint32_t* __restr
     const int e = rend[k];
     a.ringE[base + k] = bit_test(Xb, e) ? base + ring_index(e >> 6, e & 63, tw, th) : a.ring_term;
     a.ringIn[base + k] = 0u;
+    if (a.rcache) a.rcache[base + k] = (uint16_t)(e | (bit_test(R, e) ? 0x8000 : 0));
     const int li = ring_cell(k, tw, th);
     xt[0] = bit_test(Xb, li);
     xs[0] = base + k;
@@ -1102,6 +1191,7 @@ __global__ __launch_bounds__(kWB) void k_tag_local(TileArgs a, uint32_t* __restr
     if ((tmask >> j) & 1u) a.tag[cell_of(j)] = 1;
   seg_append_mask<CPW, kWB>(tmask, cell_of, sl.seg[0] + (size_t)tile * TC, &sl.cnt[0][tile]);
   seg_append<1, kWB>(xs, xt, sl.seg[1] + (size_t)tile * TC, &sl.cnt[1][tile]);
+  if (a.tdirty && threadIdx.x == 0) a.tdirty[tile] = 0u;  // the cache now holds this tile's walks
   // profiling: algorithmic bytes of the tile (rcv in; ring slots, tags and
   // list entries out)
   if (a.prof_cells && threadIdx.x == 0)
@@ -1406,6 +1496,23 @@ int tag_state(pff_ctx* ctx, const GridD& g, uint32_t** wbits, uint8_t** tag, boo
   return 0;
 }
 
+int ring_cache(pff_ctx* ctx, const GridD& g, uint16_t** rcache, uint32_t** tdirty) {
+  const size_t nt = (size_t)tiles_count(g);
+  const size_t cb = (nt * kRing * 2 + 255) & ~(size_t)255;
+  void* p;
+  PFF_TRY(scratch(ctx, S_RCACHE, cb + nt * 4, &p));
+  *rcache = (uint16_t*)p;
+  *tdirty = (uint32_t*)((char*)p + cb);
+  return 0;
+}
+
+int ring_cache_reset(pff_ctx* ctx, const GridD& g) {
+  uint16_t* rc;
+  uint32_t* td;
+  PFF_TRY(ring_cache(ctx, g, &rc, &td));
+  return launch_fill_u32(ctx, tiles_count(g), td, 1u);
+}
+
 int saddle_tags_walk(pff_ctx* ctx, const GridD& g, const int32_t* rcv, const int32_t* bid, uint32_t* wbits,
                      uint8_t* tag, const uint32_t* tact, int rounds, int32_t* tag_list) {
   TileBufs b;
@@ -1432,6 +1539,7 @@ int saddle_tags_walk(pff_ctx* ctx, const GridD& g, const int32_t* rcv, const int
     ringP2 = (int32_t*)(a.ringS + (nslots + 1));
     a.ring_term = nslots;
   }
+  PFF_TRY(ring_cache(ctx, g, &a.rcache, &a.tdirty));
   SegLists sl{};
   {
     void *d0, *d1, *co;

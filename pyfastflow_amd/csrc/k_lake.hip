@@ -979,14 +979,22 @@ __global__ __launch_bounds__(kBT) void k_carve_link_list(const int32_t* __restri
 
 // every changed receiver belongs to a tagged cell: copy those to the working copy
 // (and clears the tag bytes of the listed cells: zero between carves)
+// (tdirty, nullable: the tiles whose receivers change are marked dirty for
+// the ring-walk cache of the next trip's tag walks)
 __global__ __launch_bounds__(kBT) void k_carve_sync_list(const int32_t* __restrict__ list, const int64_t* nlist,
                                                          const int32_t* __restrict__ rec, int32_t* __restrict__ rec_work,
-                                                         uint8_t* __restrict__ tag) {
+                                                         uint8_t* __restrict__ tag, uint32_t* tdirty, int nx,
+                                                         int tiles_x) {
   const int64_t n = *nlist;
   for (int64_t e = (int64_t)blockIdx.x * kBT + threadIdx.x; e < n; e += (int64_t)gridDim.x * kBT) {
     const int i = list[e];
     if (rec_work) rec_work[i] = rec[i];
     tag[i] = 0;
+    if (tdirty) {
+      const int row = i / nx;
+      const int t = (row / TSL) * tiles_x + (i - row * nx) / TSL;
+      if (tdirty[t] == 0u) tdirty[t] = 1u;
+    }
   }
 }
 
@@ -1015,8 +1023,11 @@ int carve_tiled(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t* rec_i
   hipLaunchKernelGGL(k_carve_reverse_list, lg, bl, 0, ctx->stream, g.N, (const int32_t*)tl, ntag,
                      (const uint8_t*)tag, rec_in, rec);
   hipLaunchKernelGGL(k_carve_link_list, dim3(1024), bl, 0, ctx->stream, pits, npits, outlet, snode, rec);
+  uint16_t* rcache;
+  uint32_t* tdirty;
+  PFF_TRY(ring_cache(ctx, g, &rcache, &tdirty));
   hipLaunchKernelGGL(k_carve_sync_list, lg, bl, 0, ctx->stream, (const int32_t*)tl, ntag, (const int32_t*)rec,
-                     rec_work, tag);
+                     rec_work, tag, tdirty, g.nx, (g.nx + TSL - 1) / TSL);
   ctx->tag_state_dirty = false;  // k_tag_local cleared the saddle bits, k_carve_sync_list the tags
   return check_hip(hipGetLastError(), "carve_tiled");
 }
@@ -1067,7 +1078,7 @@ int carve_split(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t* rec_i
   hipLaunchKernelGGL(k_carve_sync, dim3(stride_grid()), dim3(kBlock), 0, s, (const int32_t*)list, w,
                      (uint32_t*)tagw, rec, rec_work, (uint8_t*)nullptr);
   hipLaunchKernelGGL(k_carve_sync_list, lg, bl, 0, s, (const int32_t*)tl, (const int64_t*)ntag, (const int32_t*)rec,
-                     rec_work, tag);
+                     rec_work, tag, (uint32_t*)nullptr, g.nx, (g.nx + TSL - 1) / TSL);
   ctx->tag_state_dirty = false;
   return check_hip(hipGetLastError(), "carve_split");
 }

@@ -118,6 +118,7 @@ enum Slot : int {
   S_SEGD0, S_SEGD1,     // per-tile list segments (4096 entries per tile), two lists
   S_SEGCO,              // their per-tile counts and offsets
   S_RING,               // ring-slot arrays of the carve tag walks (4 x (256 per tile + 1))
+  S_RCACHE,             // cached local ends of the ring walks (u16 per ring slot) + per-tile dirty words
   S_NSLOTS
 };
 static_assert(S_NSLOTS <= pff_ctx::kSlots, "pff_ctx::kSlots must cover every scratch slot");
@@ -214,6 +215,11 @@ int tag_state(pff_ctx* ctx, const GridD& g, uint32_t** wbits, uint8_t** tag, boo
 int saddle_tags_walk(pff_ctx* ctx, const GridD& g, const int32_t* rcv, const int32_t* bid, uint32_t* wbits,
                      uint8_t* tag, const uint32_t* tact, int rounds, int32_t* tag_list);
 int tiles_count(const GridD& g);
+// Cached ring-walk ends of the carve tag walks (k_tag_local) and the per-tile
+// dirty words (1 = receivers changed since the tile's walks were cached).
+// reset: every tile dirty (the receivers of a new reroute_flow call).
+int ring_cache(pff_ctx* ctx, const GridD& g, uint16_t** rcache, uint32_t** tdirty);
+int ring_cache_reset(pff_ctx* ctx, const GridD& g);
 int carve_tiled(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t* rec_in, const int32_t* bid,
                 const int32_t* snode, const int64_t* outlet, int32_t* rec_work, const uint32_t* tact,
                 const int32_t* pits, const int64_t* npits, int rounds, int32_t* cycle);

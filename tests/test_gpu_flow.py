@@ -268,3 +268,48 @@ def test_lake_outlets_per_trip(pf, ora, case):
         rest = np.ones(ol.size, bool)
         rest[b] = False
         assert (ol[rest] == INVALID_KEY).all() and (snl[rest] == -1).all(), f"trip {t}: basins missing from the log"
+
+
+def _snake_dem(n=96):
+    """A channel winding through the grid (channel rows 1, 3, 5, ... joined
+    at alternating ends, walls between them), descending 0.01 m per cell to a
+    pit at its end; a low border cell next to the channel's start.  The pit's
+    basin is the whole channel: its carve path runs back up thousands of
+    cells, and inside each 64 x 64 tile the walks along it are thousands of
+    steps long."""
+    z = np.full((n, n), 500.0)
+    order = []
+    rows = list(range(1, n - 1, 2))
+    for k, r in enumerate(rows):
+        cols = list(range(1, n - 1)) if k % 2 == 0 else list(range(n - 2, 0, -1))
+        order += [(r, c) for c in cols]
+        if k + 1 < len(rows):
+            order.append((r + 1, cols[-1]))
+    L = len(order)
+    for s, (r, c) in enumerate(order):
+        z[r, c] = 100.0 + 0.01 * (L - s)
+    z[order[-1]] = 10.0
+    z[0, 2] = 5.0
+    return z.astype(np.float32)
+
+
+@pytest.mark.parametrize("n,mode", [(96, "normal"), (160, "normal"), (130, "periodic_EW")])
+def test_reroute_long_in_tile_paths(pf, ora, n, mode):
+    """Carve along a winding channel: in-tile walks far longer than the
+    walk cap (the tag walks then find local ends by pointer jumping in LDS).
+    Receivers, first-trip labels and per-trip outlets bit-exact."""
+    print(f"case snake {n} {mode}", flush=True)
+    z = _snake_dem(n)
+    r = _router(pf, n, n, 10.0, mode, z)
+    r.compute_receivers()
+    bf = torch.empty(n * n, dtype=torch.int32, device="cuda")
+    r.reroute_flow(carve=True, bid_first=bf, log_outlets=True)
+    g = _ograd(ora, n, n, 10.0, mode)
+    rcv, _ = ora.compute_receivers(g, z)
+    lk = ora.reroute_flow(g, z, rcv, carve=True, log_iters=16)
+    np.testing.assert_array_equal(bf.cpu().numpy(), lk["bid_first"])
+    np.testing.assert_array_equal(r.receivers.to_numpy(), lk["rec"])
+    assert ora.depression_counter(g, lk["rec"]) == 0
+    for t, (b, o, sn) in enumerate(r.last_lake_outlets):
+        np.testing.assert_array_equal(o, lk["outlet_log"][t][b])
+        np.testing.assert_array_equal(sn, lk["saddlenode_log"][t][b])
