@@ -277,6 +277,27 @@ int pff_batch_route(pff_ctx* ctx, int32_t ntiles, const pff_tile* tiles, float d
 int pff_conv_pack_weights(pff_ctx* ctx, int32_t cin, int32_t cout, const float* w, uint16_t* wf, uint16_t* wt);
 int pff_conv3x3_fwd(pff_ctx* ctx, int32_t n, int32_t h, int32_t w, int32_t cin, int32_t cout, const uint16_t* x,
                     const uint16_t* wf, const float* bias, int32_t relu, uint16_t* y);
+/* Inference form of the forward convolution that reads its input from the
+   layers in front of it, so that their outputs never go through memory.
+   src->mode 0: x [n][h][w][cin] (as pff_conv3x3_fwd); 1: concat(upsample2(x),
+   skip) with x [n][h/2][w/2][ca] and skip [n][h][w][cin-ca] (pff_upsample2 +
+   pff_concat); 2: the f32 planes [nplanes][n*h*w] rounded to bf16 and zero
+   padded to cin = 32 channels (pff_pack_input).  head_out non-NULL (cout =
+   32): y is not written and head_out[p] = head_b[0] + sum_c head_w[c] *
+   act[p][c] (pff_head1x1 of the activated output).  Same bits as the
+   unfused sequence.  Needs w a multiple of 32 and h a multiple of 16. */
+typedef struct {
+  int32_t mode;
+  const uint16_t* x;
+  const uint16_t* skip;
+  const float* planes;
+  int32_t ca, nplanes;
+  const float* head_w;
+  const float* head_b;
+  float* head_out;
+} pff_conv_src;
+int pff_conv3x3_fwd_fused(pff_ctx* ctx, int32_t n, int32_t h, int32_t w, int32_t cin, int32_t cout,
+                          const pff_conv_src* src, const uint16_t* wf, const float* bias, int32_t relu, uint16_t* y);
 int pff_conv3x3_bwd_data(pff_ctx* ctx, int32_t n, int32_t h, int32_t w, int32_t cin, int32_t cout,
                          const uint16_t* dy, const uint16_t* wt, uint16_t* dx);
 int pff_conv3x3_bwd_weight(pff_ctx* ctx, int32_t n, int32_t h, int32_t w, int32_t cin, int32_t cout,

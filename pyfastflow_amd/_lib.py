@@ -62,6 +62,11 @@ class PffGfStatus(ctypes.Structure):
     ]
 
 
+class PffConvSrc(ctypes.Structure):
+    _fields_ = [("mode", c_int), ("x", c_p), ("skip", c_p), ("planes", c_p), ("ca", c_int), ("nplanes", c_int),
+                ("head_w", c_p), ("head_b", c_p), ("head_out", c_p)]
+
+
 class PffTile(ctypes.Structure):
     _fields_ = [("nx", c_int), ("ny", c_int), ("z", c_p), ("rcv", c_p), ("Q", c_p)]
 
@@ -117,6 +122,8 @@ SIGNATURES = {
                         ctypes.POINTER(PffBatchStatus)],
     "pff_conv_pack_weights": [c_p, c_int, c_int, c_p, c_p, c_p],
     "pff_conv3x3_fwd": [c_p, c_int, c_int, c_int, c_int, c_int, c_p, c_p, c_p, c_int, c_p],
+    "pff_conv3x3_fwd_fused": [c_p, c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(PffConvSrc), c_p, c_p, c_int,
+                              c_p],
     "pff_conv3x3_bwd_data": [c_p, c_int, c_int, c_int, c_int, c_int, c_p, c_p, c_p],
     "pff_conv3x3_bwd_weight": [c_p, c_int, c_int, c_int, c_int, c_int, c_p, c_p, c_p, c_p],
     "pff_relu_bwd": [c_p, c_i64, c_p, c_p],

@@ -106,6 +106,191 @@ __global__ __launch_bounds__(NW * 64) void k_conv3x3_fwd(int H, int W, int Cin, 
   }
 }
 
+// Forward 3x3 convolution, tiled version for images whose W is a multiple of
+// 32 and H a multiple of 16.  A block (4 waves) computes a TH x 32 pixel tile
+// for CO output channels: 16 x 32 pixels for CO = 32, 8 x 32 for CO = 64, so
+// that each wave owns 128 or 64 pixels (8 or 4 MFMA tiles of 16 pixels along
+// a row) and all CO channels (2 or 4 MFMA tiles).  Every A operand (16 pixels
+// x 32 input channels) then serves CO/16 MFMAs and every B operand (32 input
+// x 16 output channels) 8 or 4, so that the LDS operand reads keep up with
+// the matrix cores.  Per 32-channel input chunk the block stages in LDS the
+// (TH+2) x 34 input pixels its taps read (zero outside the image) and the
+// chunk's weights of all 9 taps for its CO channels, both with an 80-byte
+// pitch so that the 16-byte operand reads are conflict-free.  With CO = 32
+// the result goes through LDS so that each pixel's 64-byte run of channels is
+// stored at once.
+//
+// The input can also be read from the operands of the layers in front of the
+// convolution, for inference, so that their outputs never go through memory:
+// SRC_UPCAT reads concat(upsample2(lo), skip) (nearest 2x upsampling of a
+// half-resolution input, then its channels followed by the skip channels),
+// SRC_PLANES the network's f32 input planes rounded to bf16 and zero padded to
+// 32 channels (pack_input).  With HEAD, the 1x1 head (y = b + sum_c w_c x_c)
+// is applied to the activated output instead of storing it.
+constexpr int kTX = 32;  // tile width (pixels)
+enum ConvSrc : int { SRC_X = 0, SRC_UPCAT = 1, SRC_PLANES = 2 };
+
+struct ConvIn {
+  const uint16_t* x;     // SRC_X: [n][H][W][Cin]; SRC_UPCAT: [n][H/2][W/2][Ca]
+  const uint16_t* skip;  // SRC_UPCAT: [n][H][W][Cin - Ca]
+  const float* planes;   // SRC_PLANES: [nplanes][n*H*W]
+  int Ca, nplanes;
+  const float* hw;       // HEAD: weights (Cout) and bias (1) of the 1x1 head, f32 output
+  const float* hb;
+  float* hy;
+};
+
+__host__ __device__ constexpr bool conv_tiled_ok(int H, int W) { return W % kTX == 0 && H % 16 == 0; }
+
+template <int SRC>
+__device__ __forceinline__ uint4 conv_load8(const ConvIn& in, int img, int nimg, int H, int W, int Cin, int gy,
+                                            int gx, int c0) {
+  if constexpr (SRC == SRC_X) {
+    return *reinterpret_cast<const uint4*>(in.x + (((size_t)img * H + gy) * W + gx) * Cin + c0);
+  } else if constexpr (SRC == SRC_UPCAT) {
+    if (c0 < in.Ca)
+      return *reinterpret_cast<const uint4*>(in.x + (((size_t)img * (H / 2) + gy / 2) * (W / 2) + gx / 2) * in.Ca + c0);
+    const int Cb = Cin - in.Ca;
+    return *reinterpret_cast<const uint4*>(in.skip + (((size_t)img * H + gy) * W + gx) * Cb + (c0 - in.Ca));
+  } else {
+    const int64_t npix = (int64_t)nimg * H * W;
+    const int64_t p = ((int64_t)img * H + gy) * W + gx;
+    uint16_t e[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) e[j] = (c0 + j < in.nplanes) ? f2bf(in.planes[(int64_t)(c0 + j) * npix + p]) : (uint16_t)0;
+    uint4 v;
+    v.x = e[0] | ((uint32_t)e[1] << 16);
+    v.y = e[2] | ((uint32_t)e[3] << 16);
+    v.z = e[4] | ((uint32_t)e[5] << 16);
+    v.w = e[6] | ((uint32_t)e[7] << 16);
+    return v;
+  }
+}
+
+template <int CO, int SRC, bool HEAD>
+__global__ __launch_bounds__(256) void k_conv3x3_fwd_t(int nimg, int H, int W, int Cin, int Cout, ConvIn in,
+                                                       const uint16_t* __restrict__ wf,
+                                                       const float* __restrict__ bias, int relu,
+                                                       uint16_t* __restrict__ y) {
+  constexpr int NJ = CO / 16;          // channel tiles per wave
+  constexpr int TH = CO == 32 ? 16 : 8;  // tile rows
+  constexpr int WR = TH / 4;           // rows per wave
+  constexpr int NA = 2 * WR;           // pixel tiles per wave
+  constexpr int SWd = kTX + 2, SN = (TH + 2) * SWd;
+  constexpr int OP = CO + 8;  // output staging pitch (CO = 32)
+  static_assert(CO != 32 || TH * kTX * OP <= SN * kRow, "the output staging fits the input window");
+  __shared__ __attribute__((aligned(16))) uint16_t xs[SN][kRow];
+  __shared__ __attribute__((aligned(16))) uint16_t ws[9][CO][kRow];
+  const int tiles_x = W / kTX;
+  const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+  const int img = blockIdx.y;
+  const int co0 = blockIdx.z * CO;
+  const int x0 = tx * kTX, y0 = ty * TH;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int kq = 8 * (lane >> 4);
+  const size_t img_base = (size_t)img * H * W;
+  f32x4 acc[NA][NJ];
+#pragma unroll
+  for (int t = 0; t < NA; ++t)
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) acc[t][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int ci0 = 0; ci0 < Cin; ci0 += kKC) {
+    // input window rows y0-1 .. y0+TH, columns x0-1 .. x0+32, 4 x 16 bytes per pixel
+    for (int it = threadIdx.x; it < SN * 4; it += 256) {
+      const int px = it >> 2, q = it & 3;
+      const int r = px / SWd, c = px - r * SWd;
+      const int gy = y0 - 1 + r, gx = x0 - 1 + c;
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (gy >= 0 && gy < H && gx >= 0 && gx < W) v = conv_load8<SRC>(in, img, nimg, H, W, Cin, gy, gx, ci0 + 8 * q);
+      *reinterpret_cast<uint4*>(&xs[px][8 * q]) = v;
+    }
+    // weights wf[tap][co][ci] of this chunk
+    for (int it = threadIdx.x; it < 9 * CO * 4; it += 256) {
+      const int q = it & 3, rest = it >> 2;
+      const int tap = rest / CO, c = rest - tap * CO;
+      *reinterpret_cast<uint4*>(&ws[tap][c][8 * q]) =
+          *reinterpret_cast<const uint4*>(wf + ((size_t)tap * Cout + co0 + c) * Cin + ci0 + 8 * q);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+      const int ky = tap / 3, kx = tap - 3 * ky;
+      bf16x8 b[NJ];
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) b[j] = *reinterpret_cast<const bf16x8*>(&ws[tap][16 * j + (lane & 15)][kq]);
+#pragma unroll
+      for (int t = 0; t < NA; ++t) {
+        // pixel tile t: row wave*WR + t/2, columns 16*(t%2) .. +15
+        const int qy = wave * WR + (t >> 1), qx = 16 * (t & 1) + (lane & 15);
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&xs[(qy + ky) * SWd + qx + kx][kq]);
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) acc[t][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[t][j], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+  // D layout: lane holds pixels 4*(lane/16)+r of the 16-pixel tile, channel lane%16
+  if constexpr (CO == 32) {
+    uint16_t* os = &xs[0][0];  // reused: [pixel][CO + 8] bf16, pixel = row * 32 + column
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+      const int c = 16 * j + (lane & 15);
+      const float bv = bias ? bias[co0 + c] : 0.f;
+#pragma unroll
+      for (int t = 0; t < NA; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int q = (wave * WR + (t >> 1)) * kTX + 16 * (t & 1) + 4 * (lane >> 4) + r;
+          float v = acc[t][j][r] + bv;
+          if (relu) v = fmaxf(v, 0.f);
+          os[q * OP + c] = f2bf(v);
+        }
+    }
+    __syncthreads();
+    if constexpr (HEAD) {
+      // k_head's sum, in the same order: b, then c = 0 .. CO-1
+      for (int q = threadIdx.x; q < TH * kTX; q += 256) {
+        float sum = in.hb[0];
+        for (int c = 0; c < CO; ++c) sum = __builtin_fmaf(bf2f(os[q * OP + c]), in.hw[c], sum);
+        in.hy[img_base + (size_t)(y0 + q / kTX) * W + x0 + (q % kTX)] = sum;
+      }
+    } else {
+      for (int it = threadIdx.x; it < TH * kTX * (CO / 8); it += 256) {
+        const int q = it / (CO / 8), c8 = it - q * (CO / 8);
+        const size_t p = img_base + (size_t)(y0 + q / kTX) * W + x0 + (q % kTX);
+        *reinterpret_cast<uint4*>(y + p * Cout + co0 + 8 * c8) = *reinterpret_cast<const uint4*>(&os[q * OP + 8 * c8]);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+      const int c = co0 + 16 * j + (lane & 15);
+      const float bv = bias ? bias[c] : 0.f;
+#pragma unroll
+      for (int t = 0; t < NA; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int qy = wave * WR + (t >> 1), qx = 16 * (t & 1) + 4 * (lane >> 4) + r;
+          float v = acc[t][j][r] + bv;
+          if (relu) v = fmaxf(v, 0.f);
+          y[(img_base + (size_t)(y0 + qy) * W + x0 + qx) * Cout + c] = f2bf(v);
+        }
+    }
+  }
+}
+
+template <int SRC, bool HEAD>
+int launch_conv_t(pff_ctx* ctx, int n_img, int H, int W, int Cin, int Cout, const ConvIn& in, const uint16_t* wf,
+                  const float* bias, int relu, uint16_t* y) {
+  if (Cout % 64 == 0 && !HEAD)
+    hipLaunchKernelGGL((k_conv3x3_fwd_t<64, SRC, HEAD>), dim3((W / kTX) * (H / 8), n_img, Cout / 64), dim3(256), 0,
+                       ctx->stream, n_img, H, W, Cin, Cout, in, wf, bias, relu, y);
+  else
+    hipLaunchKernelGGL((k_conv3x3_fwd_t<32, SRC, HEAD>), dim3((W / kTX) * (H / 16), n_img, Cout / 32), dim3(256), 0,
+                       ctx->stream, n_img, H, W, Cin, Cout, in, wf, bias, relu, y);
+  return check_hip(hipGetLastError(), "conv3x3_fwd (tiled)");
+}
+
 // weight gradient dw[co][ci][ky][kx] += sum over pixels of dy[p][co] *
 // x[p shifted by the tap][ci].  A block takes 64 output channels x 32 input
 // channels and a run of image rows; per 64-pixel segment it stages dy
@@ -316,7 +501,7 @@ __global__ __launch_bounds__(256) void k_head(int64_t npix, int C, const uint16_
                                              float* __restrict__ y) {
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += (int64_t)gridDim.x * blockDim.x) {
     float s = b[0];
-    for (int c = 0; c < C; ++c) s += bf2f(x[p * C + c]) * w[c];
+    for (int c = 0; c < C; ++c) s = __builtin_fmaf(bf2f(x[p * C + c]), w[c], s);
     y[p] = s;
   }
 }
@@ -364,6 +549,11 @@ unsigned ew_grid(int64_t n) {
 
 int launch_conv3x3_fwd(pff_ctx* ctx, int n_img, int H, int W, int Cin, int Cout, const uint16_t* x,
                        const uint16_t* wf, const float* bias, int relu, uint16_t* y) {
+  if (conv_tiled_ok(H, W)) {
+    ConvIn in{};
+    in.x = x;
+    return launch_conv_t<SRC_X, false>(ctx, n_img, H, W, Cin, Cout, in, wf, bias, relu, y);
+  }
   const dim3 grid((W + kPix - 1) / kPix, n_img * H, Cout / 64 > 0 && Cout % 64 == 0 ? Cout / 64 : Cout / 32);
   if (Cout % 64 == 0)
     hipLaunchKernelGGL(k_conv3x3_fwd<4>, grid, dim3(256), 0, ctx->stream, H, W, Cin, Cout, x, wf, bias, relu, y);
@@ -479,6 +669,46 @@ int pff_conv3x3_fwd(pff_ctx* ctx, int32_t n, int32_t h, int32_t w, int32_t cin, 
   PFF_TRY(sg_conv_shape(n, h, w, cin, cout, "conv3x3_fwd"));
   if (!x || !wf || !y) return fail(PFF_EINVAL, "conv3x3_fwd: NULL array");
   return launch_conv3x3_fwd(ctx, n, h, w, cin, cout, x, wf, bias, relu, y);
+}
+
+int pff_conv3x3_fwd_fused(pff_ctx* ctx, int32_t n, int32_t h, int32_t w, int32_t cin, int32_t cout,
+                          const pff_conv_src* src, const uint16_t* wf, const float* bias, int32_t relu, uint16_t* y) {
+  PFF_TRY(sg_begin(ctx));
+  PFF_TRY(sg_conv_shape(n, h, w, cin, cout, "conv3x3_fwd_fused"));
+  if (!src || !wf) return fail(PFF_EINVAL, "conv3x3_fwd_fused: NULL argument");
+  if (!conv_tiled_ok(h, w))
+    return fail(PFF_EINVAL, "conv3x3_fwd_fused: w must be a multiple of 32 and h a multiple of 16");
+  const bool head = src->head_out != nullptr;
+  if (head && (cout != 32 || !src->head_w || !src->head_b))
+    return fail(PFF_EINVAL, "conv3x3_fwd_fused: the head needs cout = 32 and its weights");
+  if (!head && !y) return fail(PFF_EINVAL, "conv3x3_fwd_fused: NULL output");
+  ConvIn in{};
+  in.x = src->x;
+  in.skip = src->skip;
+  in.planes = src->planes;
+  in.Ca = src->ca;
+  in.nplanes = src->nplanes;
+  in.hw = src->head_w;
+  in.hb = src->head_b;
+  in.hy = src->head_out;
+  switch (src->mode) {
+    case SRC_X:
+      if (!in.x) return fail(PFF_EINVAL, "conv3x3_fwd_fused: NULL x");
+      return head ? launch_conv_t<SRC_X, true>(ctx, n, h, w, cin, cout, in, wf, bias, relu, y)
+                  : launch_conv_t<SRC_X, false>(ctx, n, h, w, cin, cout, in, wf, bias, relu, y);
+    case SRC_UPCAT:
+      if (!in.x || !in.skip || in.Ca < 32 || in.Ca % 32 || in.Ca >= cin || (h % 2) || (w % 2))
+        return fail(PFF_EINVAL, "conv3x3_fwd_fused: upsample+concat needs x, skip, 32 <= ca < cin (multiple of 32)");
+      return head ? launch_conv_t<SRC_UPCAT, true>(ctx, n, h, w, cin, cout, in, wf, bias, relu, y)
+                  : launch_conv_t<SRC_UPCAT, false>(ctx, n, h, w, cin, cout, in, wf, bias, relu, y);
+    case SRC_PLANES:
+      if (!in.planes || cin != 32 || in.nplanes < 1 || in.nplanes > 32)
+        return fail(PFF_EINVAL, "conv3x3_fwd_fused: planes need cin = 32 and 1..32 planes");
+      return head ? launch_conv_t<SRC_PLANES, true>(ctx, n, h, w, cin, cout, in, wf, bias, relu, y)
+                  : launch_conv_t<SRC_PLANES, false>(ctx, n, h, w, cin, cout, in, wf, bias, relu, y);
+    default:
+      return fail(PFF_EINVAL, "conv3x3_fwd_fused: mode must be 0, 1 or 2");
+  }
 }
 
 int pff_conv3x3_bwd_data(pff_ctx* ctx, int32_t n, int32_t h, int32_t w, int32_t cin, int32_t cout,

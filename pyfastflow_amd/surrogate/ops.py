@@ -171,3 +171,52 @@ def pack_input(planes: torch.Tensor, channels: int = 32) -> torch.Tensor:
     pl = planes.to(torch.float32).permute(1, 0, 2, 3).contiguous()  # [p][n*h*w]
     _lib.call("pff_pack_input", _ctx(pl), n * h * w, p, channels, _p(pl), _p(out))
     return out
+
+
+def tiled_ok(h: int, w: int) -> bool:
+    """Shapes the fused inference convolutions take (pff_conv3x3_fwd_fused)."""
+    return w % 32 == 0 and h % 16 == 0
+
+
+def conv3x3_fused(w, b, relu: bool = True, *, x=None, lo=None, skip=None, planes=None, head=None):
+    """Inference-only 3x3 convolution that reads its input from the layers in
+    front of it (no autograd): x (n, h, w, cin) NHWC bf16; or lo (n, h/2,
+    w/2, ca) and skip (n, h, w, cb) for concat(upsample2(lo), skip); or planes
+    (n, p, h, w) f32 zero padded to 32 channels (pack_input).  head = (hw, hb):
+    returns the 1x1 head of the activated output, (n, h, w) f32, instead of
+    the activations.  Same bits as the unfused ops."""
+    cout, cin = w.shape[0], w.shape[1]
+    if x is not None:
+        n, h, wd, _ = x.shape
+        src = _lib.PffConvSrc(0, _p(x.contiguous()), None, None, 0, 0, None, None, None)
+        keep = (x,)
+    elif lo is not None:
+        n, h, wd, _ = skip.shape
+        lo, skip = lo.contiguous(), skip.contiguous()
+        src = _lib.PffConvSrc(1, _p(lo), _p(skip), None, lo.shape[3], 0, None, None, None)
+        keep = (lo, skip)
+    else:
+        n, p, h, wd = planes.shape
+        pl = planes.to(torch.float32).permute(1, 0, 2, 3).contiguous()  # [p][n*h*w]
+        src = _lib.PffConvSrc(2, None, None, _p(pl), 0, p, None, None, None)
+        keep = (pl,)
+    dev = w.device
+    wf = torch.empty(9 * cout * cin, dtype=torch.bfloat16, device=dev)
+    wt = torch.empty_like(wf)
+    c = _lib.ctx_for(dev)
+    _lib.call("pff_conv_pack_weights", c, cin, cout, _p(w.contiguous()), _p(wf), _p(wt))
+    y = None
+    if head is not None:
+        hw, hb = head[0].detach().contiguous(), head[1].detach().contiguous()
+        out = torch.empty((n, h, wd), dtype=torch.float32, device=dev)
+        src.head_w, src.head_b, src.head_out = _p(hw), _p(hb), _p(out)
+        keep = keep + (hw, hb)
+    else:
+        y = torch.empty((n, h, wd, cout), dtype=torch.bfloat16, device=dev)
+        out = y
+    import ctypes
+
+    _lib.call("pff_conv3x3_fwd_fused", c, n, h, wd, cin, cout, ctypes.byref(src), _p(wf), _p(b.detach()),
+              int(relu), _p(y))
+    del keep
+    return out

@@ -45,7 +45,11 @@ class UNet(nn.Module):
 
     def forward(self, planes: torch.Tensor) -> torch.Tensor:
         """planes (n, in_planes, h, w) f32, h and w multiples of 2^(levels-1)
-        -> predicted log water depth (n, h, w) f32."""
+        -> predicted log water depth (n, h, w) f32.  Without gradients and on
+        shapes the fused kernels take, the input packing, the upsampling and
+        concatenation and the head run inside the convolutions (same bits)."""
+        if not torch.is_grad_enabled() and self._fused_ok(planes):
+            return self._forward_fused(planes)
         x = ops.pack_input(planes, self.base)
         skips = []
         i = 0
@@ -62,6 +66,31 @@ class UNet(nn.Module):
             x = ops.conv3x3(x, self.w[i + 1], self.b[i + 1])
             i += 2
         return ops.head1x1(x, self.hw, self.hb)
+
+    def _fused_ok(self, planes: torch.Tensor) -> bool:
+        _, p, h, w = planes.shape
+        if p > self.base or self.base != 32:
+            return False
+        return all(ops.tiled_ok(h >> k, w >> k) for k in range(self.levels))
+
+    def _forward_fused(self, planes: torch.Tensor) -> torch.Tensor:
+        w, b = self.w, self.b
+        x = ops.conv3x3_fused(w[0], b[0], planes=planes)
+        x = ops.conv3x3_fused(w[1], b[1], x=x)
+        skips = [x]
+        i = 2
+        for k in range(1, self.levels):
+            x = ops.avgpool2(x)
+            x = ops.conv3x3_fused(w[i], b[i], x=x)
+            x = ops.conv3x3_fused(w[i + 1], b[i + 1], x=x)
+            i += 2
+            skips.append(x)
+        for k in range(self.levels - 2, -1, -1):
+            x = ops.conv3x3_fused(w[i], b[i], lo=x, skip=skips[k])
+            last = k == 0
+            x = ops.conv3x3_fused(w[i + 1], b[i + 1], x=x, head=(self.hw, self.hb) if last else None)
+            i += 2
+        return x
 
     def flops_per_pixel(self) -> float:
         """Multiply-adds x 2 of the 3x3 convolutions and the head, per input pixel."""

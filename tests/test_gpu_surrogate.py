@@ -37,6 +37,12 @@ def _close(got, ref, rel=2.0**-8, absf=1e-3, what=""):
     (2, 16, 70, 32, 64, True),
     (1, 33, 64, 64, 32, False),
     (1, 8, 130, 96, 128, True),
+    # tiled kernel (W a multiple of 32, H of 16): 16 x 32 and 8 x 32 tiles
+    (1, 16, 128, 32, 32, False),
+    (2, 32, 96, 64, 64, True),
+    (1, 16, 64, 64, 128, False),
+    (3, 16, 32, 96, 32, True),
+    (1, 48, 160, 160, 64, False),
 ])
 def test_conv3x3_forward(pf, n, h, w, cin, cout, relu):
     from pyfastflow_amd.surrogate import ops
@@ -55,7 +61,8 @@ def test_conv3x3_forward(pf, n, h, w, cin, cout, relu):
     _close(y, ref, what="conv3x3 forward")
 
 
-@pytest.mark.parametrize("n,h,w,cin,cout", [(1, 6, 9, 32, 32), (2, 16, 66, 64, 32), (1, 12, 40, 32, 64)])
+@pytest.mark.parametrize("n,h,w,cin,cout", [(1, 6, 9, 32, 32), (2, 16, 66, 64, 32), (1, 12, 40, 32, 64),
+                                             (2, 16, 128, 32, 64), (1, 32, 32, 64, 32)])
 def test_conv3x3_backward(pf, n, h, w, cin, cout):
     from pyfastflow_amd.surrogate import ops
 
@@ -169,3 +176,21 @@ def test_unet_gradients_against_cpu(pf):
         cos = float(torch.dot(a, r) / (a.norm() * r.norm() + 1e-30))
         rel = float((a - r).norm() / (r.norm() + 1e-30))
         assert cos > 0.99 and rel < 0.1, f"parameter {k}: cosine {cos:.4f}, relative error {rel:.4f}"
+
+
+@pytest.mark.parametrize("size", [256, 512])
+def test_unet_fused_inference_same_bits(pf, size):
+    """Inference with the fused convolutions (input packing, upsampling +
+    concatenation and head read inside the convolutions) gives the same bits
+    as the layer-by-layer forward pass."""
+    from pyfastflow_amd.surrogate import UNet
+
+    m = UNet(seed=3).cuda()
+    g = torch.Generator().manual_seed(size)
+    planes = torch.randn(1, m.in_planes, size, size, generator=g).cuda()
+    with torch.no_grad():
+        assert m._fused_ok(planes)
+        fused = m(planes)
+    ref = m(planes.requires_grad_(True)).detach()  # grad enabled: the unfused path
+    assert fused.shape == ref.shape
+    assert torch.equal(fused.view(torch.int32), ref.view(torch.int32))
