@@ -184,6 +184,64 @@ def graphflood_block(pf, _lib, synth, s, args):
     return out
 
 
+def surrogate_block():
+    """Matrix-core throughput of the surrogate's BF16 MFMA kernels: 2048^2
+    inference (fused convolutions) and one training step (forward + backward
+    of a batch of 8 256^2 tiles), FLOPs of the 3x3 convolutions and the head
+    (UNet.flops_per_pixel; backward = 2x forward) against the 2.5 PFLOP/s
+    dense BF16 peak."""
+    import torch
+
+    from pyfastflow_amd.surrogate import UNet
+    from pyfastflow_amd.surrogate.warmstart import load_weights
+
+    peak = 2500.0  # TFLOP/s, dense BF16
+    m = load_weights(UNet().cuda()).eval()
+    out = {}
+    n = 2048
+    x = torch.randn(1, m.in_planes, n, n, device="cuda")
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    with torch.no_grad():
+        for _ in range(2):
+            m(x)
+        torch.cuda.synchronize()
+        ev[0].record()
+        for _ in range(10):
+            m(x)
+        ev[1].record()
+        torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[1]) / 10
+    fl = m.flops_per_pixel() * n * n
+    out["inference_2048"] = {"ms": round(ms, 3), "TFLOP": round(fl / 1e12, 3), "TFLOP_per_s": round(fl / ms / 1e9, 1),
+                             "frac_of_bf16_peak": round(fl / ms / 1e9 / peak, 4)}
+    m.train()
+    xb = torch.randn(8, m.in_planes, 256, 256, device="cuda")
+    yb = torch.randn(8, 256, 256, device="cuda")
+    opt = torch.optim.Adam(m.parameters(), lr=1e-4)
+
+    def step():
+        loss = ((m(xb) - yb) ** 2).mean()
+        opt.zero_grad(set_to_none=True)
+        loss.backward()
+        opt.step()
+
+    for _ in range(2):
+        step()
+    torch.cuda.synchronize()
+    ev[0].record()
+    for _ in range(10):
+        step()
+    ev[1].record()
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[1]) / 10
+    fl = 3 * m.flops_per_pixel() * 8 * 256 * 256
+    out["training_step_8x256"] = {"ms": round(ms, 3), "TFLOP_per_s": round(fl / ms / 1e9, 1),
+                                  "frac_of_bf16_peak": round(fl / ms / 1e9 / peak, 4)}
+    del m, opt
+    torch.cuda.empty_cache()
+    return out
+
+
 def warm_start_block(pf, synth, args):
     """Surrogate warm start against the cold start on a held-out 2048^2
     config-2-style DEM (beta 2.0, relief 1000 m, 500 pits, seed 9091),
@@ -394,6 +452,7 @@ def main():
         log(f"[bench] graphflood {gflood}")
         if not args.no_warm:
             warm = warm_start_block(pf, synth, args)
+            warm["surrogate_mfma"] = surrogate_block()
             log(f"[bench] warm start {warm}")
 
     # ---- CPU baseline (oracle, the reference algorithm restated in C) and
