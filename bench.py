@@ -46,6 +46,8 @@ def parse():
                    help="iterations of the 2048^2 GraphFlood run towards max |dh| < 1e-6 m (checked every 100)")
     p.add_argument("--no-warm", action="store_true",
                    help="skip the surrogate warm start against the cold start (2048^2, two solves)")
+    p.add_argument("--no-surrogate", action="store_true",
+                   help="skip the surrogate's matrix-core line (2048^2 inference, one training step)")
     p.add_argument("--pmc-calibrate", action="store_true",
                    help="after the steps, run a 1 GiB read + 1 GiB write elementwise pass 3 times "
                         "(scale for rocprofv3 FETCH/WRITE_SIZE)")
@@ -447,13 +449,16 @@ def main():
     #      iterated towards steady state, and the warm-start comparison
     gflood = None
     warm = None
+    mfma = None
     if not args.no_graphflood:
         gflood = graphflood_block(pf, _lib, synth, s, args)
         log(f"[bench] graphflood {gflood}")
-        if not args.no_warm:
-            warm = warm_start_block(pf, synth, args)
-            warm["surrogate_mfma"] = surrogate_block()
-            log(f"[bench] warm start {warm}")
+    if not args.no_warm:
+        warm = warm_start_block(pf, synth, args)
+        log(f"[bench] warm start {warm}")
+    if not args.no_surrogate:
+        mfma = surrogate_block()
+        log(f"[bench] surrogate MFMA {mfma}")
 
     # ---- CPU baseline (oracle, the reference algorithm restated in C) and
     #      full-size parity of the GPU result against it ----------------------
@@ -554,7 +559,7 @@ def main():
                                f"relief 1000 m seed 202; receivers + reroute_flow(carve) + accumulate_constant_Q(1e-5)",
                    "cells": N, "stages": stages, "lakeflow": lake, "parity": parity, "receivers_kernel": rcv_roof,
                    "instrumented_kernels": kernel_times, "ragged_batch": batch, "graphflood": gflood,
-                   "warm_start": warm},
+                   "warm_start": warm, "surrogate_mfma": mfma},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -1710,7 +1710,10 @@ int tiled_pits(pff_ctx* ctx, const GridD& g, int32_t** pits, int64_t** npits) {
 // keeps its path.  The next trip's labels init(root) (lakeflow.py:507-511)
 // are therefore the old labels mapped through the basin-level chain
 // b -> bid[outlet(b)] -> ... to its end: an uncarved basin (its pit stays a
-// root) or 0.  The pits of the next trip are the uncarved basins.
+// root) or 0.  The pits of the next trip are the uncarved basins.  Basin N
+// (custom mode, cell N-1 an interior pit) has no per-basin slot, never gets
+// an outlet and so stays a pit; a chain that reaches it ends there, as at
+// every other uncarved basin.
 
 // M[b] = bid[outlet node of b], or b when b has no outlet (then b is kept in
 // the next pits list); M[0] = 0; *np_saved = *np (the rounds after the
@@ -1745,6 +1748,9 @@ __global__ __launch_bounds__(kBT) void k_basin_next(const int32_t* __restrict__ 
         }
         M0[b] = m;
         M1[b] = m;
+      } else if (b == N) {
+        v[0] = b;
+        take[0] = true;
       }
     }
     block_append<1, kBT>(v, take, next_pits, n_next);
@@ -1771,7 +1777,11 @@ __global__ __launch_bounds__(kBT) void k_basin_chase(const int32_t* __restrict__
     if (m == b) continue;  // uncarved: a fixed point
     bool done = false;
     for (int k = 0; k < max_steps; ++k) {
-      if (m <= 0 || m >= N) {
+      if (m == N) {  // basin N: an uncarved end without a slot
+        done = true;
+        break;
+      }
+      if (m <= 0 || m > N) {
         M[b] = 0;
         done = true;
         break;
@@ -1802,7 +1812,7 @@ __global__ __launch_bounds__(kBT) void k_basin_jump(const int32_t* __restrict__ 
       const int b = pits[w];
       if (b > 0 && b < N) {
         const int m = Min[b];
-        const int mm = (m >= 0 && m < N) ? Min[m] : 0;
+        const int mm = (m >= 0 && m < N) ? Min[m] : (m == N ? N : 0);
         Mout[b] = mm;
         changed = mm != m;
       }
@@ -1860,7 +1870,7 @@ __global__ __launch_bounds__(kBT) void k_relabel_tiles(GridD g, int tiles_x, con
     b[k] = (col < g.nx && row < g.ny) ? bid[row * g.nx + col] : 0;
   }
 #pragma unroll
-  for (int k = 0; k < RR; ++k) nb[k] = (b[k] > 0 && b[k] < g.N) ? M[b[k]] : 0;
+  for (int k = 0; k < RR; ++k) nb[k] = (b[k] > 0 && b[k] < g.N) ? M[b[k]] : (b[k] == g.N ? g.N : 0);
   int any = 0;
 #pragma unroll
   for (int k = 0; k < RR; ++k) {

@@ -383,7 +383,8 @@ __global__ void k_carve_link(int32_t N, const int64_t* __restrict__ outlet, cons
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= N) return;
   const int64_t o = outlet[b];
-  if (o != INVALID_KEY) rec[snode[b]] = key_index(o);
+  const int sn = snode[b];
+  if (o != INVALID_KEY && sn >= 0 && sn < N) rec[sn] = key_index(o);
 }
 
 // copy the carved receivers of tagged nodes to the working copy and clear tags
@@ -483,10 +484,12 @@ __global__ __launch_bounds__(kBT) void k_saddle_init_all(int32_t n, int64_t* __r
 }
 
 // also zeroes the words of this trip (clear0, clear1, and the two words at
-// clear2; nullable)
-__global__ __launch_bounds__(kBT) void k_saddle_init_list(const int32_t* __restrict__ pits, const int64_t* np,
-                                                          int64_t* saddle, int64_t* outlet, int32_t* snode,
-                                                          int64_t* clear0, int64_t* clear1, int64_t* clear2) {
+// clear2; nullable).  Basin N (custom mode, cell N-1 an interior pit) has no
+// per-basin slot and is skipped.
+__global__ __launch_bounds__(kBT) void k_saddle_init_list(int32_t N, const int32_t* __restrict__ pits,
+                                                          const int64_t* np, int64_t* saddle, int64_t* outlet,
+                                                          int32_t* snode, int64_t* clear0, int64_t* clear1,
+                                                          int64_t* clear2) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     if (clear0) *clear0 = 0;
     if (clear1) *clear1 = 0;
@@ -498,6 +501,7 @@ __global__ __launch_bounds__(kBT) void k_saddle_init_list(const int32_t* __restr
   const int64_t n = *np;
   for (int64_t w = (int64_t)blockIdx.x * kBT + threadIdx.x; w < n; w += (int64_t)gridDim.x * kBT) {
     const int b = pits[w];
+    if (b <= 0 || b >= N) continue;
     saddle[b] = INVALID_KEY;
     outlet[b] = INVALID_KEY;
     snode[b] = -1;
@@ -919,7 +923,7 @@ int saddlesort_active(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t*
     hipLaunchKernelGGL(k_saddle_init_all, dim3(8192), bl, 0, s, g.N, saddle, outlet, snode, NB[out],
                        ctx->dev_words + W_NTAG, cyc_words);
   else
-    hipLaunchKernelGGL(k_saddle_init_list, lg, bl, 0, s, pits, npits, saddle, outlet, snode, NB[out],
+    hipLaunchKernelGGL(k_saddle_init_list, lg, bl, 0, s, g.N, pits, npits, saddle, outlet, snode, NB[out],
                        ctx->dev_words + W_NTAG, cyc_words);
   const bool full = in < 0 || ctx->host_words[W_NBL0 + in] * 2 > (int64_t)g.N;
   const dim3 lgz(2048);
@@ -965,15 +969,18 @@ __global__ __launch_bounds__(kBT) void k_carve_reverse_list(int32_t N, const int
   }
 }
 
-// saddle node -> outlet (lakeflow.py:399-403) over the pits
-__global__ __launch_bounds__(kBT) void k_carve_link_list(const int32_t* __restrict__ pits, const int64_t* np,
-                                                         const int64_t* __restrict__ outlet,
+// saddle node -> outlet (lakeflow.py:399-403) over the pits.  Basin N (custom
+// mode, cell N-1 an interior pit) has no per-basin slot: it has no outlet.
+__global__ __launch_bounds__(kBT) void k_carve_link_list(int32_t N, const int32_t* __restrict__ pits,
+                                                         const int64_t* np, const int64_t* __restrict__ outlet,
                                                          const int32_t* __restrict__ snode, int32_t* rec) {
   const int64_t n = *np;
   for (int64_t w = (int64_t)blockIdx.x * kBT + threadIdx.x; w < n; w += (int64_t)gridDim.x * kBT) {
     const int b = pits[w];
+    if (b <= 0 || b >= N) continue;
     const int64_t o = outlet[b];
-    if (o != INVALID_KEY) rec[snode[b]] = key_index(o);
+    const int sn = snode[b];
+    if (o != INVALID_KEY && sn >= 0 && sn < N) rec[sn] = key_index(o);
   }
 }
 
@@ -1022,7 +1029,7 @@ int carve_tiled(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t* rec_i
   const dim3 lg(2048), bl(kBT);
   hipLaunchKernelGGL(k_carve_reverse_list, lg, bl, 0, ctx->stream, g.N, (const int32_t*)tl, ntag,
                      (const uint8_t*)tag, rec_in, rec);
-  hipLaunchKernelGGL(k_carve_link_list, dim3(1024), bl, 0, ctx->stream, pits, npits, outlet, snode, rec);
+  hipLaunchKernelGGL(k_carve_link_list, dim3(1024), bl, 0, ctx->stream, g.N, pits, npits, outlet, snode, rec);
   uint16_t* rcache;
   uint32_t* tdirty;
   PFF_TRY(ring_cache(ctx, g, &rcache, &tdirty));

@@ -23,7 +23,7 @@ import torch
 
 from .. import synth
 from .unet import UNet
-from .warmstart import load_weights, make_dataset, save_weights, solve, train
+from .warmstart import load_weights, make_dataset, make_dataset_tiles, save_weights, solve, train
 
 
 def log(*a):
@@ -42,6 +42,10 @@ def main(argv=None):
     p.add_argument("--max-iter", type=int, default=60000)
     p.add_argument("--out", default=None, help="save the trained weights (torch state dict)")
     p.add_argument("--weights", default=None, help="skip training: load these weights")
+    p.add_argument("--tiles", type=int, default=0,
+                   help="train on this many independent tile x tile DEMs solved to steady state "
+                        "(make_dataset_tiles) instead of tiles cut from --domains larger DEMs")
+    p.add_argument("--tile-iters", type=int, default=8000, help="iteration cap of each tile grid's solve")
     a = p.parse_args(argv)
 
     res = {}
@@ -50,9 +54,14 @@ def main(argv=None):
         load_weights(model, a.weights)
     else:
         t0 = time.time()
-        X, Y = make_dataset(n_domains=a.domains, size=a.size, iters=a.iters, log=log)
-        res["data"] = {"tiles": int(X.shape[0]), "tile": int(X.shape[-1]), "domains": a.domains, "domain_size": a.size,
-                       "solver_iterations": a.iters, "seconds": round(time.time() - t0, 1)}
+        if a.tiles:
+            X, Y, recs = make_dataset_tiles(n_tiles=a.tiles, max_iter=a.tile_iters, tol=a.tol, frac=a.frac, log=log)
+            res["data"] = {"tiles": int(X.shape[0]), "tile": int(X.shape[-1]), "grids": recs,
+                           "seconds": round(time.time() - t0, 1)}
+        else:
+            X, Y = make_dataset(n_domains=a.domains, size=a.size, iters=a.iters, log=log)
+            res["data"] = {"tiles": int(X.shape[0]), "tile": int(X.shape[-1]), "domains": a.domains,
+                           "domain_size": a.size, "solver_iterations": a.iters, "seconds": round(time.time() - t0, 1)}
         n_val = max(X.shape[0] // 8, 1)
         t0 = time.time()
         losses = train(model, X[n_val:], Y[n_val:], epochs=a.epochs, log=log)

@@ -125,6 +125,75 @@ def make_dataset(n_domains: int = 8, size: int = 1024, tile: int = 256, iters: i
     return torch.stack(X).contiguous(), torch.stack(Y).contiguous()
 
 
+def make_dataset_tiles(n_tiles: int = 2000, tile: int = 256, per_side: int = 10, seed: int = 505,
+                       max_iter: int = 8000, tol: float = 1e-6, frac: float = 0.5, update: str = "single",
+                       log=print):
+    """Training set of independent tile x tile DEMs (config-2 style: beta
+    2.0, relief U[300, 1000] m, pits in proportion to the area), each solved
+    by the plain GraphFlood solver to the steady-state test of solve() (at
+    most frac of the cells change by more than tol over 100 iterations, or
+    max_iter iterations).  per_side^2 tiles are solved at once on one grid in
+    custom boundary mode: each tile is framed by cells that drain out (code
+    3, a 'normal' tile of its own) and separated from its neighbours by a row
+    and column of no-data cells (code 0), so the tiles do not interact.
+    Rainfall U[5e-6, 5e-5] m/s and n U[0.02, 0.06] are drawn per grid.  The
+    solve starts from the analytic first guess of _planes (the state the
+    network corrects), Q from the accumulated rainfall.  Returns planes (S,
+    6, tile, tile), targets (S, tile, tile) and per-grid records."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    pits = int(round(2000 * tile * tile / 4096.0**2))
+    S = per_side * (tile + 1) + 1
+    X, Y, recs = [], [], []
+    done = 0
+    while done < n_tiles:
+        k = min(per_side * per_side, n_tiles - done)
+        rain = float(rng.uniform(5e-6, 5e-5))
+        man = float(rng.uniform(0.02, 0.06))
+        zc = np.zeros((S, S), np.float32)
+        codes = np.zeros((S, S), np.uint8)
+        h0 = np.zeros((S, S), np.float32)
+        slots, bases = [], []
+        t0 = time.time()
+        for t in range(k):
+            ty, tx = divmod(t, per_side)
+            y0, x0 = 1 + ty * (tile + 1), 1 + tx * (tile + 1)
+            z = synth.dem(tile, tile, 2.0, float(rng.uniform(300.0, 1000.0)), int(rng.integers(0, 2**31 - 1)),
+                          n_pits=pits, dx=10.0)
+            zc[y0:y0 + tile, x0:x0 + tile] = z
+            c = codes[y0:y0 + tile, x0:x0 + tile]
+            c[:] = 1
+            c[0, :] = c[-1, :] = 3
+            c[:, 0] = c[:, -1] = 3
+            r = FlowRouter(tile, tile, 10.0, boundary_mode="normal")
+            r.set_z(z)
+            planes, base = _planes(r, rain, man, with_base=True)
+            X.append(planes[0])
+            bases.append(base)
+            h0[y0:y0 + tile, x0:x0 + tile] = depth_of(base).cpu().numpy()
+            slots.append((y0, x0))
+            del r
+        router = FlowRouter(S, S, 10.0, boundary_mode="custom", boundaries=codes)
+        fl = Flooder(router, precipitation_rates=rain, manning=man, edge_slope=1e-2, update=update)
+        _reset_start(router, fl, zc, torch.from_numpy(h0).cuda(), rain)
+        its = 0
+        while its < max_iter:
+            st = fl.run_to_steady(tol=tol, check_every=100, max_iter=min(2000, max_iter - its), frac=frac)
+            its += int(st.iterations)
+            if st.converged or st.nan_cells:
+                break
+        torch.cuda.synchronize()
+        h = fl.h.data.view(S, S)
+        for (y0, x0), base in zip(slots, bases):
+            Y.append(target_of(h[y0:y0 + tile, x0:x0 + tile]) - base)
+        rec = {"tiles": k, "rain": rain, "manning": man, "iterations": its, "converged": bool(st.converged),
+               "above_frac": float(st.above_frac), "q50": float(st.q50), "seconds": round(time.time() - t0, 1)}
+        recs.append(rec)
+        log(f"[surrogate] grid {len(recs)}: {rec}")
+        done += k
+        del fl, router
+    return torch.stack(X).contiguous(), torch.stack(Y).contiguous(), recs
+
+
 def train(model: UNet, X: torch.Tensor, Y: torch.Tensor, epochs: int = 20, batch: int = 8, lr: float = 2e-3,
           seed: int = 0, log=print):
     """Adam on the mean squared error of log depth, with random flips and

@@ -40,6 +40,24 @@ def _codes(ny, nx, kind):
     return b
 
 
+def _check_outlet_log(trips, lk):
+    """Per-trip outlet keys and saddle nodes of every basin against the
+    oracle's log; basins not logged hold the sentinel key and no node."""
+    inv_key = 0x4CBEBC200000002A  # key(1e8f, 42), lakeflow.py:127
+    assert len(trips) == lk["n_iter"]
+    for t, (b, o, sn) in enumerate(trips):
+        ol, snl = lk["outlet_log"][t], lk["saddlenode_log"][t]
+        # basin N has no slot in the oracle's N-long log: no outlet, no node
+        slot = b < ol.size
+        assert (o[~slot] == inv_key).all() and (sn[~slot] == -1).all()
+        b, o, sn = b[slot], o[slot], sn[slot]
+        np.testing.assert_array_equal(o, ol[b])
+        np.testing.assert_array_equal(sn, snl[b])
+        rest = np.ones(ol.size, bool)
+        rest[b] = False
+        assert (ol[rest] == inv_key).all() and (snl[rest] == -1).all()
+
+
 def _pit_last_cell(z):
     """Make the last cell a local minimum (a closed pit with basin id N)."""
     z = z.copy()
@@ -64,16 +82,86 @@ def test_custom_routing_pipeline(pf, ora, kind, ny, nx, seed, pits):
     if kind == "north_outlet":
         assert rcv[-1] == ny * nx - 1  # the last cell is a pit: its basin id is N
     bf = torch.empty(nx * ny, dtype=torch.int32, device="cuda")
-    r.reroute_flow(carve=True, bid_first=bf)
-    lk = ora.reroute_flow(g, z, rcv, carve=True)
+    st = r.reroute_flow(carve=True, bid_first=bf, log_outlets=True)
+    lk = ora.reroute_flow(g, z, rcv, carve=True, log_iters=32)
     np.testing.assert_array_equal(bf.cpu().numpy(), lk["bid_first"])
     if kind == "north_outlet":
         assert lk["bid_first"][-1] == ny * nx
+    # depressions per trip: basin N (a pit on cell N-1, no per-basin slot, so
+    # never an outlet) is counted in every trip, as are no-data cells
+    assert [int(st.ndep[i]) for i in range(st.n_loop)] == lk["ndep"]
+    _check_outlet_log(r.last_lake_outlets, lk)
     np.testing.assert_array_equal(r.receivers.to_numpy(), lk["rec"])
     assert r.accumulate_constant_Q(1e-5, area=True)
     cnt = ora.upstream_count(g, lk["rec"])
     q0 = np.float32(1e-5 * 10.0 * 10.0)
     np.testing.assert_array_equal(r.Q.to_numpy(), q0 * cnt.astype(np.float32))
+
+
+def _tile_canvas(per_side, tile, seed):
+    """per_side^2 independent tile x tile DEMs on one custom-mode grid: each
+    tile framed by outlet cells (code 3), tiles separated by one row and
+    column of no-data cells (code 0).  The grid's first row, hence cell 42,
+    and its last cell are no-data pits (the last one is basin N)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    S = per_side * (tile + 1) + 1
+    zc = np.zeros((S, S), np.float32)
+    codes = np.zeros((S, S), np.uint8)
+    for t in range(per_side * per_side):
+        ty, tx = divmod(t, per_side)
+        y0, x0 = 1 + ty * (tile + 1), 1 + tx * (tile + 1)
+        zc[y0:y0 + tile, x0:x0 + tile] = synth.dem(tile, tile, 2.0, float(rng.uniform(300.0, 1000.0)),
+                                                   int(rng.integers(0, 2**31 - 1)), n_pits=8, dx=10.0)
+        c = codes[y0:y0 + tile, x0:x0 + tile]
+        c[:] = 1
+        c[0, :] = c[-1, :] = 3
+        c[:, 0] = c[:, -1] = 3
+    return zc, codes
+
+
+def test_custom_tile_canvas(pf, ora):
+    """Routing and GraphFlood on a canvas of 3 x 3 independent 256^2 tiles
+    in custom mode (the layout that solves many training tiles at once):
+    isolated no-data pits that never get a saddle, cell 42 one of them, and
+    basin N on the last cell.  Bit-exact with the oracle: receivers, first
+    labels, depressions and outlets per trip, carved receivers, Q; then 3
+    GraphFlood iterations (single depth update: the reference's double
+    update already gives NaN depths on outlet cells here by the third
+    iteration) against the f32 oracle."""
+    zc, codes = _tile_canvas(3, 256, 505)
+    S = zc.shape[0]
+    N = S * S
+    print(f"case custom tile canvas {S}x{S}", flush=True)
+    r = pf.flow.FlowRouter(S, S, 10.0, boundary_mode="custom", boundaries=codes)
+    r.set_z(zc)
+    r.compute_receivers()
+    g = ora.Grid(S, S, 10.0, "custom", codes)
+    rcv, _ = ora.compute_receivers(g, zc)
+    np.testing.assert_array_equal(r.receivers.to_numpy(), rcv)
+    assert rcv[42] == 42 and rcv[N - 1] == N - 1  # no-data pits
+    bf = torch.empty(N, dtype=torch.int32, device="cuda")
+    st = r.reroute_flow(carve=True, bid_first=bf, log_outlets=True)
+    lk = ora.reroute_flow(g, zc, rcv, carve=True, log_iters=32)
+    np.testing.assert_array_equal(bf.cpu().numpy(), lk["bid_first"])
+    assert lk["bid_first"][N - 1] == N
+    assert [int(st.ndep[i]) for i in range(st.n_loop)] == lk["ndep"]
+    _check_outlet_log(r.last_lake_outlets, lk)
+    np.testing.assert_array_equal(r.receivers.to_numpy(), lk["rec"])
+    assert r.accumulate_constant_Q(1e-5, area=True)
+    cnt = ora.upstream_count(g, lk["rec"])
+    np.testing.assert_array_equal(r.Q.to_numpy(), np.float32(1e-5 * 100.0) * cnt.astype(np.float32))
+    # GraphFlood from a cold start on the same canvas
+    r.set_z(zc)
+    r.Q.fill(0.0)
+    fl = pf.graphflood.Flooder(r, precipitation_rates=2e-5, manning=0.04, edge_slope=1e-2, update="single")
+    hy = ora.hydro(prec=2e-5, manning=0.04, edge_slope=1e-2, dt=5e-3, update=1)
+    zh, h, Q = zc.ravel().copy(), np.zeros(N, np.float32), np.zeros(N, np.float32)
+    fl.run_graphflood(3)
+    for _ in range(3):
+        zh, h, Q, rec = ora.graphflood_iteration(g, hy, zh, h, Q)
+    np.testing.assert_array_equal(r.receivers.to_numpy(), rec)
+    np.testing.assert_array_equal(fl.h.to_numpy().view(np.uint32), h.view(np.uint32))
+    np.testing.assert_array_equal(r.Q.to_numpy().view(np.uint32), Q.view(np.uint32))
 
 
 @pytest.mark.parametrize("kind", ["north_outlet", "outlets_no_wrap"])
