@@ -155,6 +155,10 @@ def graphflood_block(pf, _lib, synth, s, args):
         out["fp64_drift"] = gpu_drift(pf, synth, n_small, it_small)
     # towards steady state: cold start (h = 0, Q = 0), reference update,
     # max |h(t) - h(t-100)| < 1e-6 m checked every 100 iterations
+    if args.steady_budget <= 0:
+        out["steady_state"] = None
+        torch.cuda.empty_cache()
+        return out
     rg = pf.flow.FlowRouter(cg["nx"], cg["ny"], cg["dx"], boundary_mode="normal")
     rg.set_z(cg["z"])
     fl = pf.graphflood.Flooder(rg, precipitation_rates=cg["precipitation"], manning=cg["manning"], edge_slope=1e-2)
@@ -422,6 +426,7 @@ def main():
     #      routed together (receivers + reroute_flow + accumulate per tile)
     batch = None
     c4 = None
+    batch_out = None
     if not args.no_batch:
         c4 = synth.ragged_batch(256)
         tiles = [t["z"] for t in c4]
@@ -437,10 +442,23 @@ def main():
         eb[1].record(s)
         torch.cuda.synchronize()
         bms = eb[0].elapsed_time(eb[1]) / reps
+        # the dominant kernel of one more batch call, timed live as above
+        _lib.call("pff_ctx_profile", ctx, 1)
+        res = pf.flow.route_batch(zt, dx=10.0, value=1e-5, out=res)
+        bk = kernel_report(_lib, ctx, ("k_flag_scatter2", "k_zprime_min_t", "k_tag_local", "k_tag_final", "k_tile"))
+        _lib.call("pff_ctx_profile", ctx, 0)
+        torch.cuda.synchronize()
+        bdom = max(bk, key=lambda k: bk[k]["launches"] * bk[k]["avg_us"])
         batch = {"workload": "256 tiles, sides round(128*8^u), u~U[0,1), seed 404, normal, dx=10 m",
                  "cells": int(bcells), "ms": round(bms, 3), "Mcells_per_s": round(bcells / (bms * 1e-3) / 1e6, 1),
-                 "status": res.status}
+                 "status": res.status,
+                 "roofline": {"bound": "hbm", "kernel": bdom, "achieved": bk[bdom]["achieved_GBs"],
+                              "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": bk[bdom]["frac_of_hbm_peak"],
+                              "launches_per_call": bk[bdom]["launches"], "avg_launch_us": bk[bdom]["avg_us"],
+                              "bytes_per_launch": bk[bdom]["bytes_per_launch"]},
+                 "instrumented_kernels": bk}
         log(f"[bench] batch {batch}")
+        batch_out = ([r.cpu().numpy() for r in res.receivers], [q.cpu().numpy() for q in res.Q])
         del zt, res
 
     # ---- GraphFlood: 2048^2 open-edge DEM, cold start, run_graphflood
@@ -534,12 +552,25 @@ def main():
             parity["div_ambiguity"]["graphflood_2048"] = {"receivers": O.receivers_div_ambiguity(g3, cg["z"]),
                                                           "gradient_bits": O.gradient_div_ambiguity(g3, cg["z"])}
         if batch is not None:
-            ra, gb = 0, 0
-            for t in c4:
+            # every tile of the batch against the oracle's single-grid run
+            ra, gb, bad_r, bad_q, worst = 0, 0, 0, 0, 0.0
+            q4 = np.float32(1e-5 * 10.0 * 10.0)
+            t0 = time.perf_counter()
+            for k, t in enumerate(c4):
                 g4 = O.Grid(t["nx"], t["ny"], t["dx"], 0)
                 ra += O.receivers_div_ambiguity(g4, t["z"])
                 gb += O.gradient_div_ambiguity(g4, t["z"])
+                r4, _ = O.compute_receivers(g4, t["z"])
+                l4 = O.reroute_flow(g4, t["z"], r4, carve=True)
+                c4n = O.upstream_count(g4, l4["rec"])
+                bad_r += int(not np.array_equal(batch_out[0][k], l4["rec"]))
+                bad_q += int(not np.array_equal(batch_out[1][k], q4 * c4n.astype(np.float32)))
+                worst = max(worst, float((np.abs(batch_out[1][k].astype(np.float64) - c4n * float(q4))
+                                          / (c4n * float(q4))).max()))
             parity["div_ambiguity"]["ragged_256"] = {"receivers": ra, "gradient_bits": gb}
+            parity["ragged_256"] = {"tiles": len(c4), "tiles_receivers_differ": bad_r, "tiles_Q_differ": bad_q,
+                                    "Q_max_rel_err_vs_f64": worst,
+                                    "oracle_seconds": round(time.perf_counter() - t0, 1)}
         log(f"[bench] parity {parity}")
 
     out = {

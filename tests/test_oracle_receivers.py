@@ -76,6 +76,18 @@ def test_stochastic_draws_are_uniform(ora):
     assert abs((u < 0.25).mean() - 0.25) < 5 * np.sqrt(0.25 * 0.75 / n)
 
 
+def test_stochastic_draws_follow_splitmix64(ora):
+    """Draw k of cell i with seed s is the top 24 bits of splitmix64 at the
+    state s + (4i + k) * 0x9E3779B97F4A7C15.  Seed 0, cell 0, k = 1, 2, 3 are
+    therefore splitmix64's first three outputs from state 0, published as
+    0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F (Vigna's
+    splitmix64.c, Java's SplittableRandom)."""
+    for k, out in ((1, 0xE220A8397B1DCDAF), (2, 0x6E789E6AA1B965F4), (3, 0x06C45D188009454F)):
+        assert ora.uniform01(0, 0, k) == (out >> 40) * 2.0**-24
+    # k = 0 at seed 0 is the state 0 itself: splitmix64's finaliser maps 0 to 0
+    assert ora.uniform01(0, 0, 0) == 0.0
+
+
 def test_stochastic_receivers_two_slope_probability(ora):
     """On the plane z = -(2 row + col) every interior cell has two lower
     neighbours, bottom (slope 2/dx) and right (1/dx).  With the slopes
