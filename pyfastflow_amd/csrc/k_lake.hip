@@ -755,39 +755,86 @@ __global__ __launch_bounds__(kBT) void k_saddle_node_list(GridD g, const int32_t
   }
 }
 
-// pass 4 (:226-250) over the pits
+// pass 4 (:226-250) for pit b: the lowest neighbour of its saddle node outside it
+template <int MODE>
+__device__ __forceinline__ void outlet_of_pit(const GridD& g, int b, const int32_t* __restrict__ bid,
+                                              const float* __restrict__ z, const int64_t* __restrict__ saddle,
+                                              const int32_t* __restrict__ snode, int64_t* __restrict__ outlet) {
+  if (b >= g.N || saddle[b] == INVALID_KEY) return;
+  const int node = snode[b];
+  if (node < 0) return;
+  int row, col;
+  rc_of(g, node, row, col);
+  float tz = 1e9f;
+  int rr = -1;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int jn = neighbour<MODE>(g, node, row, col, k);
+    if (jn < 0) continue;
+    if (bid[jn] != b && tz > z[jn]) {
+      tz = z[jn];
+      rr = jn;
+    }
+  }
+  if (rr > -1) {
+    const int64_t c = pack_key(tz, rr);
+    if (c < outlet[b]) outlet[b] = c;
+  }
+}
+
+// pass 5 (:255-281), first phase for pit b: 1 when b and the basin of its
+// outlet drain into each other and that basin has the lower id
+__device__ __forceinline__ uint8_t cycle_flag_of(int32_t N, int b, const int32_t* __restrict__ bid,
+                                                 const int64_t* __restrict__ outlet, const uint8_t* __restrict__ f42) {
+  if (outlet[b] == INVALID_KEY) return 0;
+  const int r1 = key_index(outlet[b]);
+  const int b1 = (r1 >= 0 && r1 < N) ? bid[r1] : -1;
+  if (b1 <= 0 || b1 >= N) return 0;
+  const int b2 = outlet_basin(N, bid, f42, outlet[b1], b);
+  return (b2 == b && b1 < b) ? 1 : 0;
+}
+
+// second phase for pit b: remove the flagged outlets (and the basin holding
+// cell 42 when it drains into a lower flagged basin); set the bit of every
+// remaining saddle node in the tile-ordered bitmap wbits (128 words per
+// 64x64 tile) for the carve's tag walks
+__device__ __forceinline__ void cycle_apply_of(int32_t N, int b, const int32_t* __restrict__ bid, int64_t* outlet,
+                                               int64_t* saddle, int32_t* snode, const uint8_t* __restrict__ inv,
+                                               const uint8_t* __restrict__ f42, int32_t nx, int tiles_x,
+                                               uint32_t* wbits) {
+  bool kill = inv[b] != 0;
+  if (!kill && holds42(N, bid, f42, b) && outlet[b] != INVALID_KEY) {
+    const int r1 = key_index(outlet[b]);
+    const int b1 = (r1 >= 0 && r1 < N) ? bid[r1] : -1;
+    if (b1 > 0 && b1 < b && inv[b1]) kill = true;
+  }
+  if (kill) {
+    outlet[b] = INVALID_KEY;
+    saddle[b] = INVALID_KEY;
+    snode[b] = -1;
+  } else {
+    const int sn = snode[b];
+    if (sn >= 0 && sn < N) {
+      const int row = sn / nx, col = sn - row * nx;
+      const int li = (row % TSL) * TSL + col % TSL;
+      const size_t word = (size_t)((row / TSL) * tiles_x + col / TSL) * (TSL * TSL / 32) + (li >> 5);
+      atomicOr(&wbits[word], 1u << (li & 31));
+    }
+  }
+}
+
+// pass 4 over the pits
 template <int MODE>
 __global__ __launch_bounds__(kBT) void k_outlet_list(GridD g, const int32_t* __restrict__ pits, const int64_t* np,
                                                      const int32_t* __restrict__ bid, const float* __restrict__ z,
                                                      const int64_t* __restrict__ saddle,
                                                      const int32_t* __restrict__ snode, int64_t* __restrict__ outlet) {
   const int64_t n = *np;
-  for (int64_t w = (int64_t)blockIdx.x * kBT + threadIdx.x; w < n; w += (int64_t)gridDim.x * kBT) {
-    const int b = pits[w];
-    if (b >= g.N || saddle[b] == INVALID_KEY) continue;
-    const int node = snode[b];
-    if (node < 0) continue;
-    int row, col;
-    rc_of(g, node, row, col);
-    float tz = 1e9f;
-    int rr = -1;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int jn = neighbour<MODE>(g, node, row, col, k);
-      if (jn < 0) continue;
-      if (bid[jn] != b && tz > z[jn]) {
-        tz = z[jn];
-        rr = jn;
-      }
-    }
-    if (rr > -1) {
-      const int64_t c = pack_key(tz, rr);
-      if (c < outlet[b]) outlet[b] = c;
-    }
-  }
+  for (int64_t w = (int64_t)blockIdx.x * kBT + threadIdx.x; w < n; w += (int64_t)gridDim.x * kBT)
+    outlet_of_pit<MODE>(g, pits[w], bid, z, saddle, snode, outlet);
 }
 
-// pass 5 (:255-281) over the pits, in the two phases of k_cycle_flag/apply
+// pass 5 over the pits, in two kernels (every flag is read after all are set)
 __global__ __launch_bounds__(kBT) void k_cycle_flag_list(int32_t N, const int32_t* __restrict__ pits, const int64_t* np,
                                                          const int32_t* __restrict__ bid,
                                                          const int64_t* __restrict__ outlet, uint8_t* __restrict__ inv,
@@ -795,22 +842,10 @@ __global__ __launch_bounds__(kBT) void k_cycle_flag_list(int32_t N, const int32_
   const int64_t n = *np;
   for (int64_t w = (int64_t)blockIdx.x * kBT + threadIdx.x; w < n; w += (int64_t)gridDim.x * kBT) {
     const int b = pits[w];
-    if (b >= N) continue;
-    uint8_t f = 0;
-    if (outlet[b] != INVALID_KEY) {
-      const int r1 = key_index(outlet[b]);
-      const int b1 = (r1 >= 0 && r1 < N) ? bid[r1] : -1;
-      if (b1 > 0 && b1 < N) {
-        const int b2 = outlet_basin(N, bid, f42, outlet[b1], b);
-        if (b2 == b && b1 < b) f = 1;
-      }
-    }
-    inv[b] = f;
+    if (b < N) inv[b] = cycle_flag_of(N, b, bid, outlet, f42);
   }
 }
 
-// (also sets the bit of every remaining saddle node in the tile-ordered
-// bitmap wbits, 128 words per 64x64 tile, for the carve's tag walks)
 __global__ __launch_bounds__(kBT) void k_cycle_apply_list(int32_t N, const int32_t* __restrict__ pits,
                                                           const int64_t* np, const int32_t* __restrict__ bid,
                                                           int64_t* outlet, int64_t* saddle, int32_t* snode,
@@ -820,26 +855,7 @@ __global__ __launch_bounds__(kBT) void k_cycle_apply_list(int32_t N, const int32
   const int64_t n = *np;
   for (int64_t w = (int64_t)blockIdx.x * kBT + threadIdx.x; w < n; w += (int64_t)gridDim.x * kBT) {
     const int b = pits[w];
-    if (b >= N) continue;
-    bool kill = inv[b] != 0;
-    if (!kill && holds42(N, bid, f42, b) && outlet[b] != INVALID_KEY) {
-      const int r1 = key_index(outlet[b]);
-      const int b1 = (r1 >= 0 && r1 < N) ? bid[r1] : -1;
-      if (b1 > 0 && b1 < b && inv[b1]) kill = true;
-    }
-    if (kill) {
-      outlet[b] = INVALID_KEY;
-      saddle[b] = INVALID_KEY;
-      snode[b] = -1;
-    } else {
-      const int sn = snode[b];
-      if (sn >= 0 && sn < N) {
-        const int row = sn / nx, col = sn - row * nx;
-        const int li = (row % TSL) * TSL + col % TSL;
-        const size_t word = (size_t)((row / TSL) * tiles_x + col / TSL) * (TSL * TSL / 32) + (li >> 5);
-        atomicOr(&wbits[word], 1u << (li & 31));
-      }
-    }
+    if (b < N) cycle_apply_of(N, b, bid, outlet, saddle, snode, inv, f42, nx, tiles_x, wbits);
   }
 }
 
@@ -944,13 +960,13 @@ int saddlesort_active(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t*
                                                (const int64_t*)NB[out], bid, (const float*)zp, (const int64_t*)saddle,
                                                snode));
   *border_cur = out;
+  uint32_t* wbits;
+  uint8_t* tag;
+  PFF_TRY(tag_state(ctx, g, &wbits, &tag, true));
   PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_outlet_list<M>, lg, bl, 0, s, g, pits, npits, bid, z,
                                                (const int64_t*)saddle, (const int32_t*)snode, outlet));
   hipLaunchKernelGGL(k_cycle_flag_list, lg, bl, 0, s, g.N, pits, npits, bid, (const int64_t*)outlet, (uint8_t*)inv,
                      g.F42);
-  uint32_t* wbits;
-  uint8_t* tag;
-  PFF_TRY(tag_state(ctx, g, &wbits, &tag, true));
   hipLaunchKernelGGL(k_cycle_apply_list, lg, bl, 0, s, g.N, pits, npits, bid, outlet, saddle, snode,
                      (const uint8_t*)inv, g.F42, g.nx, tiles_x, wbits);
   return check_hip(hipGetLastError(), "saddlesort_active");

@@ -38,3 +38,32 @@ def test_warm_start_gains_over_cold(pf):
     assert out["used"] == "warm"
     # ... and does not slow the solver down
     assert warm["iterations"] <= cold["iterations"]
+
+
+def test_warm_start_2048_held_out(pf):
+    """The warm-start configuration itself: a held-out 2048^2 DEM (config-2
+    style synthesis, relief 1000 m, 500 pits, seed 9091), rain 1e-5 m/s,
+    n = 0.033, single depth update.  Inference, then the cold and the warm
+    solve to the same test (at most half of the cells change by more than
+    1e-6 m over 100 iterations).  The warm start needs at most half of the
+    cold start's iterations, and the two stop states agree on the median
+    cell to 1 mm (they differ in lakes the cold run is still filling)."""
+    from pyfastflow_amd.surrogate import UNet
+    from pyfastflow_amd.surrogate.warmstart import WEIGHTS, load_weights, solve
+
+    if not os.path.exists(WEIGHTS):
+        pytest.skip("no trained weights committed")
+    model = load_weights(UNet(), WEIGHTS).cuda().eval()
+    n = 2048
+    z = synth.dem(n, n, 2.0, 1000.0, 9091, n_pits=500, dx=10.0)
+    print("case warm start 2048^2 held-out", flush=True)
+    out = solve(z, 1e-5, 0.033, model, tol=1e-6, frac=0.5, max_iter=40000, chunk=5000, update="single",
+                log=lambda *a: print(*a, flush=True))
+    cold, warm, agree = out["cold"], out["warm"], out["agreement"]
+    print(f"cold {cold['iterations']} it {cold['seconds']:.1f} s, warm {warm['iterations']} it "
+          f"{warm['seconds']:.1f} s; agreement {agree}", flush=True)
+    assert cold["converged"] and warm["converged"]
+    assert cold["nan_cells"] == 0 and warm["nan_cells"] == 0
+    assert out["used"] == "warm"
+    assert warm["iterations"] <= 0.5 * cold["iterations"]
+    assert agree["median_m"] <= 1e-3
