@@ -296,9 +296,9 @@ int reroute_core(pff_ctx* ctx, const GridD& d, int32_t* bid, int32_t* rec, int32
       exact = true;
       PFF_TRY(ensure_levels());
       PFF_TRY(labels_split(ctx, d, rec_, bid, rec__, levels, fin, rcvp, mark));
-      PFF_TRY(launch_saddlesort(ctx, d, bid, is_border, z_prime, basin_saddle, basin_saddlenode, outlet, z));
+      PFF_TRY(saddlesort_exact(ctx, d, bid, is_border, z_prime, basin_saddle, basin_saddlenode, outlet, z));
     } else {
-      PFF_TRY(launch_saddlesort(ctx, d, bid, is_border, z_prime, basin_saddle, basin_saddlenode, outlet, z));
+      PFF_TRY(saddlesort_exact(ctx, d, bid, is_border, z_prime, basin_saddle, basin_saddlenode, outlet, z));
       PFF_TRY(log_trip(ctx, d, bid, outlet, basin_saddlenode, st ? st->n_iter : it, true));
       if (st) st->n_iter++;
     }

@@ -9,23 +9,30 @@ namespace pff {
 // ---------------------------------------------------------------------------
 // depression_counter (lakeflow.py:24-44): #{i : rcv[i]==i and not can_leave(i)}
 // ---------------------------------------------------------------------------
+// A fixed grid strides over the cells; each wave counts its pits with a
+// ballot and adds them with one atomic when it found any (pits are sparse, so
+// few waves touch the counter).
 template <int MODE>
 __global__ __launch_bounds__(kBT) void k_depcount(GridD g, const int32_t* __restrict__ rcv, int64_t* out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int pit = 0;
-  if (i < g.N && rcv[i] == i) {
-    int row, col;
-    rc_of(g, i, row, col);
-    pit = can_leave<MODE>(g, i, row, col) ? 0 : 1;
+  int mine = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kBT + threadIdx.x; i < g.N; i += (int64_t)gridDim.x * kBT) {
+    if (rcv[i] == (int32_t)i) {
+      int row, col;
+      rc_of(g, (int)i, row, col);
+      mine += can_leave<MODE>(g, (int)i, row, col) ? 0 : 1;
+    }
   }
-  block_count(pit, out);
+  // wave sum of the per-lane counts (at most a few per lane)
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) mine += __shfl_xor(mine, d);
+  if ((threadIdx.x & 63) == 0 && mine) atomicAdd((unsigned long long*)out, (unsigned long long)mine);
 }
 
 int depression_count(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int64_t* out) {
   int64_t* w = ctx->dev_words + W_COUNT2;
   PFF_TRY(check_hip(hipMemsetAsync(w, 0, 8, ctx->stream), "memset"));
-  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_depcount<M>, dim3(grid_for(g.N)), dim3(kBlock), 0, ctx->stream, g,
-                                               rcv, w));
+  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_depcount<M>, dim3(grid_for(g.N, kBT, 2048)), dim3(kBT), 0,
+                                               ctx->stream, g, rcv, w));
   PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_COUNT2, w, 8, hipMemcpyDeviceToHost, ctx->stream), "d2h"));
   PFF_TRY(check_hip(hipStreamSynchronize(ctx->stream), "sync"));
   *out = ctx->host_words[W_COUNT2];
@@ -970,6 +977,29 @@ int saddlesort_active(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t*
   hipLaunchKernelGGL(k_cycle_apply_list, lg, bl, 0, s, g.N, pits, npits, bid, outlet, saddle, snode,
                      (const uint8_t*)inv, g.F42, g.nx, tiles_x, wbits);
   return check_hip(hipGetLastError(), "saddlesort_active");
+}
+
+// The five saddle passes over the whole grid for the driver's exact trips
+// (labels then need not be pit ids): as launch_saddlesort, with passes 1 and
+// 2 fused into the tiled z' pass and its per-tile minimum table.  Basin 0
+// (draining out) gets no saddle entry here; the driver never reads it.
+int saddlesort_exact(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t* isb, float* zp, int64_t* saddle,
+                     int32_t* snode, int64_t* outlet, const float* z) {
+  void* inv;
+  PFF_TRY(scratch(ctx, S_B0, (size_t)g.N, &inv));
+  const dim3 gr(grid_for(g.N)), bl(kBlock);
+  hipStream_t s = ctx->stream;
+  const int tiles_x = (g.nx + TSL - 1) / TSL;
+  const dim3 tt(tiles_x, (g.ny + TSL - 1) / TSL), cb(64, 4);
+  hipLaunchKernelGGL(k_saddle_init, gr, bl, 0, s, g.N, saddle, outlet, snode);
+  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_zprime_min_t<M>, tt, cb, 0, s, g, tiles_x, (const uint32_t*)nullptr,
+                                               bid, z, isb, zp, saddle, (int32_t*)nullptr, (int64_t*)nullptr,
+                                               (unsigned long long*)nullptr));
+  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_saddle_node<M>, gr, bl, 0, s, g, bid, isb, zp, saddle, snode));
+  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_outlet<M>, gr, bl, 0, s, g, bid, z, saddle, snode, outlet));
+  hipLaunchKernelGGL(k_cycle_flag, gr, bl, 0, s, g.N, bid, outlet, (uint8_t*)inv, g.F42);
+  hipLaunchKernelGGL(k_cycle_apply, gr, bl, 0, s, g.N, bid, outlet, saddle, snode, (const uint8_t*)inv, g.F42);
+  return check_hip(hipGetLastError(), "saddlesort_exact");
 }
 
 // tiled carve finalise (lakeflow.py:390-397): reverse the tagged links, over

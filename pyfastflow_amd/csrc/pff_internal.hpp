@@ -159,6 +159,8 @@ int launch_propagate_basin(pff_ctx* ctx, const GridD& g, const int32_t* bid_in, 
 int basin_labels(pff_ctx* ctx, const GridD& g, const int32_t* rec, int32_t* bid, int32_t* rec_out);
 int launch_saddlesort(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t* is_border, float* zp,
                       int64_t* saddle, int32_t* snode, int64_t* outlet, const float* z);
+int saddlesort_exact(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t* isb, float* zp, int64_t* saddle,
+                     int32_t* snode, int64_t* outlet, const float* z);
 // tiled contraction (forests; *cycle = 1 when the receivers are not a forest)
 int tiled_upstream(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int wmode, const int32_t* bid,
                    const int32_t* snode, float q0, float* Q, uint32_t* cnt, uint8_t* tag, const uint32_t* tact,
