@@ -105,12 +105,25 @@ __global__ __launch_bounds__(kPW* kPH) void k_batch_unpack(const TileD* __restri
   T.Q[local] = Qc[c];
 }
 
-// pits (basin ids) of the trip -> per-tile counts
+// pits (basin ids) of the trip -> per-tile counts.  The pit list comes in
+// canvas tile order, so neighbouring entries mostly belong to one batch tile:
+// each wave adds one count per run of equal tiles (the run's first lane adds
+// the run length), not one atomic per pit on a few hundred shared counters.
 __global__ __launch_bounds__(kBT) void k_batch_count(const int32_t* __restrict__ pits, const int64_t* np,
                                                      const int32_t* __restrict__ tmap, int32_t* __restrict__ cnt) {
   const int64_t n = *np;
-  for (int64_t w = (int64_t)blockIdx.x * kBT + threadIdx.x; w < n; w += (int64_t)gridDim.x * kBT)
-    atomicAdd(cnt + tmap[pits[w] - 1], 1);
+  const int lane = threadIdx.x & 63;
+  for (int64_t base = (int64_t)blockIdx.x * kBT; base < n; base += (int64_t)gridDim.x * kBT) {  // uniform per block
+    const int64_t w = base + threadIdx.x;
+    const int t = w < n ? tmap[pits[w] - 1] : -1;
+    const int tp = __shfl_up(t, 1);
+    const uint64_t heads = __ballot(lane == 0 || t != tp);
+    if (t >= 0 && ((heads >> lane) & 1ull)) {
+      const uint64_t later = lane < 63 ? heads >> (lane + 1) : 0ull;
+      const int len = later ? __builtin_ctzll(later) + 1 : 64 - lane;
+      atomicAdd(cnt + t, len);
+    }
+  }
 }
 
 // bid <- 0 on the cells of frozen tiles

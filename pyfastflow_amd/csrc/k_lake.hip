@@ -9,9 +9,9 @@ namespace pff {
 // ---------------------------------------------------------------------------
 // depression_counter (lakeflow.py:24-44): #{i : rcv[i]==i and not can_leave(i)}
 // ---------------------------------------------------------------------------
-// A fixed grid strides over the cells; each wave counts its pits with a
-// ballot and adds them with one atomic when it found any (pits are sparse, so
-// few waves touch the counter).
+// A fixed grid of 512 blocks strides over the cells; each block sums its
+// pits and adds them with one atomic (a returning or not, every atomic on the
+// one counter queues at its cache line: one per block, not one per pit).
 template <int MODE>
 __global__ __launch_bounds__(kBT) void k_depcount(GridD g, const int32_t* __restrict__ rcv, int64_t* out) {
   int mine = 0;
@@ -22,16 +22,13 @@ __global__ __launch_bounds__(kBT) void k_depcount(GridD g, const int32_t* __rest
       mine += can_leave<MODE>(g, (int)i, row, col) ? 0 : 1;
     }
   }
-  // wave sum of the per-lane counts (at most a few per lane)
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) mine += __shfl_xor(mine, d);
-  if ((threadIdx.x & 63) == 0 && mine) atomicAdd((unsigned long long*)out, (unsigned long long)mine);
+  block_count(mine, out);
 }
 
 int depression_count(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int64_t* out) {
   int64_t* w = ctx->dev_words + W_COUNT2;
   PFF_TRY(check_hip(hipMemsetAsync(w, 0, 8, ctx->stream), "memset"));
-  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_depcount<M>, dim3(grid_for(g.N, kBT, 2048)), dim3(kBT), 0,
+  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_depcount<M>, dim3(grid_for(g.N, kBT, 512)), dim3(kBT), 0,
                                                ctx->stream, g, rcv, w));
   PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_COUNT2, w, 8, hipMemcpyDeviceToHost, ctx->stream), "d2h"));
   PFF_TRY(check_hip(hipStreamSynchronize(ctx->stream), "sync"));
