@@ -392,12 +392,17 @@ def main():
     avg_us = 1e3 * t_tot / launches
     bytes_launch = b_tot / launches
     achieved = b_tot / (t_tot * 1e-3) / 1e9
-    traffic = None
+    # HBM bytes per launch of that kernel: not measured in this run (counters
+    # need a profiler run of their own); read from the stored rocprofv3
+    # FETCH_SIZE / WRITE_SIZE summary of the same workload
+    traffic, traffic_src = None, None
     tfile = os.path.join(HERE, "profiles", f"{dom}_traffic.json")
     if os.path.exists(tfile):
         traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
+        traffic_src = f"stored profile profiles/{dom}_traffic.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
+                "kernel": dom,
                 "launches_per_step": launches, "avg_launch_us": round(avg_us, 2),
                 "bytes_per_launch": round(bytes_launch), "event_overhead_us": round(ovh.value, 2)}
     log(f"[bench] roofline {roofline}")
