@@ -274,9 +274,86 @@ static int count_rounds(pff_ctx* ctx, Lists& L, uint32_t* S, int32_t* P, int32_t
   return check_hip(hipGetLastError(), "count rounds");
 }
 
+// All rounds [first_round, first_round + rounds) of k_label_gather2 +
+// k_label_scatter2 in one block, for short lists: per round the gather over
+// the whole list, a barrier, the scatter with the compaction of the live
+// nodes into the other list, a barrier.  The lists ping-pong between l0
+// (current on entry, count *c0) and l1; afterwards the current list is l0 if
+// rounds is even, else l1, and its count is in the matching word, as after
+// the two-kernel rounds.  Any list length is handled.  *record (nullable)
+// receives the length of the list on entry.
+constexpr int kLB = 1024;
+
+__global__ __launch_bounds__(kLB) void k_label_rounds_block(int32_t* l0, int32_t* l1, int64_t* c0, int64_t* c1,
+                                                            int32_t* bid, int32_t* P, int32_t* __restrict__ nb,
+                                                            int32_t* __restrict__ np, int32_t* levels, int32_t N,
+                                                            int32_t* fin, int first_round, int rounds,
+                                                            int64_t* first_empty, int64_t* self_loop,
+                                                            int64_t* record) {
+  int32_t* la = l0;
+  int32_t* lb = l1;
+  int64_t* cb = c1;
+  int64_t n = *c0;
+  if (record && threadIdx.x == 0) *record = n;
+  for (int k = first_round; k < first_round + rounds; ++k) {
+    if (n == 0 && first_empty && threadIdx.x == 0) atomicMin((long long*)first_empty, (long long)k);
+    for (int64_t w = threadIdx.x; w < n; w += kLB) {
+      const int m = la[w];
+      const int pm = P[m];
+      nb[w] = bid[pm];
+      const int q = P[pm];
+      np[w] = (q == pm) ? ~q : q;
+    }
+    __syncthreads();
+    int32_t* lvl = levels ? levels + (size_t)(k + 1) * N : nullptr;
+    int64_t out = 0;
+    for (int64_t base = 0; base < n; base += kLB) {  // uniform per block
+      const int64_t w = base + threadIdx.x;
+      bool take = false;
+      int m = 0;
+      if (w < n) {
+        m = la[w];
+        int q = np[w];
+        const bool done = q < 0;
+        if (done) q = ~q;
+        bid[m] = nb[w];
+        P[m] = q;
+        if (lvl) lvl[m] = q;
+        if (done && fin) fin[m] = k + 1;
+        if (!done && q == m && self_loop) *self_loop = 1;
+        take = !done;
+      }
+      int tot;
+      const int off = block_exclusive_scan<kLB>(take ? 1 : 0, &tot);
+      if (take) lb[out + off] = m;
+      out += tot;
+    }
+    if (threadIdx.x == 0) *cb = out;
+    int32_t* tl = la;
+    la = lb;
+    lb = tl;
+    cb = (cb == c1) ? c0 : c1;
+    n = out;
+    __syncthreads();
+  }
+}
+
+__global__ void k_copy_word(const int64_t* src, int64_t* dst) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *dst = *src;
+}
+
 static int label_rounds_run(pff_ctx* ctx, Lists& L, int32_t* bid, int32_t* P, int32_t* nb, int32_t* np,
                             int32_t* levels, int32_t N, int32_t* fin, int first_round, int rounds,
-                            int64_t* self_loop = nullptr) {
+                            int64_t* self_loop = nullptr, bool one_block = false, int64_t* record = nullptr) {
+  if (one_block) {
+    const int a = L.cur;
+    hipLaunchKernelGGL(k_label_rounds_block, dim3(1), dim3(kLB), 0, ctx->stream, L.l[a], L.l[1 - a], L.c[a],
+                       L.c[1 - a], bid, P, nb, np, levels, N, fin, first_round, rounds, ctx->dev_words + W_EMPTY,
+                       self_loop, record);
+    if (rounds & 1) L.cur = 1 - a;
+    return check_hip(hipGetLastError(), "label rounds (one block)");
+  }
+  if (record) hipLaunchKernelGGL(k_copy_word, dim3(1), dim3(64), 0, ctx->stream, (const int64_t*)L.c[L.cur], record);
   for (int k = first_round; k < first_round + rounds; ++k) {
     const int a = L.cur, b = 1 - L.cur;
     hipLaunchKernelGGL(k_label_gather2, dim3(list_grid()), dim3(kBT), 0, ctx->stream, L.l[a], L.c[a], bid, P, nb, np,
@@ -437,7 +514,7 @@ int labels_split(pff_ctx* ctx, const GridD& g, const int32_t* rec, int32_t* bid,
                                                ctx->stream, g, rec, bid, Pfinal, levels, fin, rcvp, mark, L.l[0],
                                                L.c[0]));
   return label_rounds_run(ctx, L, bid, Pfinal, (int32_t*)nbv, (int32_t*)npv, levels, g.N, fin, 0,
-                          ceil_log2i(g.N) + 1);
+                          ceil_log2i(g.N) + 1, nullptr, ctx->hint_marked <= kShortList, ctx->dev_words + W_HMARK);
 }
 
 // ---------------------------------------------------------------------------
@@ -1652,8 +1729,15 @@ int tiled_labels(pff_ctx* ctx, const GridD& g, const int32_t* rec, const uint32_
   const int cap = ceil_log2i(g.N) + 1;
   if (mark_cycles) {
     // chains that reach a root do so within cap rounds; the others keep
-    // kCycleLabel.  Nothing is read back.
-    PFF_TRY(label_rounds_run(ctx, L, b.bid2, b.P2, b.t0, b.t1, nullptr, g.N, nullptr, 0, cap));
+    // kCycleLabel.  Nothing is read back.  After the first rounds only the
+    // nodes on or draining into a cycle (and long chains) are left: when the
+    // last exact trip left few, the remaining rounds run in one block.
+    constexpr int kFirst = 6;
+    const int r0 = std::min(kFirst, cap);
+    PFF_TRY(label_rounds_run(ctx, L, b.bid2, b.P2, b.t0, b.t1, nullptr, g.N, nullptr, 0, r0));
+    const bool short_tail = ctx->hint_xlive <= kShortList;
+    PFF_TRY(label_rounds_run(ctx, L, b.bid2, b.P2, b.t0, b.t1, nullptr, g.N, nullptr, r0, cap - r0, nullptr,
+                             short_tail, ctx->dev_words + W_HXLIVE));
     return 0;
   }
   // start from the previous call's round count, then confirm on the host

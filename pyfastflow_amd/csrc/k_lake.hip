@@ -31,8 +31,19 @@ int depression_count(pff_ctx* ctx, const GridD& g, const int32_t* rcv, int64_t* 
   PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_depcount<M>, dim3(grid_for(g.N, kBT, 512)), dim3(kBT), 0,
                                                ctx->stream, g, rcv, w));
   PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_COUNT2, w, 8, hipMemcpyDeviceToHost, ctx->stream), "d2h"));
+  // the list lengths the last exact trip recorded, read with the same wait
+  static_assert(W_HXLIVE == W_HMARK + 1 && W_HCTAG == W_HMARK + 2, "hint words are adjacent");
+  PFF_TRY(check_hip(hipMemcpyAsync(ctx->host_words + W_HMARK, ctx->dev_words + W_HMARK, 24, hipMemcpyDeviceToHost,
+                                   ctx->stream), "d2h"));
   PFF_TRY(check_hip(hipStreamSynchronize(ctx->stream), "sync"));
   *out = ctx->host_words[W_COUNT2];
+  // words never written (no exact trip yet) read 0 from the context's
+  // zeroed block: keep the start values then
+  if (ctx->host_words[W_HMARK] > 0 || ctx->host_words[W_HXLIVE] > 0 || ctx->host_words[W_HCTAG] > 0) {
+    ctx->hint_marked = ctx->host_words[W_HMARK];
+    ctx->hint_xlive = ctx->host_words[W_HXLIVE];
+    ctx->hint_ctag = ctx->host_words[W_HCTAG];
+  }
   return 0;
 }
 
@@ -355,6 +366,47 @@ __global__ __launch_bounds__(kBT) void k_carve_round(int32_t N, const int32_t* _
   }
 }
 
+// The rounds t = 0 .. rounds-1 of k_carve_prep + k_carve_round in one block,
+// for a short tag list: the same words w (tagged count, count before this
+// round, count before the previous round), the same list, stopped at the
+// first round that tags nothing (every later round would return at once).
+// Any list length is handled.  *record (nullable) receives the final count.
+constexpr int kCB = 1024;
+
+__global__ __launch_bounds__(kCB) void k_carve_rounds_block(int32_t N, const int32_t* __restrict__ levels,
+                                                            const int32_t* __restrict__ fin,
+                                                            const int32_t* __restrict__ Pfinal, uint32_t* tagw,
+                                                            int32_t* list, int64_t* w, int rounds, int64_t* record) {
+  int64_t c0 = w[0], c1 = w[1], c2 = w[2];
+  for (int t = 0; t < rounds; ++t) {
+    c2 = c1;
+    c1 = c0;
+    const int64_t n = c1;
+    if (t > 0 && n == c2) break;  // the previous round tagged nothing: converged
+    for (int64_t base = 0; base < n; base += kCB) {  // uniform per block
+      const int64_t e = base + threadIdx.x;
+      bool take = false;
+      int jj = 0;
+      if (e < n) {
+        const int i = list[e];
+        jj = jump_level(levels, fin, Pfinal, N, t, i);
+        if (tagw[jj] == 0u) take = atomicCAS(&tagw[jj], 0u, 1u) == 0u;
+      }
+      int tot;
+      const int off = block_exclusive_scan<kCB>(take ? 1 : 0, &tot);
+      if (take) list[c0 + off] = jj;
+      c0 += tot;
+    }
+    __syncthreads();  // this round's tags and list entries before the next round reads them
+  }
+  if (threadIdx.x == 0) {
+    w[0] = c0;
+    w[1] = c0;
+    w[2] = c0;
+    if (record) *record = c0;
+  }
+}
+
 // finalise_reroute_carve (:390-397): reverse every tagged link i -> rec_in[i].
 // On a receiver graph with a cycle a tagged cell can have two tagged donors;
 // the reference's parallel loop then keeps one of them, the rule here is the
@@ -422,10 +474,17 @@ int carve_from_levels(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t*
   hipStream_t s = ctx->stream;
   hipLaunchKernelGGL(k_carve_seed, dim3((g.N + 16 * kBT - 1) / (16 * kBT)), dim3(kBT), 0, s, g.N, snode, (uint32_t*)tagw,
                      (int32_t*)list, w, (const uint8_t*)nullptr);
-  for (int t = 0; t <= npass; ++t) {
-    hipLaunchKernelGGL(k_carve_prep, dim3(1), dim3(1), 0, s, w);
-    hipLaunchKernelGGL(k_carve_round, dim3(stride_grid()), dim3(kBT), 0, s, g.N, levels, fin, Pfinal,
-                       (uint32_t*)tagw, (int32_t*)list, w, t);
+  if (ctx->hint_ctag <= kShortList) {
+    // the last exact trip's carve tagged few cells: all rounds in one block
+    hipLaunchKernelGGL(k_carve_rounds_block, dim3(1), dim3(kCB), 0, s, g.N, levels, fin, Pfinal, (uint32_t*)tagw,
+                       (int32_t*)list, w, npass + 1, ctx->dev_words + W_HCTAG);
+  } else {
+    for (int t = 0; t <= npass; ++t) {
+      hipLaunchKernelGGL(k_carve_prep, dim3(1), dim3(1), 0, s, w);
+      hipLaunchKernelGGL(k_carve_round, dim3(stride_grid()), dim3(kBT), 0, s, g.N, levels, fin, Pfinal,
+                         (uint32_t*)tagw, (int32_t*)list, w, t);
+    }
+    PFF_TRY(check_hip(hipMemcpyAsync(ctx->dev_words + W_HCTAG, w, 8, hipMemcpyDeviceToDevice, s), "copy"));
   }
   hipLaunchKernelGGL(k_carve_reverse, dim3(stride_grid()), dim3(kBlock), 0, s, g.N, (const int32_t*)list, w,
                      (uint32_t*)tagw, rec_in);
@@ -1115,10 +1174,17 @@ int carve_split(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t* rec_i
   PFF_TRY(check_hip(hipMemsetAsync(w, 0, 3 * 8, s), "memset"));
   hipLaunchKernelGGL(k_carve_seed, dim3((g.N + 16 * kBT - 1) / (16 * kBT)), dim3(kBT), 0, s, g.N, snode,
                      (uint32_t*)tagw, (int32_t*)list, w, mark);
-  for (int t = 0; t <= npass; ++t) {
-    hipLaunchKernelGGL(k_carve_prep, dim3(1), dim3(1), 0, s, w);
-    hipLaunchKernelGGL(k_carve_round, dim3(stride_grid()), dim3(kBT), 0, s, g.N, levels, fin, Pfinal,
-                       (uint32_t*)tagw, (int32_t*)list, w, t);
+  if (ctx->hint_ctag <= kShortList) {
+    // the last exact trip's carve tagged few cells: all rounds in one block
+    hipLaunchKernelGGL(k_carve_rounds_block, dim3(1), dim3(kCB), 0, s, g.N, levels, fin, Pfinal, (uint32_t*)tagw,
+                       (int32_t*)list, w, npass + 1, ctx->dev_words + W_HCTAG);
+  } else {
+    for (int t = 0; t <= npass; ++t) {
+      hipLaunchKernelGGL(k_carve_prep, dim3(1), dim3(1), 0, s, w);
+      hipLaunchKernelGGL(k_carve_round, dim3(stride_grid()), dim3(kBT), 0, s, g.N, levels, fin, Pfinal,
+                         (uint32_t*)tagw, (int32_t*)list, w, t);
+    }
+    PFF_TRY(check_hip(hipMemcpyAsync(ctx->dev_words + W_HCTAG, w, 8, hipMemcpyDeviceToDevice, s), "copy"));
   }
   hipLaunchKernelGGL(k_carve_reverse, dim3(stride_grid()), dim3(kBlock), 0, s, g.N, (const int32_t*)list, w,
                      (uint32_t*)tagw, rec_in);

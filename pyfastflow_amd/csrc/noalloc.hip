@@ -66,7 +66,8 @@ extern "C" int pff_accumulate_constant_Q_noalloc(void* hip_stream, const pff_gri
   int slots[32];
   size_t bytes[32];
   int nslots = 0;
-  const size_t words = 256;
+  constexpr size_t words = 256;
+  static_assert(W_NWORDS * sizeof(int64_t) <= words, "device words do not fit their window");
   const size_t need = words + tiled_upstream_slots(d, slots, bytes, &nslots);
   if (need > kNoAllocBytes)
     return fail(PFF_EINVAL, "accumulate_constant_Q_noalloc: a " + std::to_string(d.nx) + " x " +

@@ -28,6 +28,12 @@ struct pff_ctx {
   int last_count_rounds = 8;
   int last_tag_rounds = 8;
   int last_jump_rounds = 4;
+  // list lengths of the last exact trip (cells on or draining into a receiver
+  // cycle, reduced-graph nodes still live after the first rounds of the
+  // cycle-marking labels, tagged cells of the carve's level rounds), read at
+  // the next exact trip's depression count; short lists run their rounds in
+  // one block (one launch instead of two per round)
+  int64_t hint_marked = int64_t(1) << 40, hint_xlive = int64_t(1) << 40, hint_ctag = int64_t(1) << 40;
   // the saddle-node bitmap and the carve tag bytes are all zero between
   // carves (each carve clears what it set); a carve that stopped half-way
   // leaves this set and the next one clears both in full
@@ -124,7 +130,10 @@ enum Slot : int {
 static_assert(S_NSLOTS <= pff_ctx::kSlots, "pff_ctx::kSlots must cover every scratch slot");
 
 // device word indices (dev_words / host_words)
-enum Word : int { W_COUNT0 = 0, W_COUNT1, W_COUNT2, W_FLAG, W_FLAG2, W_CYCLE, W_NEXIT, W_NDEP, W_NPIT, W_LIVE, W_EMPTY, W_GFMAX, W_GFNAN, W_NPIT2, W_MCHG, W_EMPTY2, W_GFABOVE, W_GFSUM, W_NBL0, W_NBL1, W_NTAG, W_NPREV, W_LAST_WORD, W_NWORDS = 24 };
+// lists up to this length run their pointer-jumping rounds in one block
+constexpr int64_t kShortList = 16384;
+
+enum Word : int { W_COUNT0 = 0, W_COUNT1, W_COUNT2, W_FLAG, W_FLAG2, W_CYCLE, W_NEXIT, W_NDEP, W_NPIT, W_LIVE, W_EMPTY, W_GFMAX, W_GFNAN, W_NPIT2, W_MCHG, W_EMPTY2, W_GFABOVE, W_GFSUM, W_NBL0, W_NBL1, W_NTAG, W_NPREV, W_HMARK, W_HXLIVE, W_HCTAG, W_LAST_WORD, W_NWORDS = 32 };
 static_assert(W_LAST_WORD <= W_NWORDS, "device word block too small");
 
 constexpr int kBlock = 256;
