@@ -99,12 +99,25 @@ __global__ __launch_bounds__(kBT) void k_count_gather2(const int32_t* __restrict
   }
 }
 
-// phase 2: scatter-add into the 2^k-th successor, advance, compact
+// phase 2: scatter-add into the 2^k-th successor, advance, compact.  As
+// the pointers converge, many nodes of a block add into the same few
+// successors; their adds are summed per block in an LDS table first (one
+// global atomic per distinct successor and block; a successor that finds no
+// slot adds directly).  The sums are integers, so the order does not matter.
+constexpr int kAddSlots = 1024;
+
 __global__ __launch_bounds__(kBT) void k_count_scatter2(const int32_t* __restrict__ list, const int64_t* count,
                                                         uint32_t* S, int32_t* P, const int32_t* __restrict__ tgt,
                                                         const uint32_t* __restrict__ val,
                                                         const int32_t* __restrict__ nxt, int32_t* __restrict__ out,
                                                         int64_t* next_count) {
+  __shared__ int32_t tk[kAddSlots];
+  __shared__ uint32_t tv[kAddSlots];
+  for (int q = threadIdx.x; q < kAddSlots; q += kBT) {
+    tk[q] = -1;
+    tv[q] = 0u;
+  }
+  __syncthreads();
   const int64_t n = *count;
   for (int64_t base = (int64_t)blockIdx.x * kRChunk; base < n; base += (int64_t)gridDim.x * kRChunk) {
     int32_t v[kRItems];
@@ -116,7 +129,20 @@ __global__ __launch_bounds__(kBT) void k_count_scatter2(const int32_t* __restric
       v[j] = 0;
       if (w < n) {
         const int m = list[w];
-        atomicAdd(&S[tgt[w]], val[w]);
+        const int t = tgt[w];
+        const uint32_t a = val[w];
+        if (a != 0u) {
+          bool done = false;
+          int slot = (int)(((uint32_t)t * 2654435761u) >> 22);  // 10-bit hash
+          for (int probe = 0; probe < 4 && !done; ++probe, slot = (slot + 1) & (kAddSlots - 1)) {
+            const int prev = atomicCAS(&tk[slot], -1, t);
+            if (prev == -1 || prev == t) {
+              atomicAdd(&tv[slot], a);
+              done = true;
+            }
+          }
+          if (!done) atomicAdd(&S[t], a);
+        }
         const int q = nxt[w];
         P[m] = q;
         v[j] = m;
@@ -125,6 +151,9 @@ __global__ __launch_bounds__(kBT) void k_count_scatter2(const int32_t* __restric
     }
     block_append<kRItems>(v, take, out, next_count);
   }
+  __syncthreads();
+  for (int q = threadIdx.x; q < kAddSlots; q += kBT)
+    if (tk[q] >= 0 && tv[q] != 0u) atomicAdd(&S[tk[q]], tv[q]);
 }
 
 // phase 2 for flags ("is anything upstream marked"): OR into the 2^k-th
