@@ -1859,8 +1859,7 @@ __global__ __launch_bounds__(kBT) void k_basin_next(const int32_t* __restrict__ 
           v[0] = b;
           take[0] = true;
         }
-        M0[b] = m;
-        M1[b] = m;
+        M0[b] = m;  // M1 gets every pit's entry from the first jump round, if one runs
       } else if (b == N) {
         v[0] = b;
         take[0] = true;

@@ -781,18 +781,23 @@ __global__ __launch_bounds__(kBT) void k_zprime_list(GridD g, const int32_t* __r
   saddle_table_flush(t, saddle);
 }
 
-// pass 3 (:199-219) for one border cell of a closed basin: largest cell index wins
+// Pass 3 with pass 4's neighbour scan folded in, for the tiled trips: a
+// saddle-node candidate also finds its lowest neighbour outside the basin
+// (first direction wins ties, strict < from 1e9: the outlet the reference
+// computes for the final saddle node, :226-250) and offers (i << 3) | k,
+// k that neighbour's direction or 4 for none (also when the basin has no
+// saddle key).  The largest offer is the largest candidate index with its
+// own direction; k_outlet_list decodes it (cell indices below 2^28).
 template <int MODE>
-__device__ __forceinline__ void saddle_node_cell(const GridD& g, int i, int row, int col, int bi,
-                                                 const int32_t* __restrict__ bid, const float* __restrict__ zp,
-                                                 const int64_t* __restrict__ saddle, int32_t* snode) {
-  // the neighbours' labels are loaded before the saddle key arrives (one
-  // round trip fewer than loading them after the comparison)
-  int bj[4];
+__device__ __forceinline__ void saddle_node_cell_packed(const GridD& g, int i, int row, int col, int bi,
+                                                        const int32_t* __restrict__ bid, const float* __restrict__ z,
+                                                        const float* __restrict__ zp,
+                                                        const int64_t* __restrict__ saddle, int32_t* snode) {
+  int jn[4], bj[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const int jn = neighbour<MODE>(g, i, row, col, k);
-    bj[k] = jn >= 0 ? bid[jn] : -1;
+    jn[k] = neighbour<MODE>(g, i, row, col, k);
+    bj[k] = jn[k] >= 0 ? bid[jn[k]] : -1;
   }
   const int64_t key = saddle[bi];
   if (zp[i] != key_float(key)) return;
@@ -800,49 +805,54 @@ __device__ __forceinline__ void saddle_node_cell(const GridD& g, int i, int row,
   bool here = false;
 #pragma unroll
   for (int k = 0; k < 4; ++k) here |= bj[k] >= 0 && bj[k] == tb;
-  if (here) atomicMax(&snode[bi], i);
+  if (!here) return;
+  int code = 4;
+  if (key != INVALID_KEY) {
+    float tz = 1e9f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (jn[k] < 0 || bj[k] == bi) continue;
+      const float zj = z[jn[k]];
+      if (tz > zj) {
+        tz = zj;
+        code = k;
+      }
+    }
+  }
+  atomicMax(&snode[bi], (i << 3) | code);
 }
 
 // over this trip's border list
 template <int MODE>
 __global__ __launch_bounds__(kBT) void k_saddle_node_list(GridD g, const int32_t* __restrict__ list,
                                                           const int64_t* nlist, const int32_t* __restrict__ bid,
-                                                          const float* __restrict__ zp,
+                                                          const float* __restrict__ z, const float* __restrict__ zp,
                                                           const int64_t* __restrict__ saddle, int32_t* snode) {
   const int64_t n = *nlist;
   for (int64_t e = (int64_t)blockIdx.x * kBT + threadIdx.x; e < n; e += (int64_t)gridDim.x * kBT) {
     const int i = list[e];
     int row, col;
     rc_of(g, i, row, col);
-    saddle_node_cell<MODE>(g, i, row, col, bid[i], bid, zp, saddle, snode);
+    saddle_node_cell_packed<MODE>(g, i, row, col, bid[i], bid, z, zp, saddle, snode);
   }
 }
 
-// pass 4 (:226-250) for pit b: the lowest neighbour of its saddle node outside it
+// pass 4 for pit b from its packed saddle-node offer: the saddle node, and
+// the outlet key of the neighbour its direction names
 template <int MODE>
-__device__ __forceinline__ void outlet_of_pit(const GridD& g, int b, const int32_t* __restrict__ bid,
-                                              const float* __restrict__ z, const int64_t* __restrict__ saddle,
-                                              const int32_t* __restrict__ snode, int64_t* __restrict__ outlet) {
-  if (b >= g.N || saddle[b] == INVALID_KEY) return;
-  const int node = snode[b];
-  if (node < 0) return;
+__device__ __forceinline__ void outlet_of_packed(const GridD& g, int b, const float* __restrict__ z, int32_t* snode,
+                                                 int64_t* __restrict__ outlet) {
+  if (b <= 0 || b >= g.N) return;
+  const int sp = snode[b];
+  if (sp < 0) return;
+  const int node = sp >> 3, code = sp & 7;
+  snode[b] = node;
+  if (code >= 4) return;
   int row, col;
   rc_of(g, node, row, col);
-  float tz = 1e9f;
-  int rr = -1;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int jn = neighbour<MODE>(g, node, row, col, k);
-    if (jn < 0) continue;
-    if (bid[jn] != b && tz > z[jn]) {
-      tz = z[jn];
-      rr = jn;
-    }
-  }
-  if (rr > -1) {
-    const int64_t c = pack_key(tz, rr);
-    if (c < outlet[b]) outlet[b] = c;
-  }
+  const int rr = neighbour<MODE>(g, node, row, col, code);
+  const int64_t c = pack_key(z[rr], rr);
+  if (c < outlet[b]) outlet[b] = c;
 }
 
 // pass 5 (:255-281), first phase for pit b: 1 when b and the basin of its
@@ -886,15 +896,14 @@ __device__ __forceinline__ void cycle_apply_of(int32_t N, int b, const int32_t* 
   }
 }
 
-// pass 4 over the pits
+// pass 4 over the pits, from the packed saddle-node offers
 template <int MODE>
 __global__ __launch_bounds__(kBT) void k_outlet_list(GridD g, const int32_t* __restrict__ pits, const int64_t* np,
-                                                     const int32_t* __restrict__ bid, const float* __restrict__ z,
-                                                     const int64_t* __restrict__ saddle,
-                                                     const int32_t* __restrict__ snode, int64_t* __restrict__ outlet) {
+                                                     const float* __restrict__ z, int32_t* snode,
+                                                     int64_t* __restrict__ outlet) {
   const int64_t n = *np;
   for (int64_t w = (int64_t)blockIdx.x * kBT + threadIdx.x; w < n; w += (int64_t)gridDim.x * kBT)
-    outlet_of_pit<MODE>(g, pits[w], bid, z, saddle, snode, outlet);
+    outlet_of_packed<MODE>(g, pits[w], z, snode, outlet);
 }
 
 // pass 5 over the pits, in two kernels (every flag is read after all are set)
@@ -1019,15 +1028,15 @@ int saddlesort_active(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t*
       PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL((k_zprime_list<M, 2>), lgz, bl, 0, s, g, (const int32_t*)BL[in],
                                                    (const int64_t*)NB[in], bid, z, isb, zp, saddle, BL[out], NB[out]));
   }
+  if (g.N > (1 << 28)) return fail(PFF_EINVAL, "saddlesort_active: grids above 2^28 cells are not supported");
   PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_saddle_node_list<M>, lgz, bl, 0, s, g, (const int32_t*)BL[out],
-                                               (const int64_t*)NB[out], bid, (const float*)zp, (const int64_t*)saddle,
-                                               snode));
+                                               (const int64_t*)NB[out], bid, z, (const float*)zp,
+                                               (const int64_t*)saddle, snode));
   *border_cur = out;
   uint32_t* wbits;
   uint8_t* tag;
   PFF_TRY(tag_state(ctx, g, &wbits, &tag, true));
-  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_outlet_list<M>, lg, bl, 0, s, g, pits, npits, bid, z,
-                                               (const int64_t*)saddle, (const int32_t*)snode, outlet));
+  PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_outlet_list<M>, lg, bl, 0, s, g, pits, npits, z, snode, outlet));
   hipLaunchKernelGGL(k_cycle_flag_list, lg, bl, 0, s, g.N, pits, npits, bid, (const int64_t*)outlet, (uint8_t*)inv,
                      g.F42);
   hipLaunchKernelGGL(k_cycle_apply_list, lg, bl, 0, s, g.N, pits, npits, bid, outlet, saddle, snode,
