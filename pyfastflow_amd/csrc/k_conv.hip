@@ -377,18 +377,46 @@ __global__ __launch_bounds__(NW * 64) void k_conv3x3_wgrad(int n_img, int H, int
       }
 }
 
-// bias gradient db[c] += sum over pixels of dy[p][c]
-__global__ __launch_bounds__(256) void k_bias_grad(int64_t npix, int C, const uint16_t* __restrict__ dy,
-                                                  float* __restrict__ db) {
-  extern __shared__ float part[];
-  for (int c = threadIdx.x; c < C; c += 256) part[c] = 0.f;
+// bias gradient db[c] += sum over pixels of dy[p][c], any C: per-element
+// adds into a per-block LDS copy, one global add per channel and block
+__global__ __launch_bounds__(256) void k_bias_grad_any(int64_t npix, int C, const uint16_t* __restrict__ dy,
+                                                      float* __restrict__ db) {
+  extern __shared__ float partc[];
+  for (int c = threadIdx.x; c < C; c += 256) partc[c] = 0.f;
   __syncthreads();
   const int64_t per = (npix + gridDim.x - 1) / gridDim.x;
   const int64_t p0 = (int64_t)blockIdx.x * per;
   const int64_t p1 = p0 + per < npix ? p0 + per : npix;
-  for (int64_t e = p0 * C + threadIdx.x; e < p1 * C; e += 256) atomicAdd(&part[e % C], bf2f(dy[e]));
+  for (int64_t e = p0 * C + threadIdx.x; e < p1 * C; e += 256) atomicAdd(&partc[e % C], bf2f(dy[e]));
   __syncthreads();
-  for (int c = threadIdx.x; c < C; c += 256) atomicAdd(&db[c], part[c]);
+  for (int c = threadIdx.x; c < C; c += 256) atomicAdd(&db[c], partc[c]);
+}
+
+// the same for C a multiple of 8 dividing 2048 (every layer of the U-Net)
+// and dy 16-byte aligned: a thread owns 8 channels (one 16-byte load per pixel)
+// and strides over pixels with the other threads of its channel group; the
+// block then sums its threads' partial sums per channel in LDS and adds one
+// value per channel.
+__global__ __launch_bounds__(256) void k_bias_grad(int64_t npix, int C, const uint16_t* __restrict__ dy,
+                                                  float* __restrict__ db) {
+  __shared__ float part[256][9];
+  const int groups = C / 8, lanes = 256 / groups;
+  const int g = threadIdx.x % groups, pl = threadIdx.x / groups;
+  float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  for (int64_t p = (int64_t)blockIdx.x * lanes + pl; p < npix; p += (int64_t)gridDim.x * lanes) {
+    const uint4 v = *reinterpret_cast<const uint4*>(dy + p * C + 8 * g);
+    const uint16_t* e = reinterpret_cast<const uint16_t*>(&v);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc[j] += bf2f(e[j]);
+  }
+#pragma unroll
+  for (int j = 0; j < 8; ++j) part[threadIdx.x][j] = acc[j];
+  __syncthreads();
+  for (int c = threadIdx.x; c < C; c += 256) {
+    float sum = 0.f;
+    for (int l = 0; l < lanes; ++l) sum += part[l * groups + c / 8][c % 8];
+    atomicAdd(&db[c], sum);
+  }
 }
 
 // wf[tap][co][ci] and wt[tap'][ci][co] (tap' = 8 - tap) from w[co][ci][ky][kx]
@@ -575,7 +603,10 @@ int launch_conv3x3_wgrad(pff_ctx* ctx, int n_img, int H, int W, int Cin, int Cou
   PFF_TRY(check_hip(hipGetLastError(), "conv3x3_wgrad"));
   if (db) {
     const int64_t npix = nrows * W;
-    hipLaunchKernelGGL(k_bias_grad, dim3(512), dim3(256), (size_t)Cout * 4, ctx->stream, npix, Cout, dy, db);
+    if (Cout % 8 == 0 && (256 * 8) % Cout == 0 && (reinterpret_cast<uintptr_t>(dy) & 15) == 0)
+      hipLaunchKernelGGL(k_bias_grad, dim3(512), dim3(256), 0, ctx->stream, npix, Cout, dy, db);
+    else
+      hipLaunchKernelGGL(k_bias_grad_any, dim3(512), dim3(256), (size_t)Cout * 4, ctx->stream, npix, Cout, dy, db);
     PFF_TRY(check_hip(hipGetLastError(), "bias_grad"));
   }
   return 0;
