@@ -6,17 +6,24 @@ namespace pff {
 
 // ---------------------------------------------------------------------------
 // D4 steepest-descent receivers (receivers.py:15-58).
-// Each wavefront owns a strip of 64 columns and marches down kRcvRows rows:
-// a lane keeps the elevations of its column in the rows above, at and below
-// the current row in registers (one coalesced 256 B load per row and wave,
-// issued one row ahead), takes the left and right elevations from its
-// neighbour lanes with a shuffle, and loads from memory only the neighbours
-// outside the strip (the strip's edge lanes, periodic wraps).  Every z is
-// thus read once from HBM.  Ties keep the earliest direction (strict >),
-// invalid links never win, and the slope is the IEEE f32 quotient
-// (z_i - z_j) / dx, so receivers match the reference bit for bit.
+// Each wavefront owns a strip of 64 columns and kRcvRows rows.  Ties keep the
+// earliest direction (strict >), invalid links never win, and the slope is
+// the IEEE f32 quotient (z_i - z_j) / dx, so receivers match the reference
+// bit for bit.
+//
+// Inside the grid (the strip and its rows away from the border, the common
+// case) a wave loads its kRcvRows + 2 rows (and their codes in the custom and
+// canvas modes) and, with one load, the columns either side of the strip, all
+// before its first store (on gfx9 a wait for a load also waits for every
+// store issued before it); left and right elevations come from the neighbour
+// lanes, the strip-edge ones by readlane.  A cell whose four links are valid
+// needs one division (steepest4).  Elsewhere (grid border, stochastic draws)
+// each lane keeps its column's rows above, at and below in registers,
+// marching down, and loads the neighbours outside them.  The kernel is bound by its
+// vector instructions (a wave64 instruction takes four cycles), which is
+// what the interior path cuts.
 // ---------------------------------------------------------------------------
-constexpr int kRcvRows = 16;
+constexpr int kRcvRows = 4;
 // Uniform draw in [0, 1) for cell i, direction k of one stochastic call
 // (RAND_RCV, receivers.py:46-47): splitmix64 (Steele, Lea and Flood 2014) of
 // seed + (4i+k) times the golden-ratio constant, top 24 bits times 2^-24
@@ -29,15 +36,126 @@ __device__ __forceinline__ float uniform01(uint64_t seed, int i, int k) {
   return (float)(x >> 40) * 0x1p-24f;
 }
 
+// Slope and receiver of a cell whose four links are valid (neighbours i-nx,
+// i-1, i+1, i+nx in this order), dx > 0.  fl(d/dx) is non-decreasing in d,
+// so the largest slope is q = fl(dmax/dx) and the first direction whose
+// difference equals dmax has it.  A direction with d < dmax rounds to q only
+// when d is within 2^-22 of dmax relative to it (q normal); such near ties,
+// and q tiny, subnormal or infinite, take the four divisions of the
+// reference.  NaN differences never win, as in the reference (NaN > s is
+// false): fmaxf drops them, and a cell whose differences are all NaN keeps
+// itself with slope 0.
+__device__ __forceinline__ void steepest4(float zc, float zu, float zl, float zr, float zd, float dx, int i, int nx,
+                                          int& rcv_out, float& grad_out) {
+  const float d0 = zc - zu, d1 = zc - zl, d2 = zc - zr, d3 = zc - zd;
+  const float dm = fmaxf(fmaxf(d0, d1), fmaxf(d2, d3));
+  const float q = dm / dx;
+  const bool pos = q > 0.0f;
+  int r = d2 == dm ? i + 1 : i + nx;
+  r = d1 == dm ? i - 1 : r;
+  r = d0 == dm ? i - nx : r;
+  const float thr = dm - dm * 0x1p-20f;
+  const bool near = ((d0 < dm) & (d0 >= thr)) | ((d1 < dm) & (d1 >= thr)) | ((d2 < dm) & (d2 >= thr)) |
+                    ((d3 < dm) & (d3 >= thr)) | !((q >= 0x1p-100f) & (q <= 3.402823466e38f) & (dm >= 0x1p-100f));
+  rcv_out = pos ? r : i;
+  grad_out = pos ? q : 0.0f;
+  if (__builtin_expect(pos & near, 0)) {
+    const float dd[4] = {d0, d1, d2, d3};
+    const int jj[4] = {i - nx, i - 1, i + 1, i + nx};
+    int rr = i;
+    float sr = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const float t = dd[k] / dx;
+      if (t > sr) {
+        sr = t;
+        rr = jj[k];
+      }
+    }
+    rcv_out = rr;
+    grad_out = sr;
+  }
+}
+
 template <int MODE, bool STOCH = false>
 __global__ __launch_bounds__(256) void k_receivers(GridD g, const float* __restrict__ z, int32_t* __restrict__ rcv,
                                                    float* __restrict__ grad, uint64_t seed = 0) {
+  static_assert(kRcvRows <= 32, "strip-edge columns are held in lanes 0..kRcvRows-1 and 32..31+kRcvRows");
   int bx, by;
   xcd_block(bx, by);
   const int lane = threadIdx.x;
-  const int col = bx * 64 + lane;
+  const int col0 = bx * 64, col = col0 + lane;
   const int r0 = (by * 4 + threadIdx.y) * kRcvRows;
   if (r0 >= g.ny) return;  // uniform per wavefront
+  const int nx = g.nx;
+  if (!STOCH && col0 > 0 && col0 + 64 < nx && r0 > 0 && r0 + kRcvRows < g.ny && g.dx > 0.0f) {
+    // every cell of the wave is inside: rows r0-1 .. r0+kRcvRows and columns
+    // col0-1 .. col0+64 exist.  Away from the border a link is the plain D4
+    // one in modes 0-2, and in modes 3 and 4 it is valid when both of its
+    // cells have a nonzero code (neighbourer_flat.py:360-462 for an interior
+    // cell; the canvas rule).
+    constexpr bool kCodes = MODE == 3 || MODE == 4;
+    float zz[kRcvRows + 2];
+    uint32_t bb[kRcvRows + 2];
+#pragma unroll
+    for (int k = 0; k < kRcvRows + 2; ++k) zz[k] = z[(size_t)(r0 - 1 + k) * nx + col];
+    if constexpr (kCodes) {
+#pragma unroll
+      for (int k = 0; k < kRcvRows + 2; ++k) bb[k] = g.B[(size_t)(r0 - 1 + k) * nx + col];
+    }
+    // lanes 0..kRcvRows-1: column col0-1 of row r0+lane; lanes 32..: column col0+64
+    const int hk = lane & 31;
+    const size_t hcell = (size_t)(r0 + hk) * nx + (lane < 32 ? col0 - 1 : col0 + 64);
+    float halo = 0.0f;
+    uint32_t hcode = 0;
+    if (hk < kRcvRows) {
+      halo = z[hcell];
+      if constexpr (kCodes) hcode = g.B[hcell];
+    }
+#pragma unroll
+    for (int k = 0; k < kRcvRows; ++k) {
+      const float zc = zz[k + 1];
+      float zl = __shfl_up(zc, 1);
+      float zr = __shfl_down(zc, 1);
+      zl = lane == 0 ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(halo), k)) : zl;
+      zr = lane == 63 ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(halo), 32 + k)) : zr;
+      const int i = (r0 + k) * nx + col;
+      int r;
+      float s;
+      if constexpr (!kCodes) {
+        steepest4(zc, zz[k], zl, zr, zz[k + 2], g.dx, i, nx, r, s);
+      } else {
+        const uint32_t bc = bb[k + 1];
+        uint32_t bl = __shfl_up(bc, 1), br = __shfl_down(bc, 1);
+        bl = lane == 0 ? __builtin_amdgcn_readlane(hcode, k) : bl;
+        br = lane == 63 ? __builtin_amdgcn_readlane(hcode, 32 + k) : br;
+        const bool ok[4] = {bb[k] != 0, bl != 0, br != 0, bb[k + 2] != 0};
+        if (bc != 0 && ok[0] && ok[1] && ok[2] && ok[3]) {
+          steepest4(zc, zz[k], zl, zr, zz[k + 2], g.dx, i, nx, r, s);
+        } else {
+          // some links invalid: the reference loop over the valid ones
+          const float zn[4] = {zz[k], zl, zr, zz[k + 2]};
+          const int jj[4] = {i - nx, i - 1, i + 1, i + nx};
+          r = i;
+          s = 0.0f;
+          if (bc != 0) {
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+              if (!ok[d]) continue;
+              const float t = (zc - zn[d]) / g.dx;
+              if (t > s) {
+                s = t;
+                r = jj[d];
+              }
+            }
+          }
+        }
+      }
+      rcv[i] = r;
+      grad[i] = s;
+    }
+    return;
+  }
   const int r1 = min(r0 + kRcvRows, g.ny);
   const bool cv = col < g.nx;
   float zu = (cv && r0 > 0) ? z[(r0 - 1) * g.nx + col] : 0.0f;  // row above
