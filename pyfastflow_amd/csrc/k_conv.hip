@@ -377,6 +377,174 @@ __global__ __launch_bounds__(NW * 64) void k_conv3x3_wgrad(int n_img, int H, int
       }
 }
 
+// Weight gradient for W % 32 == 0 (every U-Net level): the GEMM
+// dw[co][ci][tap] = sum over pixels p of dy[p][co] * x[p + tap offset][ci],
+// k = pixels.  An MFMA operand lane holds 8 consecutive k of one row or
+// column, i.e. 8 pixels of one channel, while memory holds 8 channels of a
+// pixel contiguous: each staging thread loads an 8-pixel x 8-channel block
+// (eight 16-byte loads), transposes it in registers and writes eight 16-byte
+// channel rows to LDS.  A block (3 waves, wave = ky) takes 32 output x 32
+// input channels, all 9 taps, over a chunk of row segments (SEG pixels of one
+// image row).  x is staged once per segment (pixels x0-8 .. x0+SEG+7 of the
+// three rows); the kx = 0 and kx = 2 operands are the kx = 1 one shifted by one
+// pixel, built from the 16-byte read and its neighbouring halves with
+// v_alignbit.  The next segment's blocks are loaded into registers before the
+// current one is computed.  Each block writes its partial sums (9 x 32 x 32
+// floats) to a workspace, and k_wgrad_reduce adds the chunks into dw.
+constexpr int kWgPadT = 8;
+constexpr int kWgTile = 9 * 32 * 32;  // partial sums per block
+
+__device__ __forceinline__ uint32_t u4w(const uint4& v, int d) { return d == 0 ? v.x : d == 1 ? v.y : d == 2 ? v.z : v.w; }
+
+// v[p] = channels 0..7 of pixel p -> t[c] = pixels 0..7 of channel c
+__device__ __forceinline__ void transpose8x8_bf16(const uint4 (&v)[8], uint4 (&t)[8]) {
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    uint32_t o[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const uint32_t a = u4w(v[2 * e], c >> 1), b = u4w(v[2 * e + 1], c >> 1);
+      o[e] = (c & 1) ? ((a >> 16) | (b & 0xFFFF0000u)) : ((a & 0xFFFFu) | (b << 16));
+    }
+    t[c] = make_uint4(o[0], o[1], o[2], o[3]);
+  }
+}
+
+template <int SEG>
+__global__ __launch_bounds__(192) void k_wgrad_t(int H, int W, int Cin, int Cout, int64_t nseg, int64_t per,
+                                                 const uint16_t* __restrict__ x, const uint16_t* __restrict__ dy,
+                                                 float* __restrict__ part) {
+  constexpr int XP = SEG + 16;                      // staged x pixels: x0-8 .. x0+SEG+7
+  constexpr int TDY = (SEG / 8) * 4;                // dy staging tasks (8 px x 8 ch each)
+  constexpr int TX = 3 * (XP / 8) * 4;              // x staging tasks
+  static_assert(TDY + TX <= 192, "one staging task per thread");
+  __shared__ __attribute__((aligned(16))) uint16_t dyt[32][SEG + kWgPadT];     // [co][px]
+  __shared__ __attribute__((aligned(16))) uint16_t xt[3][32][XP + kWgPadT];    // [ky][ci][px]
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int co0 = blockIdx.y * 32, ci0 = blockIdx.z * 32;
+  const int segs_row = W / SEG;
+  const int64_t s0 = (int64_t)blockIdx.x * per, s1 = s0 + per < nseg ? s0 + per : nseg;
+  // this thread's staging task
+  const int t = threadIdx.x;
+  const bool is_dy = t < TDY, is_x = t >= TDY && t < TDY + TX;
+  const int q = t & 3;                              // channel block
+  const int u = is_dy ? t >> 2 : (t - TDY) >> 2;
+  const int b = is_dy ? u : u % (XP / 8);           // pixel block
+  const int r = is_dy ? 0 : u / (XP / 8);           // x row (ky)
+  // global address of this thread's block in segment sg (nullptr: zeros)
+  auto src_of = [&](int64_t sg) -> const uint16_t* {
+    const int64_t rr = sg / segs_row;
+    const int x0 = (int)(sg - rr * segs_row) * SEG;
+    const int64_t img = rr / H;
+    const int yy = (int)(rr - img * H);
+    if (is_dy) return dy + (((size_t)img * H + yy) * W + x0 + 8 * b) * Cout + co0 + 8 * q;
+    const int gy = yy + r - 1, gx = x0 - 8 + 8 * b;  // gx % 8 == 0 and W % 8 == 0: the block is in or out
+    if (!is_x || gy < 0 || gy >= H || gx < 0 || gx >= W) return nullptr;
+    return x + (((size_t)img * H + gy) * W + gx) * Cin + ci0 + 8 * q;
+  };
+  const int cstride = is_dy ? Cout : Cin;
+  f32x4 acc[3][2][2];  // [kx][j][i]
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int i = 0; i < 2; ++i) acc[a][j][i] = f32x4{0.f, 0.f, 0.f, 0.f};
+  uint4 v[8];
+#pragma unroll
+  for (int pp = 0; pp < 8; ++pp) v[pp] = make_uint4(0u, 0u, 0u, 0u);
+  if (s0 < s1) {
+    const uint16_t* src = src_of(s0);
+    if (src) {
+#pragma unroll
+      for (int pp = 0; pp < 8; ++pp) v[pp] = *reinterpret_cast<const uint4*>(src + (size_t)pp * cstride);
+    }
+  }
+  for (int64_t sg = s0; sg < s1; ++sg) {
+    if (is_dy || is_x) {
+      uint4 tv[8];
+      transpose8x8_bf16(v, tv);
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        uint16_t* dst = is_dy ? &dyt[8 * q + c][8 * b] : &xt[r][8 * q + c][8 * b];
+        *reinterpret_cast<uint4*>(dst) = tv[c];
+      }
+    }
+    __syncthreads();
+    if (sg + 1 < s1) {  // the next segment's block, in flight while this one is computed
+      const uint16_t* src = src_of(sg + 1);
+#pragma unroll
+      for (int pp = 0; pp < 8; ++pp)
+        v[pp] = src ? *reinterpret_cast<const uint4*>(src + (size_t)pp * cstride) : make_uint4(0u, 0u, 0u, 0u);
+    }
+#pragma unroll
+    for (int ks = 0; ks < SEG / 32; ++ks) {
+      const int kp = 32 * ks + 8 * (lane >> 4);
+      bf16x8 A[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) A[i] = *reinterpret_cast<const bf16x8*>(&dyt[16 * i + (lane & 15)][kp]);
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const uint16_t* row = &xt[wave][16 * j + (lane & 15)][0];
+        const int p0 = 8 + kp;  // staged index of pixel x0 + kp
+        const uint4 mid = *reinterpret_cast<const uint4*>(row + p0);
+        const uint32_t prv = *reinterpret_cast<const uint32_t*>(row + p0 - 2);
+        const uint32_t nxt = *reinterpret_cast<const uint32_t*>(row + p0 + 8);
+        // pixels shifted by -1 (kx = 0) and +1 (kx = 2)
+        const uint32_t a0 = __builtin_amdgcn_alignbit(mid.x, prv, 16);
+        const uint32_t a1 = __builtin_amdgcn_alignbit(mid.y, mid.x, 16);
+        const uint32_t a2 = __builtin_amdgcn_alignbit(mid.z, mid.y, 16);
+        const uint32_t a3 = __builtin_amdgcn_alignbit(mid.w, mid.z, 16);
+        const uint32_t a4 = __builtin_amdgcn_alignbit(nxt, mid.w, 16);
+        const uint4 b0 = make_uint4(a0, a1, a2, a3), b2 = make_uint4(a1, a2, a3, a4);
+        const bf16x8 B0 = *reinterpret_cast<const bf16x8*>(&b0);
+        const bf16x8 B1 = *reinterpret_cast<const bf16x8*>(&mid);
+        const bf16x8 B2 = *reinterpret_cast<const bf16x8*>(&b2);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          acc[0][j][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[i], B0, acc[0][j][i], 0, 0, 0);
+          acc[1][j][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[i], B1, acc[1][j][i], 0, 0, 0);
+          acc[2][j][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[i], B2, acc[2][j][i], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // partial sums of this chunk: part[chunk][tile][tap][co_l][ci_l]
+  const int64_t tile = (int64_t)blockIdx.y * gridDim.z + blockIdx.z;
+  const int64_t ntile = (int64_t)gridDim.y * gridDim.z;
+  float* out = part + ((int64_t)blockIdx.x * ntile + tile) * kWgTile;
+#pragma unroll
+  for (int kx = 0; kx < 3; ++kx)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int col = 16 * i + 4 * (lane >> 4) + e, cil = 16 * j + (lane & 15);
+          out[(3 * wave + kx) * 1024 + col * 32 + cil] = acc[kx][j][i][e];
+        }
+}
+
+// dw[co][ci][tap] += sum over chunks of the partial sums; blockIdx.y takes a
+// group of kWgGroup chunks
+constexpr int kWgGroup = 32;
+__global__ __launch_bounds__(256) void k_wgrad_reduce(const float* __restrict__ part, int64_t chunks, int ntile,
+                                                      int ci_tiles, int Cin, float* __restrict__ dw) {
+  const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t total = (int64_t)ntile * kWgTile;
+  if (o >= total) return;
+  const int64_t c0 = (int64_t)blockIdx.y * kWgGroup;
+  const int64_t c1 = c0 + kWgGroup < chunks ? c0 + kWgGroup : chunks;
+  float s = 0.f;
+  for (int64_t c = c0; c < c1; ++c) s += part[c * total + o];
+  const int tile = (int)(o / kWgTile), rem = (int)(o - (int64_t)tile * kWgTile);
+  const int tap = rem >> 10, col = (rem >> 5) & 31, cil = rem & 31;
+  const int co = (tile / ci_tiles) * 32 + col, ci = (tile % ci_tiles) * 32 + cil;
+  atomicAdd(&dw[((size_t)co * Cin + ci) * 9 + tap], s);
+}
+
 // bias gradient db[c] += sum over pixels of dy[p][c], any C: per-element
 // adds into a per-block LDS copy, one global add per channel and block
 __global__ __launch_bounds__(256) void k_bias_grad_any(int64_t npix, int C, const uint16_t* __restrict__ dy,
@@ -594,7 +762,29 @@ int launch_conv3x3_wgrad(pff_ctx* ctx, int n_img, int H, int W, int Cin, int Cou
                          const uint16_t* dy, float* dw, float* db) {
   const int64_t nrows = (int64_t)n_img * H;
   const unsigned gx = (unsigned)((nrows + kWgRows - 1) / kWgRows);
-  if (Cout % 64 == 0)
+  const bool aligned = ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(dy)) & 15) == 0;
+  if (W % 32 == 0 && aligned) {
+    // chunks of row segments: about 1024 blocks in all (partial sums 36 KB each)
+    const int seg = W % 64 == 0 ? 64 : 32;
+    const int64_t nseg = nrows * (W / seg);
+    const int ntile = (Cout / 32) * (Cin / 32);
+    int64_t chunks = std::min<int64_t>(std::max<int64_t>((1024 + ntile - 1) / ntile, 1), nseg);
+    const int64_t per = (nseg + chunks - 1) / chunks;
+    chunks = (nseg + per - 1) / per;
+    void* p;
+    PFF_TRY(scratch(ctx, S_WGPART, (size_t)chunks * ntile * kWgTile * sizeof(float), &p));
+    float* part = (float*)p;
+    const dim3 grid((unsigned)chunks, Cout / 32, Cin / 32);
+    if (seg == 64)
+      hipLaunchKernelGGL(k_wgrad_t<64>, grid, dim3(192), 0, ctx->stream, H, W, Cin, Cout, nseg, per, x, dy, part);
+    else
+      hipLaunchKernelGGL(k_wgrad_t<32>, grid, dim3(192), 0, ctx->stream, H, W, Cin, Cout, nseg, per, x, dy, part);
+    PFF_TRY(check_hip(hipGetLastError(), "conv3x3_wgrad (transposed staging)"));
+    const int64_t total = (int64_t)ntile * kWgTile;
+    hipLaunchKernelGGL(k_wgrad_reduce, dim3((unsigned)((total + 255) / 256), (unsigned)((chunks + kWgGroup - 1) / kWgGroup)),
+                       dim3(256), 0, ctx->stream, part, chunks, ntile, Cin / 32, Cin, dw);
+    PFF_TRY(check_hip(hipGetLastError(), "conv3x3_wgrad reduce"));
+  } else if (Cout % 64 == 0)
     hipLaunchKernelGGL(k_conv3x3_wgrad<4>, dim3(gx, Cout / 64, Cin / 32), dim3(256), 0, ctx->stream, n_img, H, W, Cin,
                        Cout, x, dy, dw);
   else

@@ -125,6 +125,7 @@ enum Slot : int {
   S_SEGCO,              // their per-tile counts and offsets
   S_RING,               // ring-slot arrays of the carve tag walks (4 x (256 per tile + 1))
   S_RCACHE,             // cached local ends of the ring walks (u16 per ring slot) + per-tile dirty words
+  S_WGPART,             // per-chunk partial weight gradients of a 3x3 convolution
   S_NSLOTS
 };
 static_assert(S_NSLOTS <= pff_ctx::kSlots, "pff_ctx::kSlots must cover every scratch slot");

@@ -62,7 +62,8 @@ def test_conv3x3_forward(pf, n, h, w, cin, cout, relu):
 
 
 @pytest.mark.parametrize("n,h,w,cin,cout", [(1, 6, 9, 32, 32), (2, 16, 66, 64, 32), (1, 12, 40, 32, 64),
-                                             (2, 16, 128, 32, 64), (1, 32, 32, 64, 32)])
+                                             (2, 16, 128, 32, 64), (1, 32, 32, 64, 32), (2, 8, 64, 96, 32),
+                                             (1, 5, 256, 32, 32), (3, 4, 32, 256, 128), (2, 3, 96, 64, 64)])
 def test_conv3x3_backward(pf, n, h, w, cin, cout):
     from pyfastflow_amd.surrogate import ops
 
