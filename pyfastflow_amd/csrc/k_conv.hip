@@ -530,6 +530,10 @@ __global__ __launch_bounds__(192) void k_wgrad_t(int H, int W, int Cin, int Cout
 // dw[co][ci][tap] += sum over chunks of the partial sums; blockIdx.y takes a
 // group of kWgGroup chunks
 constexpr int kWgGroup = 32;
+#ifndef PFF_WG_BLOCKS
+#define PFF_WG_BLOCKS 512
+#endif
+constexpr int kWgBlocks = PFF_WG_BLOCKS;  // blocks per weight-gradient launch (partial sums 36 KB each); 256, 384, 512, 1024, 2048 measured, 512 fastest
 __global__ __launch_bounds__(256) void k_wgrad_reduce(const float* __restrict__ part, int64_t chunks, int ntile,
                                                       int ci_tiles, int Cin, float* __restrict__ dw) {
   const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -724,6 +728,55 @@ __global__ __launch_bounds__(256) void k_head_bwd(int64_t npix, int C, const uin
   if (threadIdx.x == 0) atomicAdd(db, pb);
 }
 
+// the same for C a multiple of 8 dividing 2048 and 16-byte aligned x, dx: a
+// thread owns 8 channels (16-byte loads and stores) and strides over pixels
+// with the other threads of its channel group; the block sums its threads'
+// partial sums per channel in LDS and adds one value per channel (and one for
+// the bias, from the threads of channel group 0)
+__global__ __launch_bounds__(256) void k_head_bwd8(int64_t npix, int C, const uint16_t* __restrict__ x,
+                                                  const float* __restrict__ w, const float* __restrict__ dy,
+                                                  uint16_t* __restrict__ dx, float* __restrict__ dw,
+                                                  float* __restrict__ db) {
+  __shared__ float part[256][9];
+  const int groups = C / 8, lanes = 256 / groups;
+  const int g = threadIdx.x % groups, pl = threadIdx.x / groups;
+  float wv[8], acc[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    wv[j] = w[8 * g + j];
+    acc[j] = 0.f;
+  }
+  float accb = 0.f;
+  for (int64_t p = (int64_t)blockIdx.x * lanes + pl; p < npix; p += (int64_t)gridDim.x * lanes) {
+    const float gy = dy[p];
+    if (g == 0) accb += gy;
+    const uint4 v = *reinterpret_cast<const uint4*>(x + p * C + 8 * g);
+    const uint16_t* e = reinterpret_cast<const uint16_t*>(&v);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc[j] += gy * bf2f(e[j]);
+    if (dx) {
+      uint16_t o[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) o[j] = f2bf(gy * wv[j]);
+      *reinterpret_cast<uint4*>(dx + p * C + 8 * g) = *reinterpret_cast<const uint4*>(o);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 8; ++j) part[threadIdx.x][j] = acc[j];
+  part[threadIdx.x][8] = accb;
+  __syncthreads();
+  for (int c = threadIdx.x; c < C; c += 256) {
+    float sum = 0.f;
+    for (int l = 0; l < lanes; ++l) sum += part[l * groups + c / 8][c % 8];
+    atomicAdd(&dw[c], sum);
+  }
+  if (threadIdx.x == 0) {
+    float sb = 0.f;
+    for (int l = 0; l < lanes; ++l) sb += part[l * groups][8];
+    atomicAdd(db, sb);
+  }
+}
+
 // f32 planes -> bf16 NHWC with C channels (planes first, zeros after)
 __global__ void k_pack_input(int64_t npix, int nplanes, int C, const float* __restrict__ planes,
                              uint16_t* __restrict__ x) {
@@ -764,11 +817,11 @@ int launch_conv3x3_wgrad(pff_ctx* ctx, int n_img, int H, int W, int Cin, int Cou
   const unsigned gx = (unsigned)((nrows + kWgRows - 1) / kWgRows);
   const bool aligned = ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(dy)) & 15) == 0;
   if (W % 32 == 0 && aligned) {
-    // chunks of row segments: about 1024 blocks in all (partial sums 36 KB each)
+    // chunks of row segments: about kWgBlocks blocks in all
     const int seg = W % 64 == 0 ? 64 : 32;
     const int64_t nseg = nrows * (W / seg);
     const int ntile = (Cout / 32) * (Cin / 32);
-    int64_t chunks = std::min<int64_t>(std::max<int64_t>((1024 + ntile - 1) / ntile, 1), nseg);
+    int64_t chunks = std::min<int64_t>(std::max<int64_t>((kWgBlocks + ntile - 1) / ntile, 1), nseg);
     const int64_t per = (nseg + chunks - 1) / chunks;
     chunks = (nseg + per - 1) / per;
     void* p;
@@ -846,8 +899,13 @@ int launch_head(pff_ctx* ctx, int64_t npix, int C, const uint16_t* x, const floa
 int launch_head_bwd(pff_ctx* ctx, int64_t npix, int C, const uint16_t* x, const float* w, const float* dy,
                     uint16_t* dx, float* dw, float* db) {
   const int64_t nb = (npix + 255) / 256;
-  hipLaunchKernelGGL(k_head_bwd, dim3((unsigned)(nb < 2048 ? (nb < 1 ? 1 : nb) : 2048)), dim3(256), 0, ctx->stream,
-                     npix, C, x, w, dy, dx, dw, db);
+  const bool vec = C % 8 == 0 && (256 * 8) % C == 0 &&
+                   ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(dx)) & 15) == 0;
+  if (vec)
+    hipLaunchKernelGGL(k_head_bwd8, dim3(512), dim3(256), 0, ctx->stream, npix, C, x, w, dy, dx, dw, db);
+  else
+    hipLaunchKernelGGL(k_head_bwd, dim3((unsigned)(nb < 2048 ? (nb < 1 ? 1 : nb) : 2048)), dim3(256), 0, ctx->stream,
+                       npix, C, x, w, dy, dx, dw, db);
   return check_hip(hipGetLastError(), "head_bwd");
 }
 
