@@ -683,51 +683,157 @@ __device__ __forceinline__ bool zprime_cell(const GridD& g, int i, int row, int 
   return true;
 }
 
+// z', border flag and offer of a cell whose neighbours' elevations, labels
+// and link validity are already in registers (d = 0 up, 1 left, 2 right,
+// 3 down, the order of neighbour()): the arithmetic of zprime_cell
+__device__ __forceinline__ bool zprime_cell_held(int N, int i, float zi, int bi, const float (&zj)[4],
+                                                 const int (&bj)[4], const bool (&ok)[4], uint8_t* __restrict__ isb,
+                                                 float* __restrict__ zp, int& keyb, int64_t& res) {
+  bool border = false;
+  float zn = 1e9f;
+  int nbj[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    nbj[k] = -1;
+    if (ok[k] && bj[k] != bi) {
+      border = true;
+      zn = fminf(zn, zj[k]);
+      nbj[k] = bj[k];
+    }
+  }
+  isb[i] = border ? 1 : 0;
+  const float zpi = border ? fmaxf(zi, zn) : 1e9f;
+  zp[i] = zpi;
+  if (!border || bi == 0 || bi >= N) return false;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (nbj[k] < 0) continue;
+    const int64_t c = pack_key(zpi, nbj[k]);
+    res = c < res ? c : res;
+  }
+  if (res != INVALID_KEY) keyb = bi;
+  return true;
+}
+
 // passes 1+2 fused (lakeflow.py:132-194) over the active tiles: z', border
 // flag and the packed minimum of (z'_i, bid_j) per basin; the border cells of
-// closed basins are appended to blist in row-major order (blist may be null)
+// closed basins are appended to blist in row-major order (blist may be null).
+// One block per 64x64 tile; wave w takes rows 16w..16w+15 of the tile.  A wave
+// whose rows and columns, with a one-cell frame, lie inside the grid loads
+// every elevation, label (and code) it needs before its first store (on gfx9
+// a wait for a load also waits for every store issued before it) and takes
+// the left and right neighbours from the neighbour lanes (the frame columns
+// by readlane); there every link is the plain D4 one (modes 0-2) or valid
+// when both codes are nonzero (modes 3, 4), as neighbour() gives away from
+// the border.  Other waves go cell by cell through neighbour().
+constexpr int kZRows = TSL / 4;  // rows per wave
 template <int MODE>
 __global__ __launch_bounds__(kBT) void k_zprime_min_t(GridD g, int tiles_x, const uint32_t* __restrict__ tact,
                                                       const int32_t* __restrict__ bid, const float* __restrict__ z,
                                                       uint8_t* __restrict__ isb, float* __restrict__ zp,
                                                       int64_t* saddle, int32_t* blist, int64_t* nblist,
                                                       unsigned long long* prof) {
-  // one block per 64x64 tile, one wave per row: one table flush and one list
-  // append per tile
+  static_assert(kZRows <= 32, "frame columns are held in lanes 0..kZRows-1 and 32..31+kZRows");
+  constexpr bool kCodes = MODE == 3 || MODE == 4;
   __shared__ SaddleTable t;
   int bx_, by_;
   xcd_block(bx_, by_);
   if (tact && tact[by_ * tiles_x + bx_] == 0u) return;  // uniform per block
   saddle_table_init(t);
-  constexpr int R = TSL / 4;
+  constexpr int R = kZRows;
   int32_t lv[R];
   bool lt[R];
-  const int col = bx_ * 64 + threadIdx.x;
-  // no early exit inside the row loop: the whole wave (one row of 64 cells)
-  // takes part in the segmented minimum
+  const int lane = threadIdx.x;
+  const int col0 = bx_ * 64, col = col0 + lane;
+  const int r0 = by_ * TSL + kZRows * threadIdx.y;
+  const int nx = g.nx;
+  if (col0 > 0 && col0 + 64 < nx && r0 > 0 && r0 + R < g.ny) {  // uniform per wave
+    float zz[R + 2];
+    int bb[R + 2];
+    uint32_t cc[R + 2];
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int row = by_ * TSL + threadIdx.y + 4 * r;
-    lv[r] = 0;
-    lt[r] = false;
-    if (row >= g.ny) continue;  // uniform per wave
-    const bool inside = col < g.nx;
-    const int i = inside ? row * g.nx + col : 0;
-    int keyb = -1;
-    int64_t res = INVALID_KEY;
-    if (inside) {
-      const float zi = z[i];
-      if (can_leave<MODE>(g, i, row, col)) {
+    for (int k = 0; k < R + 2; ++k) {
+      const size_t c = (size_t)(r0 - 1 + k) * nx + col;
+      zz[k] = z[c];
+      bb[k] = bid[c];
+      if constexpr (kCodes) cc[k] = g.B[c];
+    }
+    // lanes 0..R-1: column col0-1 of row r0+lane; lanes 32..: column col0+64
+    const int hk = lane & 31;
+    const size_t hcell = (size_t)(r0 + hk) * nx + (lane < 32 ? col0 - 1 : col0 + 64);
+    float hz = 0.f;
+    int hb = 0;
+    uint32_t hc = 0u;
+    if (hk < R) {
+      hz = z[hcell];
+      hb = bid[hcell];
+      if constexpr (kCodes) hc = g.B[hcell];
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int i = (r0 + k) * nx + col;
+      const float zi = zz[k + 1];
+      const int bi = bb[k + 1];
+      float zl = __shfl_up(zi, 1), zr = __shfl_down(zi, 1);
+      int bl = __shfl_up(bi, 1), br = __shfl_down(bi, 1);
+      zl = lane == 0 ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(hz), k)) : zl;
+      zr = lane == 63 ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(hz), 32 + k)) : zr;
+      bl = lane == 0 ? __builtin_amdgcn_readlane(hb, k) : bl;
+      br = lane == 63 ? __builtin_amdgcn_readlane(hb, 32 + k) : br;
+      bool ok[4] = {true, true, true, true};
+      bool leave = false;
+      if constexpr (kCodes) {
+        const uint32_t ci = cc[k + 1];
+        uint32_t cl = __shfl_up(ci, 1), cr = __shfl_down(ci, 1);
+        cl = lane == 0 ? (uint32_t)__builtin_amdgcn_readlane((int)hc, k) : cl;
+        cr = lane == 63 ? (uint32_t)__builtin_amdgcn_readlane((int)hc, 32 + k) : cr;
+        ok[0] = ci != 0 && cc[k] != 0;
+        ok[1] = ci != 0 && cl != 0;
+        ok[2] = ci != 0 && cr != 0;
+        ok[3] = ci != 0 && cc[k + 2] != 0;
+        leave = MODE == 4 ? ci != 1 : ci == 3;  // can_leave()
+      }
+      int keyb = -1;
+      int64_t res = INVALID_KEY;
+      lv[k] = i;
+      lt[k] = false;
+      if (leave) {
         isb[i] = 0;
         zp[i] = zi;
       } else {
-        lt[r] = zprime_cell<MODE>(g, i, row, col, zi, bid, z, isb, zp, keyb, res);
-        lv[r] = i;
+        const float zj[4] = {zz[k], zl, zr, zz[k + 2]};
+        const int bj[4] = {bb[k], bl, br, bb[k + 2]};
+        lt[k] = zprime_cell_held(g.N, i, zi, bi, zj, bj, ok, isb, zp, keyb, res);
       }
+      saddle_offer(t, keyb, res, saddle);
     }
-    saddle_offer(t, keyb, res, saddle);
+  } else {
+    // no early exit inside the row loop: the whole wave (one row of 64
+    // cells) takes part in the segmented minimum
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int row = r0 + r;
+      lv[r] = 0;
+      lt[r] = false;
+      if (row >= g.ny) continue;  // uniform per wave
+      const bool inside = col < nx;
+      const int i = inside ? row * nx + col : 0;
+      int keyb = -1;
+      int64_t res = INVALID_KEY;
+      if (inside) {
+        const float zi = z[i];
+        if (can_leave<MODE>(g, i, row, col)) {
+          isb[i] = 0;
+          zp[i] = zi;
+        } else {
+          lt[r] = zprime_cell<MODE>(g, i, row, col, zi, bid, z, isb, zp, keyb, res);
+          lv[r] = i;
+        }
+      }
+      saddle_offer(t, keyb, res, saddle);
+    }
   }
-  if (blist) block_append_ordered<R>(lv, lt, blist, nblist);
+  if (blist) block_append_wave_major<R>(lv, lt, blist, nblist);
   if (prof) {
     // profiling: algorithmic bytes of the tile (z, bid in; z', border flag
     // out; 4 per listed border cell)

@@ -96,6 +96,44 @@ __device__ __forceinline__ void block_append_ordered(const int32_t (&val)[R], co
   __syncthreads();
 }
 
+// The same in (wave, round, lane) order: a wave that visits R consecutive
+// rows, one per round, with the waves stacked, also appends row by row.
+template <int R, int BT = kBT>
+__device__ __forceinline__ void block_append_wave_major(const int32_t (&val)[R], const bool (&take)[R],
+                                                        int32_t* list, int64_t* count) {
+  constexpr int W = BT / 64;
+  __shared__ int off_s[R * W];
+  __shared__ int64_t base_s;
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  const int lane = tid & 63, wid = tid >> 6;
+  uint64_t mask[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    mask[r] = __ballot(take[r]);
+    if (lane == 0) off_s[wid * R + r] = __popcll(mask[r]);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int acc = 0;
+    for (int k = 0; k < R * W; ++k) {
+      const int c = off_s[k];
+      off_s[k] = acc;
+      acc += c;
+    }
+    base_s = acc ? (int64_t)atomicAdd((unsigned long long*)count, (unsigned long long)acc) : 0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (take[r]) {
+      const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask[r] >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)mask[r], 0u));
+      list[base_s + off_s[wid * R + r] + below] = val[r];
+    }
+  }
+  __syncthreads();
+}
+
 // Compact the taken values of every thread into this block's own segment
 // (seg, room for every item of the block) and store the count: no global
 // atomic.  Appends of thousands of blocks to one shared counter serialise at
