@@ -282,10 +282,15 @@ int pff_conv3x3_fwd(pff_ctx* ctx, int32_t n, int32_t h, int32_t w, int32_t cin, 
    src->mode 0: x [n][h][w][cin] (as pff_conv3x3_fwd); 1: concat(upsample2(x),
    skip) with x [n][h/2][w/2][ca] and skip [n][h][w][cin-ca] (pff_upsample2 +
    pff_concat); 2: the f32 planes [nplanes][n*h*w] rounded to bf16 and zero
-   padded to cin = 32 channels (pff_pack_input).  head_out non-NULL (cout =
-   32): y is not written and head_out[p] = head_b[0] + sum_c head_w[c] *
-   act[p][c] (pff_head1x1 of the activated output).  Same bits as the
-   unfused sequence.  Needs w a multiple of 32 and h a multiple of 16. */
+   padded to cin = 32 channels (pff_pack_input); 3: x [n][h][w][cin] of which
+   only channels 0..7 are read (the others taken as zero; cout = 32, no
+   head).  head_out non-NULL (cout = 32): y is not written and head_out[p] =
+   head_b[0] + sum_c head_w[c] * act[p][c] (pff_head1x1 of the activated
+   output).  Mode 2 with at most 8 planes, cout = 32 and no head sums over
+   (tap, channel 0..7) as mode 3 does, and gives the same bits as
+   pff_pack_input followed by mode 3; otherwise each mode gives the same bits
+   as the unfused sequence.  Needs w a multiple of 32 and h a multiple of
+   16. */
 typedef struct {
   int32_t mode;
   const uint16_t* x;

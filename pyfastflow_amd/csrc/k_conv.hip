@@ -128,7 +128,7 @@ __global__ __launch_bounds__(NW * 64) void k_conv3x3_fwd(int H, int W, int Cin, 
 // 32 channels (pack_input).  With HEAD, the 1x1 head (y = b + sum_c w_c x_c)
 // is applied to the activated output instead of storing it.
 constexpr int kTX = 32;  // tile width (pixels)
-enum ConvSrc : int { SRC_X = 0, SRC_UPCAT = 1, SRC_PLANES = 2 };
+enum ConvSrc : int { SRC_X = 0, SRC_UPCAT = 1, SRC_PLANES = 2, SRC_X8 = 3 };
 
 struct ConvIn {
   const uint16_t* x;     // SRC_X: [n][H][W][Cin]; SRC_UPCAT: [n][H/2][W/2][Ca]
@@ -277,6 +277,110 @@ __global__ __launch_bounds__(256) void k_conv3x3_fwd_t(int nimg, int H, int W, i
         }
     }
   }
+}
+
+// First convolution of the U-Net: at most 8 nonzero input channels (the
+// input planes; SRC_PLANES reads them from the f32 planes, SRC_X8 from the
+// first 8 channels of x) and 32 output channels.  k = (tap, channel): a
+// k-step of 32 holds 4 taps x 8 channels (lane group g takes tap 4*ks + g),
+// so the 9 taps take 3 MFMA k-steps instead of 9 of 32 channels with at most
+// 8 nonzero.  The input window keeps 8 channels (16 bytes) per pixel.  Tile:
+// 16 rows x 32 columns, 4 waves of 4 rows (8 pixel tiles of 16 each), output
+// staged through LDS as in k_conv3x3_fwd_t.  SRC_PLANES and SRC_X8 on the
+// same values give the same bits.
+constexpr int kFirstTH = 16;
+template <int SRC>
+__global__ __launch_bounds__(256) void k_conv3x3_first(int nimg, int H, int W, int Cin, ConvIn in,
+                                                       const uint16_t* __restrict__ wf,
+                                                       const float* __restrict__ bias, int relu,
+                                                       uint16_t* __restrict__ y) {
+  constexpr int CO = 32, NJ = 2, TH = kFirstTH, WR = TH / 4, NA = 2 * WR;
+  constexpr int SWd = kTX + 2, SN = (TH + 2) * SWd;
+  constexpr int OP = CO + 8;
+  constexpr int BUF = TH * kTX * OP > SN * 8 ? TH * kTX * OP : SN * 8;
+  __shared__ __attribute__((aligned(16))) uint16_t buf[BUF];  // input window [SN][8], then the output [pixel][OP]
+  __shared__ __attribute__((aligned(16))) uint16_t ws[3][CO][kRow];  // [k-step][co][4 taps x 8 channels]
+  const int tiles_x = W / kTX;
+  const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+  const int img = blockIdx.y;
+  const int x0 = tx * kTX, y0 = ty * TH;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int kq = 8 * (lane >> 4);
+  const size_t img_base = (size_t)img * H * W;
+  for (int px = threadIdx.x; px < SN; px += 256) {
+    const int r = px / SWd, c = px - r * SWd;
+    const int gy = y0 - 1 + r, gx = x0 - 1 + c;
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+    if (gy >= 0 && gy < H && gx >= 0 && gx < W) {
+      if constexpr (SRC == SRC_X8)
+        v = *reinterpret_cast<const uint4*>(in.x + ((img_base + (size_t)gy * W + gx) * Cin));
+      else
+        v = conv_load8<SRC_PLANES>(in, img, nimg, H, W, Cin, gy, gx, 0);
+    }
+    *reinterpret_cast<uint4*>(&buf[px * 8]) = v;
+  }
+  // weights: channels 0..7 of tap 4*ks + g at [ks][co][8g], zero past tap 8
+  for (int it = threadIdx.x; it < 3 * CO * 4; it += 256) {
+    const int g = it & 3, c = (it >> 2) % CO, ks = it / (4 * CO);
+    const int tap = 4 * ks + g;
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+    if (tap < 9) v = *reinterpret_cast<const uint4*>(wf + ((size_t)tap * 32 + c) * Cin);
+    *reinterpret_cast<uint4*>(&ws[ks][c][8 * g]) = v;
+  }
+  __syncthreads();
+  f32x4 acc[NA][NJ];
+#pragma unroll
+  for (int t = 0; t < NA; ++t)
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) acc[t][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int ks = 0; ks < 3; ++ks) {
+    const int tap = 4 * ks + (lane >> 4);
+    const bool live = tap < 9;
+    const int ky = live ? tap / 3 : 0, kx = live ? tap - 3 * (tap / 3) : 0;
+    bf16x8 b[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) b[j] = *reinterpret_cast<const bf16x8*>(&ws[ks][16 * j + (lane & 15)][kq]);
+#pragma unroll
+    for (int t = 0; t < NA; ++t) {
+      const int qy = wave * WR + (t >> 1), qx = 16 * (t & 1) + (lane & 15);
+      uint4 av = *reinterpret_cast<const uint4*>(&buf[((qy + ky) * SWd + qx + kx) * 8]);
+      if (!live) av = make_uint4(0u, 0u, 0u, 0u);
+      const bf16x8 a = *reinterpret_cast<const bf16x8*>(&av);
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) acc[t][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[t][j], 0, 0, 0);
+    }
+  }
+  __syncthreads();  // the window is read; its space takes the output
+  uint16_t* os = buf;
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const int c = 16 * j + (lane & 15);
+    const float bv = bias ? bias[c] : 0.f;
+#pragma unroll
+    for (int t = 0; t < NA; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int q = (wave * WR + (t >> 1)) * kTX + 16 * (t & 1) + 4 * (lane >> 4) + r;
+        float v = acc[t][j][r] + bv;
+        if (relu) v = fmaxf(v, 0.f);
+        os[q * OP + c] = f2bf(v);
+      }
+  }
+  __syncthreads();
+  for (int it = threadIdx.x; it < TH * kTX * (CO / 8); it += 256) {
+    const int q = it / (CO / 8), c8 = it - q * (CO / 8);
+    const size_t p = img_base + (size_t)(y0 + q / kTX) * W + x0 + (q % kTX);
+    *reinterpret_cast<uint4*>(y + p * CO + 8 * c8) = *reinterpret_cast<const uint4*>(&os[q * OP + 8 * c8]);
+  }
+}
+
+template <int SRC>
+int launch_conv_first(pff_ctx* ctx, int n_img, int H, int W, int Cin, const ConvIn& in, const uint16_t* wf,
+                      const float* bias, int relu, uint16_t* y) {
+  hipLaunchKernelGGL((k_conv3x3_first<SRC>), dim3((W / kTX) * (H / kFirstTH), n_img, 1), dim3(256), 0, ctx->stream,
+                     n_img, H, W, Cin, in, wf, bias, relu, y);
+  return check_hip(hipGetLastError(), "conv3x3_fwd (first layer)");
 }
 
 template <int SRC, bool HEAD>
@@ -983,10 +1087,16 @@ int pff_conv3x3_fwd_fused(pff_ctx* ctx, int32_t n, int32_t h, int32_t w, int32_t
     case SRC_PLANES:
       if (!in.planes || cin != 32 || in.nplanes < 1 || in.nplanes > 32)
         return fail(PFF_EINVAL, "conv3x3_fwd_fused: planes need cin = 32 and 1..32 planes");
+      if (in.nplanes <= 8 && cout == 32 && !head)
+        return launch_conv_first<SRC_PLANES>(ctx, n, h, w, cin, in, wf, bias, relu, y);
       return head ? launch_conv_t<SRC_PLANES, true>(ctx, n, h, w, cin, cout, in, wf, bias, relu, y)
                   : launch_conv_t<SRC_PLANES, false>(ctx, n, h, w, cin, cout, in, wf, bias, relu, y);
+    case SRC_X8:
+      if (!in.x || cout != 32 || head)
+        return fail(PFF_EINVAL, "conv3x3_fwd_fused: mode 3 needs x, cout = 32 and no head");
+      return launch_conv_first<SRC_X8>(ctx, n, h, w, cin, in, wf, bias, relu, y);
     default:
-      return fail(PFF_EINVAL, "conv3x3_fwd_fused: mode must be 0, 1 or 2");
+      return fail(PFF_EINVAL, "conv3x3_fwd_fused: mode must be 0, 1, 2 or 3");
   }
 }
 

@@ -21,7 +21,7 @@ def _p(t):
 
 class _Conv3x3(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, x, w, b, relu: bool):
+    def forward(ctx, x, w, b, relu: bool, first8: bool = False):
         n, h, wd, cin = x.shape
         cout = w.shape[0]
         if w.shape != (cout, cin, 3, 3):
@@ -33,7 +33,16 @@ class _Conv3x3(torch.autograd.Function):
         wt = torch.empty_like(wf)
         _lib.call("pff_conv_pack_weights", c, cin, cout, _p(w), _p(wf), _p(wt))
         y = torch.empty((n, h, wd, cout), dtype=torch.bfloat16, device=x.device)
-        _lib.call("pff_conv3x3_fwd", c, n, h, wd, cin, cout, _p(x), _p(wf), _p(b), int(relu), _p(y))
+        if first8:
+            # channels 8.. of x are zero: the first-layer kernel (same bits as
+            # the fused inference from the planes)
+            import ctypes
+
+            src = _lib.PffConvSrc(3, _p(x), None, None, 0, 0, None, None, None)
+            _lib.call("pff_conv3x3_fwd_fused", c, n, h, wd, cin, cout, ctypes.byref(src), _p(wf), _p(b), int(relu),
+                      _p(y))
+        else:
+            _lib.call("pff_conv3x3_fwd", c, n, h, wd, cin, cout, _p(x), _p(wf), _p(b), int(relu), _p(y))
         ctx.save_for_backward(x, y, wt)
         ctx.relu = relu
         ctx.shape = (n, h, wd, cin, cout)
@@ -55,7 +64,7 @@ class _Conv3x3(torch.autograd.Function):
         dw = torch.zeros((cout, cin, 3, 3), dtype=torch.float32, device=x.device)
         db = torch.zeros(cout, dtype=torch.float32, device=x.device)
         _lib.call("pff_conv3x3_bwd_weight", c, n, h, wd, cin, cout, _p(x), _p(g), _p(dw), _p(db))
-        return dx, dw, db, None
+        return dx, dw, db, None, None
 
 
 class _AvgPool2(torch.autograd.Function):
@@ -141,10 +150,14 @@ class _Head1x1(torch.autograd.Function):
         return dx, dw, db
 
 
-def conv3x3(x, w, b, relu: bool = True):
+def conv3x3(x, w, b, relu: bool = True, first8: bool = False):
     """3x3 convolution (stride 1, zero padding 1) of NHWC bf16 x with f32
-    weights w[cout][cin][3][3] and bias b[cout]; optional fused ReLU."""
-    return _Conv3x3.apply(x, w, b, relu)
+    weights w[cout][cin][3][3] and bias b[cout]; optional fused ReLU.
+    first8: x holds zeros past channel 7 (the packed input planes) and the
+    first-layer kernel runs (cout 32, shapes tiled_ok; else the plain one)."""
+    n, h, wd, _ = x.shape
+    use8 = first8 and w.shape[0] == 32 and tiled_ok(h, wd)
+    return _Conv3x3.apply(x, w, b, relu, use8)
 
 
 def avgpool2(x):

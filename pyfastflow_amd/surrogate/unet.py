@@ -56,7 +56,9 @@ class UNet(nn.Module):
         for k in range(self.levels):
             if k > 0:
                 x = ops.avgpool2(x)
-            x = ops.conv3x3(x, self.w[i], self.b[i])
+            # the first convolution sees at most 8 nonzero channels when the
+            # planes are few: the first-layer kernel, as the fused inference
+            x = ops.conv3x3(x, self.w[i], self.b[i], first8=(i == 0 and planes.shape[1] <= 8))
             x = ops.conv3x3(x, self.w[i + 1], self.b[i + 1])
             i += 2
             skips.append(x)
@@ -93,13 +95,17 @@ class UNet(nn.Module):
         return x
 
     def flops_per_pixel(self) -> float:
-        """Multiply-adds x 2 of the 3x3 convolutions and the head, per input pixel."""
+        """Multiply-adds x 2 of the 3x3 convolutions and the head, per input
+        pixel; the first convolution counts its real input planes, not the
+        zero channels they are padded with."""
         f = 0.0
         i = 0
         for k in range(self.levels):
             scale = 4.0 ** (-k)
             for _ in range(2):
                 cin, cout = self.shapes[i]
+                if i == 0:
+                    cin = min(cin, self.in_planes)
                 f += 2 * 9 * cin * cout * scale
                 i += 1
         for k in range(self.levels - 2, -1, -1):

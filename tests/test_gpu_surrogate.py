@@ -91,6 +91,40 @@ def test_conv3x3_backward(pf, n, h, w, cin, cout):
     _close(bg.grad, br.grad, rel=1e-3, absf=1e-3, what="db")
 
 
+@pytest.mark.parametrize("n,h,w,p", [(1, 16, 32, 6), (2, 16, 64, 1), (1, 32, 96, 8), (3, 48, 32, 3)])
+def test_conv3x3_first_layer(pf, n, h, w, p):
+    """The first-layer kernel (k = tap x 8 channels; x zero past channel 7)
+    against float64, forward and gradients, and the fused form that reads
+    the f32 planes gives the same bits."""
+    from pyfastflow_amd.surrogate import ops
+
+    print(f"case conv first {n}x{h}x{w} planes={p}", flush=True)
+    g = torch.Generator().manual_seed(100 + p + h)
+    planes = torch.randn(n, p, h, w, generator=g)
+    x = ops.pack_input(planes.cuda(), 32).cpu()
+    assert torch.count_nonzero(x[..., p:]) == 0
+    wt = torch.randn(32, 32, 3, 3, generator=g) * 0.1
+    b = torch.randn(32, generator=g) * 0.1
+    gy = _bf(torch.randn(n, h, w, 32, generator=g))
+    xg = x.cuda()
+    wg = wt.cuda().requires_grad_(True)
+    bg = b.cuda().requires_grad_(True)
+    y = ops.conv3x3(xg, wg, bg, relu=True, first8=True)
+    y.backward(gy.cuda())
+    mask = (y.detach().cpu().to(torch.float64) > 0).permute(0, 3, 1, 2)
+    xr = _ref64(x).permute(0, 3, 1, 2)
+    wr = _ref64(_bf(wt)).requires_grad_(True)
+    br = _ref64(b).requires_grad_(True)
+    yr = F.conv2d(xr, wr, br, padding=1)
+    _close(y, yr.clamp_min(0).permute(0, 2, 3, 1), what="first-layer forward")
+    yr.backward(_ref64(gy).permute(0, 3, 1, 2) * mask)
+    _close(wg.grad, wr.grad, rel=1e-3, absf=1e-3, what="first-layer dw")
+    _close(bg.grad, br.grad, rel=1e-3, absf=1e-3, what="first-layer db")
+    with torch.no_grad():
+        yf = ops.conv3x3_fused(wt.cuda(), b.cuda(), planes=planes.cuda())
+    assert torch.equal(yf.view(torch.int16).cpu(), y.detach().view(torch.int16).cpu())
+
+
 def test_pool_upsample_concat_head(pf):
     from pyfastflow_amd.surrogate import ops
 
