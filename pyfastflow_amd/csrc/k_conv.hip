@@ -798,6 +798,131 @@ __global__ void k_concat(int64_t npix, int Ca, int Cb, const uint16_t* __restric
   }
 }
 
+// The same data movement 8 channels at a time (C a multiple of 8, 16-byte
+// aligned arrays): one 16-byte access per pixel and channel group, the pixel
+// indices computed once per group; per element the arithmetic of the kernels
+// above, so the results are the same bits.
+__device__ __forceinline__ uint16_t u4h(const uint4& v, int j) {
+  const uint32_t w = j < 2 ? v.x : j < 4 ? v.y : j < 6 ? v.z : v.w;
+  return (uint16_t)((j & 1) ? (w >> 16) : (w & 0xFFFFu));
+}
+__device__ __forceinline__ uint4 u4pack(const uint16_t (&e)[8]) {
+  return make_uint4(e[0] | ((uint32_t)e[1] << 16), e[2] | ((uint32_t)e[3] << 16), e[4] | ((uint32_t)e[5] << 16),
+                    e[6] | ((uint32_t)e[7] << 16));
+}
+
+__global__ void k_relu_bwd8(int64_t n8, const uint4* __restrict__ y, uint4* __restrict__ dy) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n8; e += (int64_t)gridDim.x * blockDim.x) {
+    const uint4 yv = y[e], dv = dy[e];
+    uint16_t o[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) o[j] = bf2f(u4h(yv, j)) > 0.f ? u4h(dv, j) : (uint16_t)0;
+    dy[e] = u4pack(o);
+  }
+}
+
+__global__ void k_avgpool2_8(int n_img, int H, int W, int C, const uint16_t* __restrict__ x, uint16_t* __restrict__ y) {
+  const int Ho = H / 2, Wo = W / 2, G = C / 8;
+  const int64_t n = (int64_t)n_img * Ho * Wo * G;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int g = (int)(e % G);
+    const int64_t p = e / G;
+    const int xo = (int)(p % Wo);
+    const int64_t q = p / Wo;
+    const int yo = (int)(q % Ho);
+    const int64_t img = q / Ho;
+    const size_t b = (((size_t)img * H + 2 * yo) * W + 2 * xo) * C + 8 * g;
+    const uint4 v0 = *reinterpret_cast<const uint4*>(x + b);
+    const uint4 v1 = *reinterpret_cast<const uint4*>(x + b + C);
+    const uint4 v2 = *reinterpret_cast<const uint4*>(x + b + (size_t)W * C);
+    const uint4 v3 = *reinterpret_cast<const uint4*>(x + b + (size_t)W * C + C);
+    uint16_t o[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const float sm = bf2f(u4h(v0, j)) + bf2f(u4h(v1, j)) + bf2f(u4h(v2, j)) + bf2f(u4h(v3, j));
+      o[j] = f2bf(0.25f * sm);
+    }
+    *reinterpret_cast<uint4*>(y + (size_t)p * C + 8 * g) = u4pack(o);
+  }
+}
+
+__global__ void k_avgpool2_bwd8(int n_img, int H, int W, int C, const uint16_t* __restrict__ dy,
+                                uint16_t* __restrict__ dx) {
+  const int Ho = H / 2, Wo = W / 2, G = C / 8;
+  const int64_t n = (int64_t)n_img * H * W * G;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int g = (int)(e % G);
+    const int64_t p = e / G;
+    const int xx = (int)(p % W);
+    const int64_t q = p / W;
+    const int yy = (int)(q % H);
+    const int64_t img = q / H;
+    const uint4 v = *reinterpret_cast<const uint4*>(dy + (((size_t)img * Ho + yy / 2) * Wo + xx / 2) * C + 8 * g);
+    uint16_t o[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) o[j] = f2bf(0.25f * bf2f(u4h(v, j)));
+    *reinterpret_cast<uint4*>(dx + (size_t)p * C + 8 * g) = u4pack(o);
+  }
+}
+
+__global__ void k_upsample2_8(int n_img, int H, int W, int C, const uint16_t* __restrict__ x, uint16_t* __restrict__ y) {
+  const int Ho = 2 * H, Wo = 2 * W, G = C / 8;
+  const int64_t n = (int64_t)n_img * Ho * Wo * G;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int g = (int)(e % G);
+    const int64_t p = e / G;
+    const int xo = (int)(p % Wo);
+    const int64_t q = p / Wo;
+    const int yo = (int)(q % Ho);
+    const int64_t img = q / Ho;
+    *reinterpret_cast<uint4*>(y + (size_t)p * C + 8 * g) =
+        *reinterpret_cast<const uint4*>(x + (((size_t)img * H + yo / 2) * W + xo / 2) * C + 8 * g);
+  }
+}
+
+__global__ void k_upsample2_bwd8(int n_img, int H, int W, int C, const uint16_t* __restrict__ dy,
+                                 uint16_t* __restrict__ dx) {
+  const int Wo = 2 * W, G = C / 8;
+  const int64_t n = (int64_t)n_img * H * W * G;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int g = (int)(e % G);
+    const int64_t p = e / G;
+    const int xx = (int)(p % W);
+    const int64_t q = p / W;
+    const int yy = (int)(q % H);
+    const int64_t img = q / H;
+    const size_t b = (((size_t)img * 2 * H + 2 * yy) * Wo + 2 * xx) * C + 8 * g;
+    const uint4 v0 = *reinterpret_cast<const uint4*>(dy + b);
+    const uint4 v1 = *reinterpret_cast<const uint4*>(dy + b + C);
+    const uint4 v2 = *reinterpret_cast<const uint4*>(dy + b + (size_t)Wo * C);
+    const uint4 v3 = *reinterpret_cast<const uint4*>(dy + b + (size_t)Wo * C + C);
+    uint16_t o[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      o[j] = f2bf(bf2f(u4h(v0, j)) + bf2f(u4h(v1, j)) + bf2f(u4h(v2, j)) + bf2f(u4h(v3, j)));
+    *reinterpret_cast<uint4*>(dx + (size_t)p * C + 8 * g) = u4pack(o);
+  }
+}
+
+// concatenation (or, backward, the split) 8 channels at a time; Ca, Cb
+// multiples of 8
+__global__ void k_concat8(int64_t npix, int Ca, int Cb, const uint16_t* __restrict__ a,
+                          const uint16_t* __restrict__ b, uint16_t* __restrict__ y, int backward) {
+  const int Ga = Ca / 8, G = (Ca + Cb) / 8, C = Ca + Cb;
+  const int64_t n = npix * G;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int g = (int)(e % G);
+    const int64_t p = e / G;
+    uint16_t* src_dst = g < Ga ? const_cast<uint16_t*>(a) + (size_t)p * Ca + 8 * g
+                               : const_cast<uint16_t*>(b) + (size_t)p * Cb + 8 * (g - Ga);
+    uint4* yv = reinterpret_cast<uint4*>(y + (size_t)p * C + 8 * g);
+    if (!backward)
+      *yv = *reinterpret_cast<const uint4*>(src_dst);
+    else
+      *reinterpret_cast<uint4*>(src_dst) = *yv;
+  }
+}
+
 // 1x1 head to one f32 channel: y[p] = b + sum_c x[p][c] w[c]; backward gives
 // dx (bf16), and accumulates dw, db
 __global__ __launch_bounds__(256) void k_head(int64_t npix, int C, const uint16_t* __restrict__ x,
@@ -965,15 +1090,26 @@ int launch_pack_weights(pff_ctx* ctx, int Cin, int Cout, const float* w, uint16_
   return check_hip(hipGetLastError(), "pack_weights");
 }
 
+static bool a16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+
 int launch_relu_bwd(pff_ctx* ctx, int64_t n, const uint16_t* y, uint16_t* dy) {
-  hipLaunchKernelGGL(k_relu_bwd, dim3(ew_grid(n)), dim3(256), 0, ctx->stream, n, y, dy);
+  if (n % 8 == 0 && a16(y) && a16(dy))
+    hipLaunchKernelGGL(k_relu_bwd8, dim3(ew_grid(n / 8)), dim3(256), 0, ctx->stream, n / 8,
+                       reinterpret_cast<const uint4*>(y), reinterpret_cast<uint4*>(dy));
+  else
+    hipLaunchKernelGGL(k_relu_bwd, dim3(ew_grid(n)), dim3(256), 0, ctx->stream, n, y, dy);
   return check_hip(hipGetLastError(), "relu_bwd");
 }
 
 int launch_avgpool2(pff_ctx* ctx, int n_img, int H, int W, int C, const uint16_t* x, uint16_t* y, int backward) {
   const int64_t n = backward ? (int64_t)n_img * H * W * C : (int64_t)n_img * (H / 2) * (W / 2) * C;
-  if (backward)
+  const bool vec = C % 8 == 0 && a16(x) && a16(y);
+  if (backward && vec)
+    hipLaunchKernelGGL(k_avgpool2_bwd8, dim3(ew_grid(n / 8)), dim3(256), 0, ctx->stream, n_img, H, W, C, x, y);
+  else if (backward)
     hipLaunchKernelGGL(k_avgpool2_bwd, dim3(ew_grid(n)), dim3(256), 0, ctx->stream, n_img, H, W, C, x, y);
+  else if (vec)
+    hipLaunchKernelGGL(k_avgpool2_8, dim3(ew_grid(n / 8)), dim3(256), 0, ctx->stream, n_img, H, W, C, x, y);
   else
     hipLaunchKernelGGL(k_avgpool2, dim3(ew_grid(n)), dim3(256), 0, ctx->stream, n_img, H, W, C, x, y);
   return check_hip(hipGetLastError(), "avgpool2");
@@ -981,8 +1117,13 @@ int launch_avgpool2(pff_ctx* ctx, int n_img, int H, int W, int C, const uint16_t
 
 int launch_upsample2(pff_ctx* ctx, int n_img, int H, int W, int C, const uint16_t* x, uint16_t* y, int backward) {
   const int64_t n = backward ? (int64_t)n_img * H * W * C : (int64_t)n_img * 4 * H * W * C;
-  if (backward)
+  const bool vec = C % 8 == 0 && a16(x) && a16(y);
+  if (backward && vec)
+    hipLaunchKernelGGL(k_upsample2_bwd8, dim3(ew_grid(n / 8)), dim3(256), 0, ctx->stream, n_img, H, W, C, x, y);
+  else if (backward)
     hipLaunchKernelGGL(k_upsample2_bwd, dim3(ew_grid(n)), dim3(256), 0, ctx->stream, n_img, H, W, C, x, y);
+  else if (vec)
+    hipLaunchKernelGGL(k_upsample2_8, dim3(ew_grid(n / 8)), dim3(256), 0, ctx->stream, n_img, H, W, C, x, y);
   else
     hipLaunchKernelGGL(k_upsample2, dim3(ew_grid(n)), dim3(256), 0, ctx->stream, n_img, H, W, C, x, y);
   return check_hip(hipGetLastError(), "upsample2");
@@ -990,8 +1131,12 @@ int launch_upsample2(pff_ctx* ctx, int n_img, int H, int W, int C, const uint16_
 
 int launch_concat(pff_ctx* ctx, int64_t npix, int Ca, int Cb, const uint16_t* a, const uint16_t* b, uint16_t* y,
                   int backward) {
-  hipLaunchKernelGGL(k_concat, dim3(ew_grid(npix * (Ca + Cb))), dim3(256), 0, ctx->stream, npix, Ca, Cb, a, b, y,
-                     backward);
+  if (Ca % 8 == 0 && Cb % 8 == 0 && a16(a) && a16(b) && a16(y))
+    hipLaunchKernelGGL(k_concat8, dim3(ew_grid(npix * ((Ca + Cb) / 8))), dim3(256), 0, ctx->stream, npix, Ca, Cb, a,
+                       b, y, backward);
+  else
+    hipLaunchKernelGGL(k_concat, dim3(ew_grid(npix * (Ca + Cb))), dim3(256), 0, ctx->stream, npix, Ca, Cb, a, b, y,
+                       backward);
   return check_hip(hipGetLastError(), "concat");
 }
 
