@@ -638,6 +638,25 @@ __device__ __forceinline__ void saddle_table_flush(SaddleTable& t, int64_t* sadd
     if (b[k] >= 0 && v[k] < cur[k]) atomicMin((long long*)&saddle[b[k]], v[k]);
 }
 
+// slot of basin b in the table, or -1 when its offers went to the global
+// minimum (the probe sequence of saddle_offer: a basin that found a slot
+// keeps it, one that found none finds none later, as slots are never freed)
+__device__ __forceinline__ int saddle_table_find(const SaddleTable& t, int b) {
+  int slot = (int)(((uint32_t)b * 2654435761u) >> 22);
+  for (int probe = 0; probe < 8; ++probe, slot = (slot + 1) & (kSaddleSlots - 1)) {
+    const int v = t.b[slot];
+    if (v == b) return slot;
+    if (v == -1) return -1;
+  }
+  return -1;
+}
+
+// A tiled z' pass lists every border cell of a closed basin (cell index,
+// below 2^28) and after them, with bit 31 set (kSnCand), the cells that can
+// be saddle nodes; the list pass of the next trip skips the marked entries,
+// the saddle-node pass of this trip takes only them
+constexpr uint32_t kSnCand = 0x80000000u;
+
 // z', border flag and the offer of cell i (not an outlet cell); returns true
 // when i is a border cell of a closed basin (the cells the saddle passes use)
 template <int MODE>
@@ -833,7 +852,33 @@ __global__ __launch_bounds__(kBT) void k_zprime_min_t(GridD g, int tiles_x, cons
       saddle_offer(t, keyb, res, saddle);
     }
   }
-  if (blist) block_append_wave_major<R>(lv, lt, blist, nblist);
+  if (blist) {
+    // saddle-node candidates: a saddle node's key (z'_i, basin across) is
+    // its basin's global minimum, which is at most the tile's minimum and at
+    // least the node's own key, so the tile's minimum has the node's z'.
+    // Listed cells whose z' is the z' of their basin's tile minimum (and
+    // those whose basin went to the global minimum directly, or that made no
+    // offer) are listed again, marked.  Their labels and z' are read back
+    // (this block wrote the z'), which keeps them out of registers.
+    __syncthreads();  // every offer is in the table
+    int cb[R];
+    float cz[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      cb[r] = lt[r] ? bid[lv[r]] : 0;
+      cz[r] = lt[r] ? zp[lv[r]] : 0.f;
+    }
+    bool cand[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      cand[r] = lt[r];
+      if (lt[r]) {
+        const int slot = saddle_table_find(t, cb[r]);
+        if (slot >= 0) cand[r] = cz[r] == key_float((int64_t)t.k[slot]);
+      }
+    }
+    block_append_wave_major2<R>(lv, lt, cand, kSnCand, blist, nblist);
+  }
   if (prof) {
     // profiling: algorithmic bytes of the tile (z, bid in; z', border flag
     // out; 4 per listed border cell)
@@ -875,10 +920,12 @@ __global__ __launch_bounds__(kBT) void k_zprime_list(GridD g, const int32_t* __r
       int64_t res = INVALID_KEY;
       if (e < n) {
         const int i = lin[e];
-        int row, col;
-        rc_of(g, i, row, col);
-        lt[r] = zprime_cell<MODE>(g, i, row, col, z[i], bid, z, isb, zp, keyb, res);
-        lv[r] = i;
+        if (i >= 0) {  // marked saddle-node candidates (bit 31) are second copies
+          int row, col;
+          rc_of(g, i, row, col);
+          lt[r] = zprime_cell<MODE>(g, i, row, col, z[i], bid, z, isb, zp, keyb, res);
+          lv[r] = i;
+        }
       }
       saddle_offer(t, keyb, res, saddle);
     }
@@ -929,14 +976,19 @@ __device__ __forceinline__ void saddle_node_cell_packed(const GridD& g, int i, i
 }
 
 // over this trip's border list
+// (cand_only: only the entries a tiled z' pass marked kSnCand, else the
+// unmarked ones)
 template <int MODE>
 __global__ __launch_bounds__(kBT) void k_saddle_node_list(GridD g, const int32_t* __restrict__ list,
                                                           const int64_t* nlist, const int32_t* __restrict__ bid,
                                                           const float* __restrict__ z, const float* __restrict__ zp,
-                                                          const int64_t* __restrict__ saddle, int32_t* snode) {
+                                                          const int64_t* __restrict__ saddle, int32_t* snode,
+                                                          int cand_only) {
   const int64_t n = *nlist;
   for (int64_t e = (int64_t)blockIdx.x * kBT + threadIdx.x; e < n; e += (int64_t)gridDim.x * kBT) {
-    const int i = list[e];
+    const uint32_t v = (uint32_t)list[e];
+    if (((v & kSnCand) != 0) != (cand_only != 0)) continue;
+    const int i = (int)(v & ~kSnCand);
     int row, col;
     rc_of(g, i, row, col);
     saddle_node_cell_packed<MODE>(g, i, row, col, bid[i], bid, z, zp, saddle, snode);
@@ -1135,9 +1187,10 @@ int saddlesort_active(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t*
                                                    (const int64_t*)NB[in], bid, z, isb, zp, saddle, BL[out], NB[out]));
   }
   if (g.N > (1 << 28)) return fail(PFF_EINVAL, "saddlesort_active: grids above 2^28 cells are not supported");
+  // after the tiled pass only the entries it marked can be saddle nodes
   PFF_MODE_DISPATCH(g.mode, hipLaunchKernelGGL(k_saddle_node_list<M>, lgz, bl, 0, s, g, (const int32_t*)BL[out],
                                                (const int64_t*)NB[out], bid, z, (const float*)zp,
-                                               (const int64_t*)saddle, snode));
+                                               (const int64_t*)saddle, snode, full ? 1 : 0));
   *border_cur = out;
   uint32_t* wbits;
   uint8_t* tag;

@@ -134,6 +134,53 @@ __device__ __forceinline__ void block_append_wave_major(const int32_t (&val)[R],
   __syncthreads();
 }
 
+// block_append_wave_major of the values taken by `take`, followed in the same
+// list (one atomicAdd for both) by the values taken by `take2`, each with
+// `flag2` or'ed in (any order).  Must be reached by every thread.
+template <int R, int BT = kBT>
+__device__ __forceinline__ void block_append_wave_major2(const int32_t (&val)[R], const bool (&take)[R],
+                                                         const bool (&take2)[R], uint32_t flag2, int32_t* list,
+                                                         int64_t* count) {
+  constexpr int W = BT / 64;
+  __shared__ int off_s[R * W];
+  __shared__ int64_t base_s;
+  __shared__ int second_s;
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  const int lane = tid & 63, wid = tid >> 6;
+  uint64_t mask[R];
+  int mine2 = 0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    mask[r] = __ballot(take[r]);
+    if (lane == 0) off_s[wid * R + r] = __popcll(mask[r]);
+    mine2 += take2[r] ? 1 : 0;
+  }
+  int total2;
+  const int off2 = block_exclusive_scan<BT>(mine2, &total2);  // synchronises
+  if (tid == 0) {
+    int acc = 0;
+    for (int k = 0; k < R * W; ++k) {
+      const int c = off_s[k];
+      off_s[k] = acc;
+      acc += c;
+    }
+    second_s = acc;
+    base_s = acc + total2 ? (int64_t)atomicAdd((unsigned long long*)count, (unsigned long long)(acc + total2)) : 0;
+  }
+  __syncthreads();
+  int64_t p2 = base_s + second_s + off2;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (take[r]) {
+      const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask[r] >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)mask[r], 0u));
+      list[base_s + off_s[wid * R + r] + below] = val[r];
+    }
+    if (take2[r]) list[p2++] = (int32_t)((uint32_t)val[r] | flag2);
+  }
+  __syncthreads();
+}
+
 // Compact the taken values of every thread into this block's own segment
 // (seg, room for every item of the block) and store the count: no global
 // atomic.  Appends of thousands of blocks to one shared counter serialise at
