@@ -1151,8 +1151,10 @@ int saddlesort_active(pff_ctx* ctx, const GridD& g, const int32_t* bid, uint8_t*
                       const int64_t* npits, int* border_cur) {
   void *inv, *bl0, *bl1;
   PFF_TRY(scratch(ctx, S_B0, (size_t)g.N, &inv));
-  PFF_TRY(scratch(ctx, S_BL0, (size_t)g.N * 4, &bl0));
-  PFF_TRY(scratch(ctx, S_BL1, (size_t)g.N * 4, &bl1));
+  // a tiled pass lists every border cell of a closed basin (at most N) and a
+  // marked copy of each saddle-node candidate among them: up to 2N entries
+  PFF_TRY(scratch(ctx, S_BL0, (size_t)g.N * 8, &bl0));
+  PFF_TRY(scratch(ctx, S_BL1, (size_t)g.N * 8, &bl1));
   int32_t* BL[2] = {(int32_t*)bl0, (int32_t*)bl1};
   static_assert(W_NBL1 == W_NBL0 + 1, "border list counters are adjacent words");
   int64_t* NB[2] = {ctx->dev_words + W_NBL0, ctx->dev_words + W_NBL1};

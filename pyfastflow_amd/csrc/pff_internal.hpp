@@ -117,7 +117,7 @@ enum Slot : int {
   S_BATCH,  // tile-canvas arrays of pff_batch_route (one block)
   S_BATCHT, // tile-canvas tables (per tile)
   S_M0, S_M1, S_TPIT2,  // basin-level relabel: label maps (by basin id) and the next pits list
-  S_BL0, S_BL1,         // border cells of closed basins, this trip's and the next trip's
+  S_BL0, S_BL1,         // border cells of closed basins (and marked candidate copies), this trip's and the next trip's; 2N entries
   S_TLIST,              // cells tagged by the tiled carve
   S_RCVP, S_MARK,       // receivers with the cycle cells made roots, cycle-cell flags
   S_WBITS,              // saddle nodes of the trip, one bit per cell in tile order (128 words per tile)
