@@ -1,0 +1,77 @@
+"""The saddle-node candidate rule of the tiled z' pass, pinned on the oracle's
+saddlesort (lakeflow.py:132-250).
+
+A saddle node's key (z'_i, basin across) is its basin's global minimum, which
+is at most the basin's minimum over any 64x64 tile and at least the node's own
+key; so a saddle node's z' equals the z' of its basin's minimum over its own
+tile.  The GPU pass lists every border cell of a closed basin and again, marked,
+the cells that pass this test, and the saddle-node pass reads only the marked
+ones: the rule must keep every saddle node the oracle finds.  The list holds
+at most twice the listed cells."""
+
+import numpy as np
+import pytest
+
+from pyfastflow_amd import synth
+
+
+def _tile_minima(zp, bid, listed, nx, ny):
+    """z' of each listed cell's basin minimum over its 64x64 tile."""
+    rows, cols = np.divmod(np.arange(nx * ny), nx)
+    tile = (rows // 64) * ((nx + 63) // 64) + cols // 64
+    key = tile.astype(np.int64) * (nx * ny + 1) + bid.astype(np.int64)
+    out = np.full(nx * ny, np.inf, np.float32)
+    sel = np.flatnonzero(listed)
+    k = key[sel]
+    uk, inv = np.unique(k, return_inverse=True)
+    mins = np.full(len(uk), np.inf, np.float32)
+    np.minimum.at(mins, inv, zp[sel])
+    out[sel] = mins[inv]
+    return out
+
+
+@pytest.mark.parametrize("ny,nx,mode,pits,seed", [(200, 264, 0, 25, 51), (192, 160, 1, 12, 52), (130, 300, 2, 30, 53)])
+def test_every_saddle_node_is_a_candidate(ora, ny, nx, mode, pits, seed):
+    z = synth.dem(ny, nx, 2.0, 200.0, seed, n_pits=pits, dx=10.0).ravel()
+    g = ora.Grid(nx, ny, 10.0, mode)
+    rcv, _ = ora.compute_receivers(g, z)
+    bid = ora.basin_identification(g, rcv)
+    s = ora.saddlesort(g, bid, z)
+    N = nx * ny
+    zp, isb, sn = s["z_prime"], s["is_border"].astype(bool), s["saddlenode"]
+    leave = np.array([ora.can_leave(g, i) for i in range(N)])
+    listed = isb & ~leave & (bid != 0) & (bid < N)
+    tmin = _tile_minima(zp, bid, listed, nx, ny)
+    cand = listed & (zp == tmin)
+    basins = np.flatnonzero(sn >= 0)
+    basins = basins[(basins > 0) & (basins < N)]
+    assert len(basins) > 3, "the DEM must hold closed basins with saddle nodes"
+    nodes = sn[basins]
+    assert np.all(listed[nodes]), "a saddle node is a listed border cell"
+    missing = nodes[~cand[nodes]]
+    assert len(missing) == 0, f"saddle nodes without the candidate mark: {missing[:10]}"
+    assert int(listed.sum()) + int(cand.sum()) <= 2 * N
+    # the rule is a filter: on these DEMs it keeps a minority of the listed cells
+    assert cand.sum() < listed.sum()
+
+
+def test_a_plausible_mistake_fails(ora):
+    """Comparing the full key's z' against the minimum over the whole grid
+    (instead of the tile) would also keep every saddle node, but taking the
+    cell's own z only (not z') drops some: the test above can tell."""
+    ny, nx = 200, 264
+    z = synth.dem(ny, nx, 2.0, 200.0, 51, n_pits=25, dx=10.0).ravel()
+    g = ora.Grid(nx, ny, 10.0, 0)
+    rcv, _ = ora.compute_receivers(g, z)
+    bid = ora.basin_identification(g, rcv)
+    s = ora.saddlesort(g, bid, z)
+    N = nx * ny
+    zp, isb, sn = s["z_prime"], s["is_border"].astype(bool), s["saddlenode"]
+    leave = np.array([ora.can_leave(g, i) for i in range(N)])
+    listed = isb & ~leave & (bid != 0) & (bid < N)
+    zz = z.astype(np.float32)
+    tmin_z = _tile_minima(zz, bid, listed, nx, ny)
+    wrong = listed & (zz == tmin_z)
+    basins = np.flatnonzero(sn >= 0)
+    basins = basins[(basins > 0) & (basins < N)]
+    assert np.any(~wrong[sn[basins]])
