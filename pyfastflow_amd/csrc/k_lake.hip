@@ -523,8 +523,6 @@ constexpr int kCellRows = 16;
       const int i = row * g.nx + col;
 #define PFF_CELL_LOOP_END }
 
-static dim3 cell_grid(const GridD& g) { return dim3((g.nx + 63) / 64, (g.ny + kCellRows - 1) / kCellRows); }
-
 // every basin entry (lakeflow.py:164-167 initialises all of them): in the
 // first trip the pits touch almost every cache line of the three arrays, so
 // one streaming pass of whole lines is cheaper than the scattered per-pit
