@@ -506,23 +506,6 @@ int carve_from_levels(pff_ctx* ctx, const GridD& g, int32_t* rec, const int32_t*
 // ---------------------------------------------------------------------------
 constexpr int TSL = 64;  // tile side, as in the tiled contraction
 
-// 64x4 threads per block covering 64 columns x kCellRows rows (each thread
-// takes every fourth row), 4 blocks per 64x64 tile; a block whose tile is
-// inactive returns at once.  The loop body runs once per cell of the block;
-// `continue` leaves a cell.
-constexpr int kCellRows = 16;
-#define PFF_CELL_LOOP_BEGIN(tact, tiles_x)                                     \
-  int bx_, by_;                                                                \
-  xcd_block(bx_, by_);                                                         \
-  if (tact && tact[((by_ * kCellRows) / TSL) * (tiles_x) + bx_] == 0u) return; \
-  const int col = bx_ * 64 + threadIdx.x;                                      \
-  if (col < g.nx)                                                              \
-    for (int rr_ = threadIdx.y; rr_ < kCellRows; rr_ += 4) {                   \
-      const int row = by_ * kCellRows + rr_;                                   \
-      if (row >= g.ny) break;                                                  \
-      const int i = row * g.nx + col;
-#define PFF_CELL_LOOP_END }
-
 // every basin entry (lakeflow.py:164-167 initialises all of them): in the
 // first trip the pits touch almost every cache line of the three arrays, so
 // one streaming pass of whole lines is cheaper than the scattered per-pit
