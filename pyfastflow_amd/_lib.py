@@ -215,7 +215,7 @@ def ctx_for(device: torch.device | None = None):
         raise RuntimeError("pyfastflow_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
     dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
     idx = dev.index if dev.index is not None else torch.cuda.current_device()
-    lib = load()
+    load()
     c = _ctx.get(idx)
     stream = torch.cuda.current_stream(idx).cuda_stream
     if c is None:
@@ -225,7 +225,7 @@ def ctx_for(device: torch.device | None = None):
         c = h
         _ctx[idx] = c
     else:
-        lib.pff_ctx_set_stream(c, c_p(stream))
+        call("pff_ctx_set_stream", c, c_p(stream))
     return c
 
 
