@@ -76,8 +76,9 @@ def kernel_report(_lib, ctx, names):
 
 
 def fp64_drift(synth, iters=60, n=128):
-    """The f32 GraphFlood method against its FP64 restatement on a small
-    C3-style grid, cold start: error per result (max, p99, argmax) after
+    """The f32 GraphFlood method against its FP64 restatement on a 128^2
+    grid synthesised like the 2048^2 GraphFlood workload (beta 2.2, 300 m
+    relief, dx 10 m), cold start: error per result (max, p99, argmax) after
     `iters` iterations of the reference schedule.  Run on the CPU by the
     oracle; the GPU gives the same bits as the f32 oracle (checked by the
     caller), so this is also the GPU's error."""
