@@ -75,3 +75,26 @@ def test_a_plausible_mistake_fails(ora):
     basins = np.flatnonzero(sn >= 0)
     basins = basins[(basins > 0) & (basins < N)]
     assert np.any(~wrong[sn[basins]])
+
+
+def test_border_list_needs_more_than_n_entries_on_some_inputs(ora):
+    """The tiled pass appends every listed cell and a marked copy of each
+    candidate into one border list.  On white-noise terrain nearly every cell
+    is a border cell of a small closed basin and every basin has a candidate
+    in each tile it touches, so the list outgrows N entries: it is sized for
+    2N (each cell listed once, copied at most once)."""
+    ny, nx = 128, 192
+    rng = np.random.Generator(np.random.PCG64(61))
+    z = rng.uniform(0.0, 100.0, ny * nx).astype(np.float32)
+    g = ora.Grid(nx, ny, 10.0, 0)
+    rcv, _ = ora.compute_receivers(g, z)
+    bid = ora.basin_identification(g, rcv)
+    s = ora.saddlesort(g, bid, z)
+    N = nx * ny
+    zp, isb = s["z_prime"], s["is_border"].astype(bool)
+    leave = np.array([ora.can_leave(g, i) for i in range(N)])
+    listed = isb & ~leave & (bid != 0) & (bid < N)
+    cand = listed & (zp == _tile_minima(zp, bid, listed, nx, ny))
+    entries = int(listed.sum()) + int(cand.sum())
+    assert entries > N, f"{entries} entries for N = {N}"
+    assert entries <= 2 * N
