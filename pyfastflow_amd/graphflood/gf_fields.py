@@ -107,3 +107,41 @@ class Flooder:
 
     def get_h(self):
         return self.h.to_numpy().reshape(self.rshp)
+
+    # ---- checkpoint ------------------------------------------------------------
+    # No reference counterpart: the reference moves state by hand with
+    # set_z/get_Z and set_h/get_h (flowfields.py:202-214,310-319,
+    # gf_fields.py:143-159).  An iteration reads the surface zh (the router's
+    # z), h and the discharge Q of the previous one (the diffusion starts from
+    # it, gf_hydrodynamics.py:23-75); receivers, lakes and the fill are
+    # recomputed from zh, so these three arrays are the whole state.
+    _PARAMS = ("precipitation_rates", "manning", "edge_slope", "dt")
+
+    def save_state(self, path):
+        """Write zh, h, Q, the iteration count, the grid and the parameters to
+        an .npz file (numpy arrays only, no pickled objects)."""
+        r = self.router
+        extra = {k: np.float64(getattr(self, k)) for k in self._PARAMS}
+        np.savez(path, zh=r.z.to_numpy(), h=self.h.to_numpy(), Q=r.Q.to_numpy(), iterations=np.int64(self.iterations),
+                 nx=np.int64(self.nx), ny=np.int64(self.ny), dx=np.float64(self.dx), update=np.str_(self.update),
+                 **extra)
+
+    def load_state(self, path):
+        """Restore what save_state wrote on a Flooder of the same grid (nx, ny,
+        dx): zh into the router's z, h, Q and the iteration count, so that the
+        next iteration continues the saved run.  This Flooder keeps its own
+        parameters; the saved ones are returned (a dict).  With stochastic
+        receivers the draws continue from this process's seed counter."""
+        r = self.router
+        with np.load(path, allow_pickle=False) as f:
+            shape = (int(f["nx"]), int(f["ny"]), float(f["dx"]))
+            if shape != (self.nx, self.ny, self.dx):
+                raise ValueError(f"state of a {shape[0]} x {shape[1]} grid with dx = {shape[2]} cannot be loaded "
+                                 f"on a {self.nx} x {self.ny} grid with dx = {self.dx}")
+            r.z.from_numpy(f["zh"])
+            self.h.from_numpy(f["h"])
+            r.Q.from_numpy(f["Q"])
+            self.iterations = int(f["iterations"])
+            saved = {k: float(f[k]) for k in self._PARAMS}
+            saved["update"] = str(f["update"])
+        return saved

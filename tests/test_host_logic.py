@@ -6,6 +6,7 @@
   use (fill, copy_from, from_numpy, to_numpy, field[None]).
 - The stochastic receivers' per-call seed is splitmix64 (published vector).
 - The synthetic workloads are deterministic and have the benchmark's shapes.
+- A GraphFlood state saved to .npz loads back bit for bit, on the same grid only.
 """
 
 import numpy as np
@@ -148,3 +149,58 @@ def test_ragged_batch_workload_shape():
     assert all(t["nx"] == t["ny"] and t["z"].shape == (t["ny"], t["nx"]) for t in tiles)
     assert 128 <= min(sides) and max(sides) <= 1024
     assert sum(s * s for s in sides) == 66_888_835
+
+
+# ---- GraphFlood checkpoint ----------------------------------------------------------
+
+class _HostRouter:
+    """The router fields a Flooder's state lives in, on the CPU."""
+
+    def __init__(self, nx, ny, dx):
+        from pyfastflow_amd.field import Field, f32
+
+        self.nx, self.ny, self.dx = nx, ny, dx
+        self.rshp = (ny, nx)
+        self.device = torch.device("cpu")
+        self.z = Field(f32, (nx * ny,), device="cpu")
+        self.Q = Field(f32, (nx * ny,), device="cpu")
+
+
+def _flooder(monkeypatch, nx, ny, dx, **kw):
+    from pyfastflow_amd import constants as cte
+    from pyfastflow_amd.graphflood import Flooder
+
+    for k in ("PREC", "MANNING", "EDGESW"):  # the constructor sets these globals
+        monkeypatch.setattr(cte, k, getattr(cte, k))
+    return Flooder(_HostRouter(nx, ny, dx), **kw)
+
+
+def test_graphflood_state_round_trip(monkeypatch, tmp_path):
+    rng = np.random.Generator(np.random.PCG64(5))
+    a = _flooder(monkeypatch, 12, 7, 10.0, precipitation_rates=2e-5, manning=0.05, update="single")
+    zh, h, Q = (rng.uniform(0, 100, 84).astype(np.float32) for _ in range(3))
+    a.router.z.from_numpy(zh)
+    a.set_h(h)
+    a.router.Q.from_numpy(Q)
+    a.iterations = 1234
+    path = tmp_path / "state.npz"
+    a.save_state(path)
+
+    b = _flooder(monkeypatch, 12, 7, 10.0)  # other parameters: kept, the saved ones returned
+    saved = b.load_state(path)
+    assert np.array_equal(b.router.z.to_numpy(), zh)
+    assert np.array_equal(b.get_h(), h.reshape(7, 12))
+    assert np.array_equal(b.router.Q.to_numpy(), Q)
+    assert b.iterations == 1234
+    assert saved == {"precipitation_rates": 2e-5, "manning": 0.05, "edge_slope": 1e-2, "dt": a.dt,
+                     "update": "single"}
+    assert b.manning == 0.033 and b.update == "reference"
+
+
+def test_graphflood_state_refuses_another_grid(monkeypatch, tmp_path):
+    a = _flooder(monkeypatch, 12, 7, 10.0)
+    path = tmp_path / "state.npz"
+    a.save_state(path)
+    for nx, ny, dx in ((7, 12, 10.0), (12, 7, 5.0)):
+        with pytest.raises(ValueError, match="cannot be loaded"):
+            _flooder(monkeypatch, nx, ny, dx).load_state(path)
