@@ -22,6 +22,31 @@
  * OpenMP is used only on passes whose result is independent of the order,
  * so OMP_NUM_THREADS changes the speed, never the result (except the f32
  * sum order of ora_accumulate_* via rcv2donor slots, which is run serial).
+ *
+ * How each part is pinned (tests/test_oracle_*.py, tests/test_golden_routing.py,
+ * never by re-typing the formula):
+ *   - grid topology: numpy-roll closed forms of the neighbour tables, the
+ *     custom codes' rules;
+ *   - receivers: a vectorised numpy argmax; stochastic receivers: splitmix64's
+ *     published outputs, the draws' moments, the closed-form probability that
+ *     the steeper of two links wins;
+ *   - accumulation: brute-force path walks, the outlet-flux invariant, the
+ *     reference's rake-and-compress schedule converging to the counts;
+ *   - labels: init(root) by path walking; bid0[f^(2^P-1)(i)] on receiver
+ *     graphs with cycles;
+ *   - saddlesort: the priority-flood spill elevation (Barnes et al. 2014) on a
+ *     bowl; every saddle node passes the tiled pass's candidate rule;
+ *   - carve and jump: the explicit reversal of the pit path; DEMs on which the
+ *     sentinel rule and the trip cap fire; a carve that closes a 3-cycle;
+ *   - packed keys: the reference's own vectors (archive/test_pack_unpack.py)
+ *     and an independent closed form;
+ *   - fill: the closed form on chains, strict descent of the filled surface;
+ *     diffusion: mass balance; Manning: its steady state, the double update,
+ *     f32 against f64; pow53 against the correctly rounded double power.
+ * Parity unpinned (only running Taichi could pin them): whether Taichi's
+ * fast math turns x/DX into x*(1/DX) (ora_receivers_div_ambiguity and
+ * ora_gradient_div_ambiguity count the cells it would change), Taichi's
+ * powf/sqrtf to the ulp, the stream of ti.random().
  */
 #include "pff_oracle.h"
 
