@@ -36,8 +36,9 @@
  *     graphs with cycles;
  *   - saddlesort: the priority-flood spill elevation (Barnes et al. 2014) on a
  *     bowl; every saddle node passes the tiled pass's candidate rule;
- *   - carve and jump: the explicit reversal of the pit path; DEMs on which the
- *     sentinel rule and the trip cap fire; a carve that closes a 3-cycle;
+ *   - carve and jump: the explicit reversal of the pit path; on receiver
+ *     graphs with loops, the tags are the forward closure of the saddle
+ *     nodes; DEMs on which the sentinel rule and the trip cap fire;
  *   - packed keys: the reference's own vectors (archive/test_pack_unpack.py)
  *     and an independent closed form;
  *   - fill: the closed form on chains, strict descent of the filled surface;
