@@ -12,7 +12,7 @@ import torch
 from .. import _lib
 from ._core import check, ctx_of, grid_from_constants, n_cells
 
-__all__ = ["fill_edges", "flow_out_nodes"]
+__all__ = ["fill_edges", "flow_out_nodes", "compile_neighbourer"]
 
 
 def fill_edges(edges):
@@ -27,3 +27,11 @@ def flow_out_nodes(outnodes):
     t = check("outnodes", outnodes, n_cells(), torch.uint8)
     g = grid_from_constants()
     _lib.call("pff_flow_out_nodes", ctx_of(t), g, _lib.ptr(t))
+
+
+def compile_neighbourer():
+    """Kept for callers of the reference's initialisation
+    (neighbourer_flat.py:595-605, environment.py:36), which binds the
+    boundary-mode dispatch into Taichi functions once per process.  Here the
+    mode is an argument of every kernel call, so there is nothing to bind."""
+    return None
