@@ -5,7 +5,8 @@ text with the ast module (nothing of the reference is imported or run).
     python tests/golden/gen_reference_api.py [/root/reference]
 
 Callable from Python in the reference: @ti.kernel functions, undecorated
-functions, classes and their public methods.  @ti.func functions are device
+functions, classes and their public methods, each with its parameter
+names in order.  @ti.func functions are device
 helpers that Taichi refuses to call from Python scope, so they are listed
 apart, as is the boundary-dispatch code that compile_neighbourer rebinds."""
 
@@ -32,15 +33,17 @@ def surface(root: str) -> dict:
                 if isinstance(n, ast.FunctionDef) and not n.name.startswith("_"):
                     decs = [ast.unparse(x) for x in n.decorator_list]
                     where = f"{pkg}/{fn}:{n.lineno}"
+                    args = [a.arg for a in n.args.args]
                     if "ti.kernel" in decs:
-                        kernels.append([n.name, where])
+                        kernels.append([n.name, where, args])
                     elif "ti.func" in decs:
-                        device.append([n.name, where])
+                        device.append([n.name, where, args])
                     else:
-                        functions.append([n.name, where])
+                        functions.append([n.name, where, args])
                 elif isinstance(n, ast.ClassDef):
-                    classes[n.name] = [m.name for m in n.body
-                                       if isinstance(m, ast.FunctionDef) and not m.name.startswith("_")]
+                    classes[n.name] = {m.name: [a.arg for a in m.args.args] for m in n.body
+                                       if isinstance(m, ast.FunctionDef)
+                                       and (not m.name.startswith("_") or m.name == "__init__")}
         out[pkg] = {"kernels": kernels, "functions": functions, "device_only": device, "classes": classes}
     return out
 

@@ -4,10 +4,13 @@ tests/golden/reference_api.json lists what pyfastflow.flow and
 pyfastflow.graphflood define (tests/golden/gen_reference_api.py reads the
 reference's source text; nothing of it is imported): the @ti.kernel
 functions and plain functions a caller can reach from Python, the classes and
-their public methods.  Every one of them must exist under the same name in
-pyfastflow_amd.flow / pyfastflow_amd.graphflood, except those listed below
-with the reason."""
+their public methods, with their parameter names.  Every one of them must
+exist under the same name in pyfastflow_amd.flow / pyfastflow_amd.graphflood,
+except those listed below with the reason, and take the reference's
+parameters by the same names in the same order (it may add optional ones
+after them)."""
 
+import inspect
 import json
 import os
 
@@ -31,7 +34,7 @@ NOT_EXPOSED = {
 
 def _names(pkg):
     v = API[pkg]
-    return sorted({n for n, _ in v["kernels"]} | {n for n, _ in v["functions"]} | set(v["classes"]))
+    return sorted({e[0] for e in v["kernels"]} | {e[0] for e in v["functions"]} | set(v["classes"]))
 
 
 def test_inventory_is_complete():
@@ -57,3 +60,20 @@ def test_every_public_method_exists(pkg):
 def test_exceptions_are_reference_names():
     every = set(_names("flow")) | set(_names("graphflood"))
     assert NOT_EXPOSED <= every
+
+
+def _params(fn):
+    return list(inspect.signature(fn).parameters)
+
+
+@pytest.mark.parametrize("pkg", ["flow", "graphflood"])
+def test_parameters_match_by_name_and_order(pkg):
+    mod = getattr(pf, pkg)
+    v = API[pkg]
+    for name, where, args in v["kernels"] + v["functions"]:
+        if name in NOT_EXPOSED:
+            continue
+        assert _params(getattr(mod, name))[:len(args)] == args, f"{name} ({where})"
+    for cls, methods in v["classes"].items():
+        for m, args in methods.items():
+            assert _params(getattr(getattr(mod, cls), m))[:len(args)] == args, f"{cls}.{m}"
