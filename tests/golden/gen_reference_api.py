@@ -1,5 +1,6 @@
 """Writes tests/golden/reference_api.json: the Python-callable surface of
-pyfastflow.flow and pyfastflow.graphflood, read from the reference's source
+pyfastflow.flow and pyfastflow.graphflood, and the names pyfastflow.constants
+defines, read from the reference's source
 text with the ast module (nothing of the reference is imported or run).
 
     python tests/golden/gen_reference_api.py [/root/reference]
@@ -45,6 +46,15 @@ def surface(root: str) -> dict:
                                        if isinstance(m, ast.FunctionDef)
                                        and (not m.name.startswith("_") or m.name == "__init__")}
         out[pkg] = {"kernels": kernels, "functions": functions, "device_only": device, "classes": classes}
+    # the module globals and functions of pyfastflow.constants
+    tree = ast.parse(open(os.path.join(root, "pyfastflow", "constants.py")).read())
+    consts = []
+    for n in tree.body:
+        if isinstance(n, ast.Assign):
+            consts += [t.id for t in n.targets if isinstance(t, ast.Name) and not t.id.startswith("_")]
+        elif isinstance(n, ast.FunctionDef) and not n.name.startswith("_"):
+            consts.append(n.name)
+    out["constants"] = consts
     return out
 
 

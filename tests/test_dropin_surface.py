@@ -77,3 +77,11 @@ def test_parameters_match_by_name_and_order(pkg):
     for cls, methods in v["classes"].items():
         for m, args in methods.items():
             assert _params(getattr(getattr(mod, cls), m))[:len(args)] == args, f"{cls}.{m}"
+
+
+def test_constants_module_names():
+    """pyfastflow.constants' globals and functions (constants.py:20-103)."""
+    from pyfastflow_amd import constants
+
+    assert len(API["constants"]) >= 10
+    assert [n for n in API["constants"] if not hasattr(constants, n)] == []
