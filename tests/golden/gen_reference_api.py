@@ -21,6 +21,44 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
+def _cells_per_grid_cell(expr: ast.AST) -> int:
+    """A field extent written as products of nx, ny and integers, divided by
+    nx*ny (arithmetic read from the tree, nothing evaluated)."""
+    if isinstance(expr, ast.Name) and expr.id in ("nx", "ny"):
+        return 1
+    if isinstance(expr, ast.Attribute) and ast.unparse(expr) in ("self.nx", "self.ny"):
+        return 1
+    if isinstance(expr, ast.Constant) and isinstance(expr.value, int):
+        return expr.value
+    if isinstance(expr, ast.BinOp) and isinstance(expr.op, ast.Mult):
+        return _cells_per_grid_cell(expr.left) * _cells_per_grid_cell(expr.right)
+    raise ValueError(f"unexpected field extent {ast.unparse(expr)}")
+
+
+def class_fields(path: str, cname: str) -> dict:
+    """Fields the class constructor creates: {name: [taichi dtype, elements
+    per grid cell, or 0 for a 0-d field]} from ti.field(...) and
+    fb.dense(ti.i, extent).place(self.name)."""
+    tree = ast.parse(open(path).read())
+    cls = next(n for n in tree.body if isinstance(n, ast.ClassDef) and n.name == cname)
+    init = next(m for m in cls.body if isinstance(m, ast.FunctionDef) and m.name == "__init__")
+    fields = {}
+    for node in ast.walk(init):
+        if (isinstance(node, ast.Assign) and len(node.targets) == 1 and isinstance(node.targets[0], ast.Attribute)
+                and isinstance(node.value, ast.Call) and ast.unparse(node.value.func) == "ti.field"):
+            shape0 = any(k.arg == "shape" for k in node.value.keywords)
+            fields[node.targets[0].attr] = [ast.unparse(node.value.args[0]), 0 if shape0 else None]
+    for node in ast.walk(init):
+        if (isinstance(node, ast.Call) and isinstance(node.func, ast.Attribute) and node.func.attr == "place"
+                and isinstance(node.func.value, ast.Call) and isinstance(node.func.value.func, ast.Attribute)
+                and node.func.value.func.attr == "dense"):
+            per = _cells_per_grid_cell(node.func.value.args[1])
+            for a in node.args:
+                if isinstance(a, ast.Attribute) and a.attr in fields:
+                    fields[a.attr][1] = per
+    return fields
+
+
 def surface(root: str) -> dict:
     out = {}
     for pkg in ("flow", "graphflood"):
@@ -46,6 +84,8 @@ def surface(root: str) -> dict:
                                        if isinstance(m, ast.FunctionDef)
                                        and (not m.name.startswith("_") or m.name == "__init__")}
         out[pkg] = {"kernels": kernels, "functions": functions, "device_only": device, "classes": classes}
+    for pkg, fn, cname in (("flow", "flowfields.py", "FlowRouter"), ("graphflood", "gf_fields.py", "Flooder")):
+        out[pkg]["fields"] = {cname: class_fields(os.path.join(root, "pyfastflow", pkg, fn), cname)}
     # the module globals and functions of pyfastflow.constants
     tree = ast.parse(open(os.path.join(root, "pyfastflow", "constants.py")).read())
     consts = []
@@ -60,6 +100,7 @@ def surface(root: str) -> dict:
 
 if __name__ == "__main__":
     root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+    api = surface(root)
     with open(os.path.join(HERE, "reference_api.json"), "w") as f:
-        json.dump(surface(root), f, indent=1)
+        json.dump(api, f, indent=1)
         f.write("\n")

@@ -85,3 +85,25 @@ def test_constants_module_names():
 
     assert len(API["constants"]) >= 10
     assert [n for n in API["constants"] if not hasattr(constants, n)] == []
+
+
+TI_DTYPES = {"ti.f32": "torch.float32", "ti.i32": "torch.int32", "ti.i64": "torch.int64",
+             "ti.u1": "torch.uint8", "ti.u8": "torch.uint8"}  # ti.u1 fields take one byte per element
+
+
+def test_router_and_flooder_fields_match_the_reference_layout(monkeypatch):
+    """FlowRouter and Flooder built on host storage (the constructors only
+    allocate) carry every field the reference's constructors create, with the
+    same element type and extent (1-D of N, 4N for the donor lists, 0-d for
+    the change flag), so fields are interchangeable with the reference's."""
+    for k in ("NX", "NY", "DX", "BOUND_MODE", "PREC", "MANNING", "EDGESW"):  # set by the constructors
+        monkeypatch.setattr(pf.constants, k, getattr(pf.constants, k))
+    nx, ny = 8, 6
+    router = pf.flow.FlowRouter(nx, ny, 1.0, device="cpu")
+    flooder = pf.graphflood.Flooder(router)
+    for obj, cls in ((router, "FlowRouter"), (flooder, "Flooder")):
+        pkg = "flow" if cls == "FlowRouter" else "graphflood"
+        for name, (dtype, per_cell) in API[pkg]["fields"][cls].items():
+            f = getattr(obj, name)
+            assert str(f.dtype) == TI_DTYPES[dtype], f"{cls}.{name}"
+            assert f.shape == (() if per_cell == 0 else (per_cell * nx * ny,)), f"{cls}.{name}"
