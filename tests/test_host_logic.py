@@ -7,7 +7,10 @@
 - The stochastic receivers' per-call seed is splitmix64 (published vector).
 - The synthetic workloads are deterministic and have the benchmark's shapes.
 - A GraphFlood state saved to .npz loads back bit for bit, on the same grid only.
+- bench.py refuses more than one GPU; smoke() refuses to run without one.
 """
+
+import os
 
 import numpy as np
 import pytest
@@ -204,3 +207,26 @@ def test_graphflood_state_refuses_another_grid(monkeypatch, tmp_path):
     for nx, ny, dx in ((7, 12, 10.0), (12, 7, 5.0)):
         with pytest.raises(ValueError, match="cannot be loaded"):
             _flooder(monkeypatch, nx, ny, dx).load_state(path)
+
+
+# ---- entry points of the repository -------------------------------------------------
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import subprocess
+    import sys
+
+    out = subprocess.run([sys.executable, "bench.py", "--gpus", "2"], cwd=ROOT, capture_output=True, text=True,
+                         timeout=300)
+    assert out.returncode != 0
+    assert "one GPU" in out.stdout + out.stderr
+
+
+@no_gpu
+def test_smoke_needs_a_gpu():
+    import __graft_entry__ as g
+
+    with pytest.raises(AssertionError, match="needs a GPU"):
+        g.smoke()
